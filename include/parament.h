@@ -1,0 +1,160 @@
+/*
+ * parament.h -- C ABI of the MI355X-native equiprop library (libparament.so).
+ *
+ * Every function, type and enum value of the reference header
+ * (parament-integrator/parament src/cuda/parament.h) is re-declared here with the
+ * same name, argument order and binary layout, so existing callers (the pyparament
+ * ctypes bindings, src/python/pyparament/parament/paramentlib.py:57-72, and C
+ * programs) link against this library unchanged.  The complex types are plain
+ * {float, float} / {double, double} structs, layout-identical to the reference's
+ * cuComplex / cuDoubleComplex.
+ *
+ * Entry points after the "Additions" marker have no counterpart in the reference;
+ * each comment says which caller they are for.
+ */
+#ifndef PARAMENT_H_
+#define PARAMENT_H_
+
+#ifdef __cplusplus
+extern "C" {
+#else
+#include <stdbool.h>
+#endif
+
+#if defined(__GNUC__)
+#define LIBSPEC __attribute__((visibility("default")))
+#else
+#define LIBSPEC
+#endif
+
+/* Layout-identical stand-ins for cuComplex / cuDoubleComplex (reference parament.h:52-55). */
+typedef struct cuComplex { float x, y; } cuComplex;
+typedef struct cuDoubleComplex { double x, y; } cuDoubleComplex;
+
+struct Parament_Context_f32;
+struct Parament_Context_f64;
+typedef struct Parament_Context_f32 Parament_Context_f32;
+typedef struct Parament_Context_f64 Parament_Context_f64;
+
+/* Quadrature rule (reference parament.h:64-69; same values). */
+typedef enum Parament_QuadratureSpec {
+    PARAMENT_QUADRATURE_NONE = 0x00000000,
+    PARAMENT_QUADRATURE_MIDPOINT = 0x01000000,
+    PARAMENT_QUADRATURE_SIMPSON = 0x02000000
+} Parament_QuadratureSpec;
+
+/* Error codes (reference parament.h:74-130; same values).  CUBLAS_INIT_FAILED and
+ * CUBLAS_FAILED keep their numbers; here they mean "device runtime set-up failed"
+ * and "a device kernel or copy failed". */
+typedef enum Parament_ErrorCode {
+    PARAMENT_STATUS_SUCCESS = 0,
+    PARAMENT_STATUS_HOST_ALLOC_FAILED = 10,
+    PARAMENT_STATUS_DEVICE_ALLOC_FAILED = 20,
+    PARAMENT_STATUS_CUBLAS_INIT_FAILED = 30,
+    PARAMENT_STATUS_INVALID_VALUE = 50,
+    PARAMENT_STATUS_CUBLAS_FAILED = 60,
+    PARAMENT_STATUS_SELECT_SMALLER_DT = 70,
+    PARAMENT_STATUS_NO_HAMILTONIAN = 80,
+    PARAMENT_STATUS_INVALID_QUADRATURE_SELECTION = 90,
+    PARAMENT_FAIL = 1000
+} Parament_ErrorCode;
+
+/* ---- single precision (complex64) API: reference parament.h:155-357 ---- */
+
+/* Reference parament.h:155 / parament.cpp:51-128.  Allocates the host context only;
+ * device resources are bound to the device current at the first setHamiltonian. */
+LIBSPEC Parament_ErrorCode Parament_create(Parament_Context_f32 **handle_p);
+
+/* Reference parament.h:177 / parament.cpp:185-205.  NULL is a no-op. */
+LIBSPEC Parament_ErrorCode Parament_destroy(Parament_Context_f32 *handle);
+
+/* Reference parament.h:219-220 / parament.cpp:210-368.  H0 is dim x dim, H1 is
+ * amps consecutive dim x dim matrices, all row-major as passed by pyparament. */
+LIBSPEC Parament_ErrorCode Parament_setHamiltonian(Parament_Context_f32 *handle, cuComplex *H0, cuComplex *H1,
+    unsigned int dim, unsigned int amps, bool use_magnus, enum Parament_QuadratureSpec quadrature_mode);
+
+/* Reference parament.h:261-262 / parament.cpp:791-851.  carr holds amps arrays of
+ * pts samples (carr[k*pts + j]); out receives the dim x dim propagator.  Blocking. */
+LIBSPEC Parament_ErrorCode Parament_equiprop(Parament_Context_f32 *handle, cuComplex *carr, double dt,
+    unsigned int pts, unsigned int amps, cuComplex *out);
+
+/* Reference parament.h:286 / parament.cpp:723-739. */
+LIBSPEC int Parament_selectIterationCycles_fp32(double H_norm, double dt);
+
+/* Reference parament.h:308 / parament.cpp:771-776.  cycles must be odd and >= 3
+ * (the reference's Clenshaw loop reads out of bounds otherwise); other values
+ * return PARAMENT_STATUS_INVALID_VALUE and leave the setting unchanged. */
+LIBSPEC Parament_ErrorCode Parament_setIterationCyclesManually(Parament_Context_f32 *handle, unsigned int cycles);
+
+/* Reference parament.h:327 / parament.cpp:781-786. */
+LIBSPEC Parament_ErrorCode Parament_automaticIterationCycles(Parament_Context_f32 *handle);
+
+/* Reference parament.h:347 / parament.cpp:854-857. */
+LIBSPEC Parament_ErrorCode Parament_peekAtLastError(Parament_Context_f32 *handle);
+
+/* Reference parament.h:357 / parament.cpp:859-882 (same strings). */
+LIBSPEC const char *Parament_errorMessage(Parament_ErrorCode errorCode);
+
+/* ---- double precision (complex128) API: reference parament.h:363-400 ---- */
+
+LIBSPEC Parament_ErrorCode Parament_create_fp64(Parament_Context_f64 **handle_p);
+LIBSPEC Parament_ErrorCode Parament_destroy_fp64(Parament_Context_f64 *handle);
+LIBSPEC Parament_ErrorCode Parament_setHamiltonian_fp64(Parament_Context_f64 *handle, cuDoubleComplex *H0,
+    cuDoubleComplex *H1, unsigned int dim, unsigned int amps, bool use_magnus, Parament_QuadratureSpec quadrature_mode);
+LIBSPEC Parament_ErrorCode Parament_equiprop_fp64(Parament_Context_f64 *handle, cuDoubleComplex *carr, double dt,
+    unsigned int pts, unsigned int amps, cuDoubleComplex *out);
+LIBSPEC Parament_ErrorCode Parament_setIterationCyclesManually_fp64(Parament_Context_f64 *handle, unsigned int cycles);
+LIBSPEC Parament_ErrorCode Parament_automaticIterationCycles_fp64(Parament_Context_f64 *handle);
+LIBSPEC Parament_ErrorCode Parament_peekAtLastError_fp64(Parament_Context_f64 *handle);
+LIBSPEC int Parament_selectIterationCycles_fp64(double H_norm, double dt);
+
+/* Reference deviceInfo.c:30-59: prints the visible devices to stdout. */
+LIBSPEC void device_info(void);
+
+/* Reference mathhelper.h:67-68: max row sum of moduli of a row-major dim x dim matrix. */
+LIBSPEC double OneNorm(cuComplex *mat, unsigned int dim);
+LIBSPEC double OneNorm_fp64(cuDoubleComplex *mat, unsigned int dim);
+
+/* ======================== Additions (no reference counterpart) ======================== */
+
+/* Arithmetic of a complex128 context's equiprop (for callers that want the fast
+ * mixed-precision path behind the fp64 interface):
+ *   PARAMENT_PRECISION_NATIVE: every step in FP64 (default; the reference algorithm).
+ *   PARAMENT_PRECISION_MIXED:  FP64 for Hnorm, M (fp32 table), the Bessel
+ *       coefficients, the slice Hamiltonians, the first-order Chebyshev terms and the
+ *       whole product tree; the higher-order Chebyshev remainder runs as bf16
+ *       three-way split products with FP32 accumulation on the matrix cores.
+ *       Available for dim <= 64; larger dims run NATIVE. */
+typedef enum Parament_PrecisionMode {
+    PARAMENT_PRECISION_NATIVE = 0,
+    PARAMENT_PRECISION_MIXED = 1
+} Parament_PrecisionMode;
+
+LIBSPEC Parament_ErrorCode Parament_setPrecisionMode_fp64(Parament_Context_f64 *handle, Parament_PrecisionMode mode);
+
+/* Chebyshev order M used by the last successful equiprop (-1 before any). */
+LIBSPEC int Parament_getIterationCycles(Parament_Context_f32 *handle);
+LIBSPEC int Parament_getIterationCycles_fp64(Parament_Context_f64 *handle);
+
+/* Device-resident variant of equiprop for pipelines and benchmarks that keep the
+ * controls and the result in device memory: carr_dev and out_dev are device
+ * pointers (same layouts as Parament_equiprop).  The call is asynchronous on the
+ * context's stream; Parament_synchronize waits for it. */
+LIBSPEC Parament_ErrorCode Parament_equipropDevice(Parament_Context_f32 *handle, const cuComplex *carr_dev, double dt,
+    unsigned int pts, unsigned int amps, cuComplex *out_dev);
+LIBSPEC Parament_ErrorCode Parament_equipropDevice_fp64(Parament_Context_f64 *handle, const cuDoubleComplex *carr_dev,
+    double dt, unsigned int pts, unsigned int amps, cuDoubleComplex *out_dev);
+LIBSPEC Parament_ErrorCode Parament_synchronize(Parament_Context_f32 *handle);
+LIBSPEC Parament_ErrorCode Parament_synchronize_fp64(Parament_Context_f64 *handle);
+
+/* Selects the kernel family for testing and benchmarking:
+ *   0 = automatic (fused on-chip kernels for dim <= 64, step-wise batched kernels above),
+ *   1 = always the step-wise batched kernels (the reference's launch structure). */
+LIBSPEC Parament_ErrorCode Parament_setKernelPath(Parament_Context_f32 *handle, int path);
+LIBSPEC Parament_ErrorCode Parament_setKernelPath_fp64(Parament_Context_f64 *handle, int path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* PARAMENT_H_ */

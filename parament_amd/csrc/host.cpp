@@ -1,0 +1,581 @@
+// Host orchestration and C ABI of libparament.so (MI355X-native equiprop).
+//
+// Mirrors the reference's context lifecycle and pipeline (parament.cpp:51-882):
+// create -> setHamiltonian -> equiprop* -> destroy, with the same error codes,
+// messages, iteration tables, Bessel coefficients and slice counts.  The device work is
+// done by the kernels in kernels.hip / mixed.hip on one HIP stream per context.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/parament.h"
+#include "kernels.hpp"
+
+using pa::cplx;
+
+namespace {
+
+template <class R> struct ApiCplx;
+template <> struct ApiCplx<float> { typedef cuComplex type; };
+template <> struct ApiCplx<double> { typedef cuDoubleComplex type; };
+
+// ----- host math (mathhelper.cpp:35-114, parament.cpp:723-766) -----
+
+int select_fp32(double h, double dt) {
+    static const double b[] = {0.032516793, 0.219062571, 0.619625593, 1.218059203, 1.979888284,
+                               2.873301187, 3.872963682, 4.959398466, 6.117657121, 7.336154907,
+                               8.605792444, 9.919320831, 11.27088616, 12.65570085};
+    const double x = h * dt;
+    for (int i = 0; i < 14; ++i)
+        if (x <= b[i]) return 3 + 2 * i;
+    return -1;
+}
+
+int select_fp64(double h, double dt) {
+    static const double b[] = {0.000213616, 0.00768149, 0.0501474, 0.162592, 0.368382, 0.676861, 1.08784,
+                               1.59605, 2.19402, 2.87366, 3.62716, 4.44725, 5.3274, 6.26179,
+                               7.2453, 8.27338, 9.34206, 10.4478, 11.5875, 12.7584};
+    const double x = h * dt;
+    for (int i = 0; i < 20; ++i)
+        if (x <= b[i]) return 3 + 2 * i;
+    return -1;
+}
+
+// Max over rows of the summed moduli; single precision takes each modulus in float and
+// sums in double, as the reference does (mathhelper.cpp:82-114).
+double one_norm(const cuComplex* m, unsigned d) {
+    double best = 0;
+    for (unsigned i = 0; i < d; ++i) {
+        double s = 0;
+        for (unsigned j = 0; j < d; ++j) s += (double)hypotf(m[(size_t)d * i + j].x, m[(size_t)d * i + j].y);
+        best = std::max(best, s);
+    }
+    return best;
+}
+double one_norm(const cuDoubleComplex* m, unsigned d) {
+    double best = 0;
+    for (unsigned i = 0; i < d; ++i) {
+        double s = 0;
+        for (unsigned j = 0; j < d; ++j) s += hypot(m[(size_t)d * i + j].x, m[(size_t)d * i + j].y);
+        best = std::max(best, s);
+    }
+    return best;
+}
+
+// a_k = (-i)^k J_k(x) in FP64 (glibc jn, as the reference).
+void bessel(int m, double x, double* re, double* im) {
+    for (int k = 0; k <= m; ++k) {
+        const double j = jn(k, x);
+        switch (k % 4) {
+            case 0: re[k] = j; im[k] = 0; break;
+            case 1: re[k] = 0; im[k] = -j; break;
+            case 2: re[k] = -j; im[k] = 0; break;
+            default: re[k] = 0; im[k] = j; break;
+        }
+    }
+}
+
+// ----- context -----
+
+template <class R> struct Context {
+    typedef cplx<R> C;
+    Parament_ErrorCode last_error = PARAMENT_STATUS_SUCCESS;
+    // options (defaults of parament.cpp:72-75)
+    int mmax = 11;
+    bool manual = false;
+    int last_m = -1;
+    int precision = PARAMENT_PRECISION_NATIVE;
+    int path = 0;
+    // Hamiltonian
+    bool has_h = false;
+    unsigned dim = 0, amps = 0;
+    bool magnus = false;
+    int quad = PARAMENT_QUADRATURE_SIMPSON;
+    double hnorm = 0;
+    int ncoef = 0;
+    // device
+    bool bound = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    C* d_ops = nullptr;
+    // working memory (grow-only, like parament.cpp:418)
+    C* d_c = nullptr; size_t cap_c = 0;
+    void* d_leaves = nullptr; size_t cap_leaves = 0;
+    void* d_nodes = nullptr; size_t cap_nodes = 0;
+    C* d_gen[3] = {nullptr, nullptr, nullptr}; size_t cap_gen = 0;
+    void* d_root = nullptr; size_t cap_root = 0;  // d x d result of the last equiprop
+};
+
+template <class R> void free_hamiltonian(Context<R>* h) {
+    if (h->d_ops) hipFree(h->d_ops);
+    h->d_ops = nullptr;
+    h->has_h = false;
+}
+
+template <class R> void free_working(Context<R>* h) {
+    hipFree(h->d_c); h->d_c = nullptr; h->cap_c = 0;
+    hipFree(h->d_leaves); h->d_leaves = nullptr; h->cap_leaves = 0;
+    hipFree(h->d_nodes); h->d_nodes = nullptr; h->cap_nodes = 0;
+    for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
+    h->cap_gen = 0;
+    hipFree(h->d_root); h->d_root = nullptr; h->cap_root = 0;
+}
+
+// Grow a device buffer to at least `bytes` (contents are not preserved).
+bool grow(void** p, size_t* cap, size_t bytes) {
+    if (*cap >= bytes && *p) return true;
+    if (*p) hipFree(*p);
+    *p = nullptr; *cap = 0;
+    if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; hipGetLastError(); return false; }
+    *cap = bytes;
+    return true;
+}
+
+template <class R> Parament_ErrorCode bind_device(Context<R>* h) {
+    if (h->bound) {
+        if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
+        return PARAMENT_STATUS_SUCCESS;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { hipGetLastError(); return PARAMENT_FAIL; }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipGetLastError();
+        return PARAMENT_STATUS_CUBLAS_INIT_FAILED;
+    }
+    h->device = dev;
+    h->bound = true;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+template <class R> Parament_ErrorCode create(Context<R>** hp) {
+    if (!hp) return PARAMENT_STATUS_INVALID_VALUE;
+    *hp = new (std::nothrow) Context<R>();
+    if (!*hp) return PARAMENT_STATUS_HOST_ALLOC_FAILED;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+template <class R> Parament_ErrorCode destroy(Context<R>* h) {
+    if (!h) return PARAMENT_STATUS_SUCCESS;
+    if (h->bound) {
+        hipSetDevice(h->device);
+        hipStreamSynchronize(h->stream);
+        free_hamiltonian(h);
+        free_working(h);
+        hipStreamDestroy(h->stream);
+    }
+    delete h;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+// Magnus commutators on the device: [H0, H_k] then [H_a, H_b] (a < b, lexicographic),
+// each as A B - B A with the generic GEMM (parament.cpp:289-359).
+template <class R> hipError_t magnus_commutators(Context<R>* h) {
+    typedef cplx<R> C;
+    const unsigned d = h->dim, A = h->amps;
+    const size_t dd = (size_t)d * d;
+    C* ops = h->d_ops;
+    const C one = pa::mk<R>(1, 0), mone = pa::mk<R>(-1, 0), zero = pa::mk<R>(0, 0);
+    auto comm = [&](const C* X, const C* Y, C* out) -> hipError_t {
+        hipError_t e = pa::launch_gemm<R>(X, 0, Y, 0, nullptr, 0, out, 0, one, zero, zero, d, 1, h->stream);
+        if (e != hipSuccess) return e;
+        return pa::launch_gemm<R>(Y, 0, X, 0, out, 0, out, 0, mone, one, zero, d, 1, h->stream);
+    };
+    for (unsigned k = 0; k < A; ++k) {
+        hipError_t e = comm(ops, ops + (1 + k) * dd, ops + (1 + A + k) * dd);
+        if (e != hipSuccess) return e;
+    }
+    unsigned slot = 0;
+    for (unsigned a = 0; a < A; ++a)
+        for (unsigned b = a + 1; b < A; ++b, ++slot) {
+            hipError_t e = comm(ops + (1 + a) * dd, ops + (1 + b) * dd, ops + (1 + 2 * A + slot) * dd);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+template <class R>
+Parament_ErrorCode set_hamiltonian(Context<R>* h, const typename ApiCplx<R>::type* H0, const typename ApiCplx<R>::type* H1,
+                                   unsigned dim, unsigned amps, bool use_magnus, int quad) {
+    if (!h) return PARAMENT_STATUS_INVALID_VALUE;
+    if (h->bound) hipSetDevice(h->device);
+    free_hamiltonian(h);  // the old Hamiltonian is dropped even when this call fails (parament.cpp:216)
+    if (use_magnus && quad != PARAMENT_QUADRATURE_SIMPSON) return h->last_error = PARAMENT_STATUS_INVALID_QUADRATURE_SELECTION;
+    if (quad != PARAMENT_QUADRATURE_NONE && quad != PARAMENT_QUADRATURE_MIDPOINT && quad != PARAMENT_QUADRATURE_SIMPSON)
+        return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    if (!H0 || !H1 || dim == 0 || amps == 0) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    const int ncoef = pa::n_coeffs((int)amps, use_magnus ? 1 : 0);
+    if (ncoef > pa::kMaxCoeffs) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+
+    Parament_ErrorCode err = bind_device(h);
+    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
+
+    const size_t dd = (size_t)dim * dim;
+    const size_t nops = 1 + (size_t)ncoef;
+    if (hipMalloc(&h->d_ops, nops * dd * sizeof(cplx<R>)) != hipSuccess) {
+        hipGetLastError();
+        h->d_ops = nullptr;
+        return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    if (hipMemcpyAsync(h->d_ops, H0, dd * sizeof(cplx<R>), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(h->d_ops + dd, H1, amps * dd * sizeof(cplx<R>), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+        free_hamiltonian(h);
+        return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    }
+    h->dim = dim;
+    h->amps = amps;
+    h->magnus = use_magnus;
+    h->quad = use_magnus ? PARAMENT_QUADRATURE_SIMPSON : quad;
+    h->ncoef = ncoef;
+    h->hnorm = one_norm(H0, dim);
+    for (unsigned k = 0; k < amps; ++k) h->hnorm += one_norm(H1 + k * dd, dim);
+    if (use_magnus && magnus_commutators(h) != hipSuccess) {
+        free_hamiltonian(h);
+        return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+        free_hamiltonian(h);
+        return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    }
+    if (!grow(&h->d_root, &h->cap_root, dd * sizeof(cplx<R>))) {
+        free_hamiltonian(h);
+        return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    h->has_h = true;
+    return h->last_error = PARAMENT_STATUS_SUCCESS;
+}
+
+// Slice count and slice length per mode (parament.cpp:800-802, 820-831).
+template <class R> void slices(const Context<R>* h, unsigned pts, double dt, int* N, double* tau) {
+    const bool doubled = h->magnus || h->quad == PARAMENT_QUADRATURE_SIMPSON;
+    *tau = doubled ? 2 * dt : dt;
+    if (doubled) *N = pts >= 1 ? (int)((pts - 1) / 2) : 0;
+    else if (h->quad == PARAMENT_QUADRATURE_MIDPOINT) *N = pts >= 1 ? (int)(pts - 1) : 0;
+    else *N = (int)pts;
+}
+
+// Runs the whole pipeline on the context's stream; carr_dev / out_dev are device pointers.
+template <class R>
+Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps, cplx<R>* out_dev) {
+    typedef cplx<R> C;
+    int N = 0;
+    double tau = 0;
+    slices(h, pts, dt, &N, &tau);
+    if (N <= 0) return PARAMENT_STATUS_INVALID_VALUE;
+
+    int M = h->mmax;
+    if (!h->manual) {
+        M = (sizeof(R) == 4 || h->precision == PARAMENT_PRECISION_MIXED) ? select_fp32(h->hnorm, tau)
+                                                                         : select_fp64(h->hnorm, tau);
+        if (M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
+    }
+    if (M >= pa::kMaxCheb) return PARAMENT_STATUS_INVALID_VALUE;
+    h->last_m = M;
+    double are[pa::kMaxCheb], aim[pa::kMaxCheb];
+    bessel(M, tau * h->hnorm, are, aim);
+    // Slices are carried as E_j = U_j - I: the last Clenshaw step adds (a_0 - 1) I, formed in FP64.
+    const double a0m1 = are[0] - 1.0;
+
+    const unsigned d = h->dim;
+    const size_t dd = (size_t)d * d;
+    const bool fused = (d <= 64) && h->path == 0;
+    const bool mixed = (sizeof(R) == 8) && h->precision == PARAMENT_PRECISION_MIXED && fused && pa::mixed_supported((int)d);
+
+    pa::Controls<R> ctl;
+    ctl.c = carr_dev; ctl.pts = (int)pts; ctl.amps_in = (int)amps; ctl.amps_set = (int)h->amps;
+    ctl.mode = h->quad; ctl.magnus = h->magnus ? 1 : 0; ctl.tau = (R)tau;
+
+    if (mixed) {
+        pa::MixedArgs ma;
+        ma.ops = reinterpret_cast<const double2*>(h->d_ops);
+        ma.ncoef = h->ncoef; ma.d = (int)d;
+        ma.ctl.c = reinterpret_cast<const double2*>(carr_dev); ma.ctl.pts = (int)pts; ma.ctl.amps_in = (int)amps;
+        ma.ctl.amps_set = (int)h->amps; ma.ctl.mode = h->quad; ma.ctl.magnus = h->magnus ? 1 : 0; ma.ctl.tau = tau;
+        ma.N = N; ma.M = M; ma.s = 2.0 / h->hnorm;
+        for (int k = 0; k <= M; ++k) { ma.are[k] = are[k]; ma.aim[k] = aim[k]; }
+        ma.are[0] = a0m1;
+        const size_t nodes = (size_t)pa::tree_scratch_nodes<double>(N);
+        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)N * dd * sizeof(float2)) ||
+            !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(double2)))
+            return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+        ma.out = reinterpret_cast<float2*>(h->d_leaves);
+        if (pa::launch_cheb_mixed(ma, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        double2* nd = reinterpret_cast<double2*>(h->d_nodes);
+        if (pa::launch_tree<double>(h->d_leaves, true, N, (int)d, nd, nd + nodes * dd, reinterpret_cast<double2*>(out_dev),
+                                    h->stream) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        return PARAMENT_STATUS_SUCCESS;
+    }
+
+    pa::ChebArgs<R> ca;
+    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = (int)d; ca.ctl = ctl; ca.N = N; ca.M = M;
+    ca.s = (R)(2.0 / h->hnorm);  // the reference casts s and a_k to the working precision (parament.cpp:583-584, 406)
+    for (int k = 0; k <= M; ++k) { ca.are[k] = (R)are[k]; ca.aim[k] = (R)aim[k]; }
+    const R a_m_top = (R)are[M];
+    ca.are[0] = (R)a0m1;
+
+    if (fused) {
+        const size_t nodes = (size_t)pa::tree_scratch_nodes<R>(N);
+        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)N * dd * sizeof(C)) ||
+            !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(C)))
+            return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+        ca.out = reinterpret_cast<C*>(h->d_leaves);
+        if (pa::launch_cheb_fused<R>(ca, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        C* nd = reinterpret_cast<C*>(h->d_nodes);
+        if (pa::launch_tree<R>(h->d_leaves, false, N, (int)d, nd, nd + nodes * dd, out_dev, h->stream) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        return PARAMENT_STATUS_SUCCESS;
+    }
+
+    // Step-wise path: X, then M-1 GEMM launches, then ceil(log2 N) tree levels.
+    const size_t gen_bytes = (size_t)N * dd * sizeof(C);
+    if (h->cap_gen < gen_bytes) {
+        for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
+        h->cap_gen = 0;
+        for (auto& p : h->d_gen)
+            if (hipMalloc(&p, gen_bytes) != hipSuccess) {
+                hipGetLastError();
+                for (auto& q : h->d_gen) { hipFree(q); q = nullptr; }
+                return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+            }
+        h->cap_gen = gen_bytes;
+    }
+    C* X = h->d_gen[0];
+    C* b1 = h->d_gen[1];
+    C* b2 = h->d_gen[2];
+    const C zero = pa::mk<R>(0, 0), one = pa::mk<R>(1, 0);
+    const C aM = pa::mk<R>(a_m_top, ca.aim[M]);
+    const C saM = pa::cmul(pa::mk<R>(ca.s, 0), aM);
+    if (pa::launch_assemble<R>(ca, X, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (pa::launch_fill<R>(nullptr, zero, aM, (int)d, N, b2, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (pa::launch_fill<R>(X, saM, pa::mk<R>(ca.are[M - 1], ca.aim[M - 1]), (int)d, N, b1, h->stream) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    for (int k = M - 2; k >= 0; --k) {
+        const C beta = pa::mk<R>(k == 0 ? R(-2) : R(-1), 0);
+        // b2 <- s X b1 + beta b2 + a_k I, then swap roles
+        if (pa::launch_gemm<R>(X, 1, b1, 1, b2, 1, b2, 1, pa::mk<R>(ca.s, 0), beta, pa::mk<R>(ca.are[k], ca.aim[k]),
+                               (int)d, N, h->stream) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        std::swap(b1, b2);
+    }
+    // b1 holds E_j = U_j - I for the N slices; reduce level by level (parament.cpp:675-714)
+    // on deviations from identity: E' = A + B + A B.
+    C* rd = b1;
+    C* wr = b2;
+    int r = N;
+    while (r > 1) {
+        const int half = r / 2;
+        if (pa::launch_gemm<R>(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, one, zero, zero, (int)d, half, h->stream,
+                               true) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        if (r % 2) {
+            if (hipMemcpyAsync(wr + half * dd, rd + (size_t)(r - 1) * dd, dd * sizeof(C), hipMemcpyDeviceToDevice,
+                               h->stream) != hipSuccess)
+                return PARAMENT_STATUS_CUBLAS_FAILED;
+        }
+        r = half + (r % 2);
+        std::swap(rd, wr);
+    }
+    if (pa::launch_fill<R>(rd, one, one, (int)d, 1, out_dev, h->stream) != hipSuccess)  // U = I + E
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+template <class R> Parament_ErrorCode ensure_common(Context<R>* h, unsigned amps) {
+    if (!h->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (amps > h->amps || amps == 0) return PARAMENT_STATUS_INVALID_VALUE;
+    if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+template <class R>
+Parament_ErrorCode equiprop(Context<R>* h, const typename ApiCplx<R>::type* carr, double dt, unsigned pts, unsigned amps,
+                            typename ApiCplx<R>::type* out) {
+    typedef cplx<R> C;
+    if (!h) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (!carr || !out) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    Parament_ErrorCode err = ensure_common(h, amps);
+    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
+    const size_t dd = (size_t)h->dim * h->dim;
+    const size_t cbytes = (size_t)amps * pts * sizeof(C);
+    if (!grow(reinterpret_cast<void**>(&h->d_c), &h->cap_c, std::max<size_t>(cbytes, sizeof(C))))
+        return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    if (hipMemcpyAsync(h->d_c, carr, cbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    err = run(h, h->d_c, dt, pts, amps, reinterpret_cast<C*>(h->d_root));
+    if (err == PARAMENT_STATUS_SUCCESS) {
+        if (hipMemcpyAsync(out, h->d_root, dd * sizeof(C), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess)
+            err = PARAMENT_STATUS_CUBLAS_FAILED;
+    } else {
+        hipStreamSynchronize(h->stream);
+    }
+    return h->last_error = err;
+}
+
+template <class R>
+Parament_ErrorCode equiprop_device(Context<R>* h, const typename ApiCplx<R>::type* carr_dev, double dt, unsigned pts,
+                                   unsigned amps, typename ApiCplx<R>::type* out_dev) {
+    if (!h) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (!carr_dev || !out_dev) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    Parament_ErrorCode err = ensure_common(h, amps);
+    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
+    err = run(h, reinterpret_cast<const cplx<R>*>(carr_dev), dt, pts, amps, reinterpret_cast<cplx<R>*>(out_dev));
+    return h->last_error = err;
+}
+
+}  // namespace
+
+// ======================================================================
+// Exported C ABI (parament.h).  Complex64 contexts are Context<float>, complex128
+// contexts Context<double> (reference parament.cpp:886-954).
+// ======================================================================
+#define CTX32(h) reinterpret_cast<Context<float>*>(h)
+#define CTX64(h) reinterpret_cast<Context<double>*>(h)
+
+extern "C" {
+
+Parament_ErrorCode Parament_create(Parament_Context_f32** handle_p) {
+    return create<float>(reinterpret_cast<Context<float>**>(handle_p));
+}
+Parament_ErrorCode Parament_destroy(Parament_Context_f32* handle) { return destroy<float>(CTX32(handle)); }
+Parament_ErrorCode Parament_setHamiltonian(Parament_Context_f32* handle, cuComplex* H0, cuComplex* H1, unsigned int dim,
+                                           unsigned int amps, bool use_magnus, Parament_QuadratureSpec quadrature_mode) {
+    return set_hamiltonian<float>(CTX32(handle), H0, H1, dim, amps, use_magnus, (int)quadrature_mode);
+}
+Parament_ErrorCode Parament_equiprop(Parament_Context_f32* handle, cuComplex* carr, double dt, unsigned int pts,
+                                     unsigned int amps, cuComplex* out) {
+    return equiprop<float>(CTX32(handle), carr, dt, pts, amps, out);
+}
+int Parament_selectIterationCycles_fp32(double H_norm, double dt) { return select_fp32(H_norm, dt); }
+Parament_ErrorCode Parament_setIterationCyclesManually(Parament_Context_f32* handle, unsigned int cycles) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    if (cycles < 3 || cycles % 2 == 0 || cycles >= (unsigned)pa::kMaxCheb) return CTX32(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    CTX32(handle)->mmax = (int)cycles;
+    CTX32(handle)->manual = true;
+    return CTX32(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_automaticIterationCycles(Parament_Context_f32* handle) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    CTX32(handle)->mmax = 11;
+    CTX32(handle)->manual = false;
+    return CTX32(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_peekAtLastError(Parament_Context_f32* handle) {
+    return handle ? CTX32(handle)->last_error : PARAMENT_STATUS_INVALID_VALUE;
+}
+
+const char* Parament_errorMessage(Parament_ErrorCode errorCode) {
+    switch (errorCode) {
+        case PARAMENT_STATUS_SUCCESS: return "Success";
+        case PARAMENT_STATUS_HOST_ALLOC_FAILED: return "Memory allocation on the host failed.";
+        case PARAMENT_STATUS_DEVICE_ALLOC_FAILED: return "Memory allocation on the device failed.";
+        case PARAMENT_STATUS_CUBLAS_INIT_FAILED: return "Failed to initialize the cuBLAS library.";
+        case PARAMENT_STATUS_INVALID_VALUE: return "Invalid value.";
+        case PARAMENT_STATUS_CUBLAS_FAILED: return "Failed to execute cuBLAS function.";
+        case PARAMENT_STATUS_SELECT_SMALLER_DT: return "Timestep too large";
+        case PARAMENT_STATUS_INVALID_QUADRATURE_SELECTION: return "Invalid quadrature selection.";
+        case PARAMENT_STATUS_NO_HAMILTONIAN: return "No hamiltonian set";
+        default: return "Unknown error code";
+    }
+}
+
+Parament_ErrorCode Parament_create_fp64(Parament_Context_f64** handle_p) {
+    return create<double>(reinterpret_cast<Context<double>**>(handle_p));
+}
+Parament_ErrorCode Parament_destroy_fp64(Parament_Context_f64* handle) { return destroy<double>(CTX64(handle)); }
+Parament_ErrorCode Parament_setHamiltonian_fp64(Parament_Context_f64* handle, cuDoubleComplex* H0, cuDoubleComplex* H1,
+                                                unsigned int dim, unsigned int amps, bool use_magnus,
+                                                Parament_QuadratureSpec quadrature_mode) {
+    return set_hamiltonian<double>(CTX64(handle), H0, H1, dim, amps, use_magnus, (int)quadrature_mode);
+}
+Parament_ErrorCode Parament_equiprop_fp64(Parament_Context_f64* handle, cuDoubleComplex* carr, double dt,
+                                          unsigned int pts, unsigned int amps, cuDoubleComplex* out) {
+    return equiprop<double>(CTX64(handle), carr, dt, pts, amps, out);
+}
+Parament_ErrorCode Parament_setIterationCyclesManually_fp64(Parament_Context_f64* handle, unsigned int cycles) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    if (cycles < 3 || cycles % 2 == 0 || cycles >= (unsigned)pa::kMaxCheb) return CTX64(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    CTX64(handle)->mmax = (int)cycles;
+    CTX64(handle)->manual = true;
+    return CTX64(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_automaticIterationCycles_fp64(Parament_Context_f64* handle) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    CTX64(handle)->mmax = 11;
+    CTX64(handle)->manual = false;
+    return CTX64(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_peekAtLastError_fp64(Parament_Context_f64* handle) {
+    return handle ? CTX64(handle)->last_error : PARAMENT_STATUS_INVALID_VALUE;
+}
+int Parament_selectIterationCycles_fp64(double H_norm, double dt) { return select_fp64(H_norm, dt); }
+
+double OneNorm(cuComplex* mat, unsigned int dim) { return mat ? one_norm(mat, dim) : 0.0; }
+double OneNorm_fp64(cuDoubleComplex* mat, unsigned int dim) { return mat ? one_norm(mat, dim) : 0.0; }
+
+// ---- additions ----
+Parament_ErrorCode Parament_setPrecisionMode_fp64(Parament_Context_f64* handle, Parament_PrecisionMode mode) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    if (mode != PARAMENT_PRECISION_NATIVE && mode != PARAMENT_PRECISION_MIXED)
+        return CTX64(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    CTX64(handle)->precision = (int)mode;
+    return CTX64(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+int Parament_getIterationCycles(Parament_Context_f32* handle) { return handle ? CTX32(handle)->last_m : -1; }
+int Parament_getIterationCycles_fp64(Parament_Context_f64* handle) { return handle ? CTX64(handle)->last_m : -1; }
+
+Parament_ErrorCode Parament_equipropDevice(Parament_Context_f32* handle, const cuComplex* carr_dev, double dt,
+                                           unsigned int pts, unsigned int amps, cuComplex* out_dev) {
+    return equiprop_device<float>(CTX32(handle), carr_dev, dt, pts, amps, out_dev);
+}
+Parament_ErrorCode Parament_equipropDevice_fp64(Parament_Context_f64* handle, const cuDoubleComplex* carr_dev, double dt,
+                                                unsigned int pts, unsigned int amps, cuDoubleComplex* out_dev) {
+    return equiprop_device<double>(CTX64(handle), carr_dev, dt, pts, amps, out_dev);
+}
+Parament_ErrorCode Parament_synchronize(Parament_Context_f32* handle) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!CTX32(handle)->bound) return PARAMENT_STATUS_SUCCESS;
+    return hipStreamSynchronize(CTX32(handle)->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+}
+Parament_ErrorCode Parament_synchronize_fp64(Parament_Context_f64* handle) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!CTX64(handle)->bound) return PARAMENT_STATUS_SUCCESS;
+    return hipStreamSynchronize(CTX64(handle)->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+}
+Parament_ErrorCode Parament_setKernelPath(Parament_Context_f32* handle, int path) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    if (path != 0 && path != 1) return CTX32(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    CTX32(handle)->path = path;
+    return CTX32(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_setKernelPath_fp64(Parament_Context_f64* handle, int path) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    if (path != 0 && path != 1) return CTX64(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    CTX64(handle)->path = path;
+    return CTX64(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+
+void device_info(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { hipGetLastError(); n = 0; }
+    printf("Number of devices: %d\n", n);
+    for (int i = 0; i < n; ++i) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, i) != hipSuccess) continue;
+        printf("Device %d: %s (%s)\n", i, p.name, p.gcnArchName);
+        printf("  Compute units: %d, clock %d kHz, wavefront %d\n", p.multiProcessorCount, p.clockRate, p.warpSize);
+        printf("  Global memory: %.1f GiB, LDS per block: %zu KiB, L2: %d KiB\n",
+               p.totalGlobalMem / 1073741824.0, p.sharedMemPerBlock / 1024, p.l2CacheSize / 1024);
+    }
+    fflush(stdout);
+}
+
+}  // extern "C"

@@ -1,0 +1,418 @@
+// Equiprop device kernels for gfx950 (MI355X): fused Chebyshev slice kernel,
+// single-launch pairwise product tree, and the generic step-wise kernels.
+#include "kernels.hpp"
+
+namespace pa {
+
+// LDS row padding (elements) of the A-operand and B-operand planes; chosen so the
+// 16x16x4 operand reads of a half-wave fall into distinct banks.
+template <class R> struct Pad;
+template <> struct Pad<float> { static constexpr int A = 2, B = 16; };
+template <> struct Pad<double> { static constexpr int A = 1, B = 8; };
+
+// Work split of one slice: D = 64 uses 4 waves (one 32x32 quadrant each); D = 32 and
+// D = 16 use a single wave owning the whole (32x32 or 16x16) matrix.
+template <int D> struct Shape {
+    static constexpr int G = (D == 64) ? 4 : 1;
+    static constexpr int T = (D == 16) ? 1 : 2;
+    static __device__ __forceinline__ int row0(int w) { return D == 64 ? 32 * (w >> 1) : 0; }
+    static __device__ __forceinline__ int col0(int w) { return D == 64 ? 32 * (w & 1) : 0; }
+};
+
+// ---------------------------------------------------------------------------------------
+// Fused slice kernel: for each slice j, assemble X_j = H0 + sum_t c'_tj O_t in LDS, run the
+// Clenshaw recurrence of parament.cpp:569-652 with all states on chip, and write the slice
+// propagator as its deviation from identity, E_j = U_j - I (the last step adds
+// (a_0 - 1) I instead of a_0 I), which keeps the rounding of the near-identity slices
+// relative to |E_j| instead of |U_j|.
+//   b_M = a_M I, b_{M-1} = s a_M X + a_{M-1} I      (elementwise: b_M is diagonal)
+//   b_k = s X b_{k+1} - b_{k+2} + a_k I, k = M-2..1  (one complex GEMM each)
+//   U   = s X b_1 - 2 b_2 + a_0 I
+// X stays in LDS (A operand), b_{k+1} is staged in LDS as the B operand, and b_{k+1},
+// b_{k+2} are kept in registers in the accumulator layout for the epilogue.
+template <class R, int D>
+__global__ __launch_bounds__(64 * Shape<D>::G) void cheb_fused_kernel(ChebArgs<R> a) {
+    constexpr int G = Shape<D>::G, T = Shape<D>::T;
+    constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
+    __shared__ R Xr[D * SA], Xi[D * SA], Br[D * SB], Bi[D * SB];
+    __shared__ cplx<R> coef[kMaxCoeffs];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int row0 = Shape<D>::row0(wave), col0 = Shape<D>::col0(wave);
+    const int d = a.d, dd = d * d, M = a.M;
+    const R s = a.s;
+
+    for (int j = blockIdx.x; j < a.N; j += gridDim.x) {
+        for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
+        __syncthreads();
+        for (int e = tid; e < D * D; e += 64 * G) {
+            const int i = e / D, k = e % D;
+            R xr = R(0), xi = R(0);
+            if (i < d && k < d) {
+                cplx<R> x = a.ops[i * d + k];
+                for (int t = 0; t < a.ncoef; ++t)
+                    x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + i * d + k]));
+                xr = x.x; xi = x.y;
+            }
+            Xr[i * SA + k] = xr; Xi[i * SA + k] = xi;
+        }
+        __syncthreads();
+
+        CTile<R, T, T> b1, b2, acc;
+        {
+            const cplx<R> aM = mk<R>(a.are[M], a.aim[M]);
+            const cplx<R> aM1 = mk<R>(a.are[M - 1], a.aim[M - 1]);
+            const cplx<R> saM = mk<R>(s * aM.x, s * aM.y);
+#pragma unroll
+            for (int m = 0; m < T; ++m)
+#pragma unroll
+                for (int n = 0; n < T; ++n)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                        const bool dg = (row == col);
+                        b2.re[m][n][v] = dg ? aM.x : R(0);
+                        b2.im[m][n][v] = dg ? aM.y : R(0);
+                        const cplx<R> t = cmul(saM, mk<R>(Xr[row * SA + col], Xi[row * SA + col]));
+                        b1.re[m][n][v] = t.x + (dg ? aM1.x : R(0));
+                        b1.im[m][n][v] = t.y + (dg ? aM1.y : R(0));
+                        Br[row * SB + col] = b1.re[m][n][v];
+                        Bi[row * SB + col] = b1.im[m][n][v];
+                    }
+        }
+        __syncthreads();
+
+        for (int k = M - 2; k >= 0; --k) {
+            wave_cgemm<R, T, T, D, SA, SB>(Xr, Xi, Br, Bi, row0, col0, acc);
+            const R beta = (k == 0) ? R(-2) : R(-1);
+            // k == 0: are[0] holds a_0 - 1 (rounded from FP64), so the result is E_j = U_j - I
+            const R akr = a.are[k], aki = a.aim[k];
+#pragma unroll
+            for (int m = 0; m < T; ++m)
+#pragma unroll
+                for (int n = 0; n < T; ++n)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const bool dg = acc_row<R>(row0, m, v) == acc_col(col0, n);
+                        const R nr = s * acc.re[m][n][v] + beta * b2.re[m][n][v] + (dg ? akr : R(0));
+                        const R ni = s * acc.im[m][n][v] + beta * b2.im[m][n][v] + (dg ? aki : R(0));
+                        b2.re[m][n][v] = b1.re[m][n][v];
+                        b2.im[m][n][v] = b1.im[m][n][v];
+                        b1.re[m][n][v] = nr;
+                        b1.im[m][n][v] = ni;
+                    }
+            if (k > 0) {
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < T; ++m)
+#pragma unroll
+                    for (int n = 0; n < T; ++n)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                            Br[row * SB + col] = b1.re[m][n][v];
+                            Bi[row * SB + col] = b1.im[m][n][v];
+                        }
+                __syncthreads();
+            }
+        }
+        cplx<R>* out = a.out + (size_t)j * dd;
+#pragma unroll
+        for (int m = 0; m < T; ++m)
+#pragma unroll
+            for (int n = 0; n < T; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                    if (row < d && col < d) out[row * d + col] = mk<R>(b1.re[m][n][v], b1.im[m][n][v]);
+                }
+        __syncthreads();
+    }
+}
+
+template <class R>
+hipError_t launch_cheb_fused(const ChebArgs<R>& a, hipStream_t st) {
+    if (a.d <= 0 || a.d > 64 || a.N <= 0 || a.M < 3 || a.M >= kMaxCheb || a.ncoef > kMaxCoeffs)
+        return hipErrorInvalidValue;
+    const int grid = a.N;
+    if (a.d <= 16) hipLaunchKernelGGL((cheb_fused_kernel<R, 16>), dim3(grid), dim3(64 * Shape<16>::G), 0, st, a);
+    else if (a.d <= 32) hipLaunchKernelGGL((cheb_fused_kernel<R, 32>), dim3(grid), dim3(64 * Shape<32>::G), 0, st, a);
+    else hipLaunchKernelGGL((cheb_fused_kernel<R, 64>), dim3(grid), dim3(64 * Shape<64>::G), 0, st, a);
+    return hipGetLastError();
+}
+template hipError_t launch_cheb_fused<float>(const ChebArgs<float>&, hipStream_t);
+template hipError_t launch_cheb_fused<double>(const ChebArgs<double>&, hipStream_t);
+
+// ---------------------------------------------------------------------------------------
+// Pairwise product tree (parament.cpp:657-718 pairing, later slice on the left) on
+// deviations from identity: (I + A)(I + B) = I + (A + B + A B), every node is stored as
+// E = node - I and the root is written as I + E.
+//
+// One launch per 2^LOGB-fold reduction: work-group w reduces nodes [w B, w B + B) of the
+// current level entirely on chip and writes node w of the level LOGB above.  Inside the
+// block the pairing is the reference's (binary-counter order, completed nodes kept in
+// registers).  A ragged last block is padded with E = 0 nodes: combining with a zero node
+// returns the other node bit for bit (0 * X = 0, X + 0 + 0 = X), which reproduces the
+// reference's odd-carry rule exactly.  Node (l, i) therefore covers slices
+// [i 2^l, min((i+1) 2^l, N)) as in the reference.  No work-group depends on another
+// inside a launch; kernel boundaries order the levels.
+template <class R, int D, int LOGB, class SrcT>
+__global__ __launch_bounds__(64 * Shape<D>::G) void tree_block_kernel(const SrcT* __restrict__ src, int n_in, int d,
+                                                                         cplx<R>* __restrict__ dst, int final_level) {
+    constexpr int G = Shape<D>::G, T = Shape<D>::T;
+    constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
+    constexpr int B = 1 << LOGB;
+    __shared__ R Ar[D * SA], Ai[D * SA], Br[D * SB], Bi[D * SB];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int row0 = Shape<D>::row0(wave), col0 = Shape<D>::col0(wave);
+    const int dd = d * d;
+    const long base = (long)blockIdx.x * B;
+
+    auto stage = [&](long j, R* Pr, R* Pi, int S) {
+        const bool live = j < n_in;
+        const SrcT* p = src + (size_t)(live ? j : 0) * dd;
+        for (int e = tid; e < D * D; e += 64 * G) {
+            const int i = e / D, k = e % D;
+            R xr = R(0), xi = R(0);
+            if (live && i < d && k < d) { xr = R(p[i * d + k].x); xi = R(p[i * d + k].y); }
+            Pr[i * S + k] = xr; Pi[i * S + k] = xi;
+        }
+    };
+    auto tile_to_lds = [&](const CTile<R, T, T>& t, R* Pr, R* Pi, int S) {
+#pragma unroll
+        for (int m = 0; m < T; ++m)
+#pragma unroll
+            for (int n = 0; n < T; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                    Pr[row * S + col] = t.re[m][n][v];
+                    Pi[row * S + col] = t.im[m][n][v];
+                }
+    };
+    // out = A + B + A B from the staged planes (A later, B earlier)
+    auto combine_staged = [&](CTile<R, T, T>& out) {
+        __syncthreads();
+        wave_cgemm<R, T, T, D, SA, SB>(Ar, Ai, Br, Bi, row0, col0, out);
+#pragma unroll
+        for (int m = 0; m < T; ++m)
+#pragma unroll
+            for (int n = 0; n < T; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                    out.re[m][n][v] += Ar[row * SA + col] + Br[row * SB + col];
+                    out.im[m][n][v] += Ai[row * SA + col] + Bi[row * SB + col];
+                }
+        __syncthreads();
+    };
+
+    CTile<R, T, T> st[LOGB > 1 ? LOGB : 1];  // st[l]: completed level-l node waiting for its sibling
+    CTile<R, T, T> cur;
+#pragma unroll
+    for (int p = 0; p < B / 2; ++p) {
+        stage(base + 2 * p + 1, Ar, Ai, SA);
+        stage(base + 2 * p, Br, Bi, SB);
+        combine_staged(cur);  // level-1 node p
+#pragma unroll
+        for (int l = 1; l < LOGB; ++l) {
+            if (((p >> (l - 1)) & 1) == 0) { st[l] = cur; break; }
+            tile_to_lds(cur, Ar, Ai, SA);
+            tile_to_lds(st[l], Br, Bi, SB);
+            combine_staged(cur);
+        }
+    }
+    cplx<R>* out = dst + (size_t)blockIdx.x * dd;
+    const R diag = final_level ? R(1) : R(0);
+#pragma unroll
+    for (int m = 0; m < T; ++m)
+#pragma unroll
+        for (int n = 0; n < T; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                if (row < d && col < d)
+                    out[row * d + col] = mk<R>(cur.re[m][n][v] + (row == col ? diag : R(0)), cur.im[m][n][v]);
+            }
+}
+
+template <class R> constexpr int tree_logb() { return sizeof(R) == 4 ? 3 : 2; }
+
+template <class R, int D, class SrcT>
+static void launch_tree_level(const SrcT* src, int n, int d, cplx<R>* dst, bool final_level, hipStream_t st) {
+    constexpr int LOGB = tree_logb<R>();
+    const int grid = (n + (1 << LOGB) - 1) >> LOGB;
+    hipLaunchKernelGGL((tree_block_kernel<R, D, LOGB, SrcT>), dim3(grid), dim3(64 * Shape<D>::G), 0, st, src, n, d,
+                       dst, final_level ? 1 : 0);
+}
+
+template <class R>
+int tree_scratch_nodes(int N) {
+    constexpr int B = 1 << tree_logb<R>();
+    return (N + B - 1) / B;
+}
+template int tree_scratch_nodes<float>(int);
+template int tree_scratch_nodes<double>(int);
+
+template <class R>
+hipError_t launch_tree(const void* leaves, bool leaves_are_c64, int N, int d, cplx<R>* scratch0, cplx<R>* scratch1,
+                       cplx<R>* root, hipStream_t st) {
+    if (d <= 0 || d > 64 || N <= 0) return hipErrorInvalidValue;
+    constexpr int B = 1 << tree_logb<R>();
+    auto level = [&](const auto* src, int n, cplx<R>* dst, bool fin) {
+        if (d <= 16) launch_tree_level<R, 16>(src, n, d, dst, fin, st);
+        else if (d <= 32) launch_tree_level<R, 32>(src, n, d, dst, fin, st);
+        else launch_tree_level<R, 64>(src, n, d, dst, fin, st);
+    };
+    int n = N;
+    cplx<R>* bufs[2] = {scratch0, scratch1};
+    int which = 0;
+    // first level reads the leaves
+    {
+        const int n_out = (n + B - 1) / B;
+        cplx<R>* dst = (n_out == 1) ? root : bufs[which];
+        if (leaves_are_c64) level(reinterpret_cast<const float2*>(leaves), n, dst, n_out == 1);
+        else level(reinterpret_cast<const cplx<R>*>(leaves), n, dst, n_out == 1);
+        n = n_out;
+    }
+    while (n > 1) {
+        const int n_out = (n + B - 1) / B;
+        cplx<R>* dst = (n_out == 1) ? root : bufs[which ^ 1];
+        level(static_cast<const cplx<R>*>(bufs[which]), n, dst, n_out == 1);
+        which ^= 1;
+        n = n_out;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_tree<float>(const void*, bool, int, int, float2*, float2*, float2*, hipStream_t);
+template hipError_t launch_tree<double>(const void*, bool, int, int, double2*, double2*, double2*, hipStream_t);
+
+// ---------------------------------------------------------------------------------------
+// Generic step-wise kernels (any dim): the reference's launch structure
+// (parament.cpp:486-718) with the diagonal add and the scaling fused into the GEMM.
+template <class R>
+__global__ __launch_bounds__(256) void assemble_kernel(ChebArgs<R> a, cplx<R>* __restrict__ X, int blocks_per) {
+    __shared__ cplx<R> coef[kMaxCoeffs];
+    const int j = blockIdx.x / blocks_per, b = blockIdx.x % blocks_per;
+    for (int t = threadIdx.x; t < a.ncoef; t += blockDim.x) coef[t] = slice_coefficient(a.ctl, t, j);
+    __syncthreads();
+    const int dd = a.d * a.d;
+    const int e = b * 256 + threadIdx.x;
+    if (e >= dd) return;
+    cplx<R> x = a.ops[e];
+    for (int t = 0; t < a.ncoef; ++t) x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + e]));
+    X[(size_t)j * dd + e] = x;
+}
+
+template <class R>
+hipError_t launch_assemble(const ChebArgs<R>& a, cplx<R>* X, hipStream_t st) {
+    if (a.ncoef > kMaxCoeffs) return hipErrorInvalidValue;
+    const int blocks_per = (a.d * a.d + 255) / 256;
+    hipLaunchKernelGGL((assemble_kernel<R>), dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, X, blocks_per);
+    return hipGetLastError();
+}
+template hipError_t launch_assemble<float>(const ChebArgs<float>&, float2*, hipStream_t);
+template hipError_t launch_assemble<double>(const ChebArgs<double>&, double2*, hipStream_t);
+
+template <class R>
+__global__ __launch_bounds__(256) void fill_kernel(const cplx<R>* __restrict__ X, cplx<R> alpha, cplx<R> diag, int d,
+                                                   long total, cplx<R>* __restrict__ out) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int dd = d * d;
+    const int e = (int)(idx % dd), i = e / d, k = e % d;
+    cplx<R> v = mk<R>(0, 0);
+    if (X) v = cmul(alpha, X[idx]);
+    if (i == k) v = cadd(v, diag);
+    out[idx] = v;
+}
+
+template <class R>
+hipError_t launch_fill(const cplx<R>* X, cplx<R> alpha, cplx<R> diag, int d, int batch, cplx<R>* out,
+                       hipStream_t st) {
+    const long total = (long)batch * d * d;
+    hipLaunchKernelGGL((fill_kernel<R>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, X, alpha, diag, d,
+                       total, out);
+    return hipGetLastError();
+}
+template hipError_t launch_fill<float>(const float2*, float2, float2, int, int, float2*, hipStream_t);
+template hipError_t launch_fill<double>(const double2*, double2, double2, int, int, double2*, hipStream_t);
+
+// C_b = alpha A_b B_b + beta Cin_b + diag I (+ A_b + B_b) with 32x32 output tiles, 16-deep K tiles.
+template <class R>
+__global__ __launch_bounds__(256) void gemm_kernel(const cplx<R>* __restrict__ A, long sa, const cplx<R>* __restrict__ B,
+                                                   long sb, const cplx<R>* Cin, long sc, cplx<R>* Cout, long so,
+                                                   cplx<R> alpha, cplx<R> beta, cplx<R> diag, int d, int tiles,
+                                                   int add_ab) {
+    __shared__ cplx<R> As[32][17];
+    __shared__ cplx<R> Bs[16][33];
+    const int t2 = tiles * tiles;
+    const long b = blockIdx.x / t2;
+    const int t = blockIdx.x % t2, ti = t / tiles, tj = t % tiles;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const long dd = (long)d * d;
+    const cplx<R>* Ab = A + b * sa * dd;
+    const cplx<R>* Bb = B + b * sb * dd;
+    cplx<R> acc[2][2];
+    for (int p = 0; p < 2; ++p) for (int q = 0; q < 2; ++q) acc[p][q] = mk<R>(0, 0);
+    for (int k0 = 0; k0 < d; k0 += 16) {
+        for (int e = threadIdx.x; e < 512; e += 256) {
+            const int r = e >> 4, kk = e & 15;
+            const int gi = ti * 32 + r, gk = k0 + kk;
+            As[r][kk] = (gi < d && gk < d) ? Ab[(long)gi * d + gk] : mk<R>(0, 0);
+            const int kr = e >> 5, c = e & 31;
+            const int gk2 = k0 + kr, gj = tj * 32 + c;
+            Bs[kr][c] = (gk2 < d && gj < d) ? Bb[(long)gk2 * d + gj] : mk<R>(0, 0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            const cplx<R> a0 = As[ty][kk], a1 = As[ty + 16][kk];
+            const cplx<R> b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+            acc[0][0] = cadd(acc[0][0], cmul(a0, b0));
+            acc[0][1] = cadd(acc[0][1], cmul(a0, b1));
+            acc[1][0] = cadd(acc[1][0], cmul(a1, b0));
+            acc[1][1] = cadd(acc[1][1], cmul(a1, b1));
+        }
+        __syncthreads();
+    }
+    for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 2; ++q) {
+            const int gi = ti * 32 + ty + 16 * p, gj = tj * 32 + tx + 16 * q;
+            if (gi >= d || gj >= d) continue;
+            cplx<R> v = cmul(alpha, acc[p][q]);
+            if (Cin) v = cadd(v, cmul(beta, Cin[b * sc * dd + (long)gi * d + gj]));
+            if (add_ab) v = cadd(v, cadd(Ab[(long)gi * d + gj], Bb[(long)gi * d + gj]));
+            if (gi == gj) v = cadd(v, diag);
+            Cout[b * so * dd + (long)gi * d + gj] = v;
+        }
+}
+
+template <class R>
+hipError_t launch_gemm(const cplx<R>* A, long sa, const cplx<R>* B, long sb, const cplx<R>* Cin, long sc,
+                       cplx<R>* Cout, long so, cplx<R> alpha, cplx<R> beta, cplx<R> diag, int d, int batch,
+                       hipStream_t st, bool add_ab) {
+    const int tiles = (d + 31) / 32;
+    const long grid = (long)batch * tiles * tiles;
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL((gemm_kernel<R>), dim3((unsigned)grid), dim3(256), 0, st, A, sa, B, sb, Cin, sc, Cout, so,
+                       alpha, beta, diag, d, tiles, add_ab ? 1 : 0);
+    return hipGetLastError();
+}
+template hipError_t launch_gemm<float>(const float2*, long, const float2*, long, const float2*, long, float2*, long,
+                                       float2, float2, float2, int, int, hipStream_t, bool);
+template hipError_t launch_gemm<double>(const double2*, long, const double2*, long, const double2*, long, double2*,
+                                        long, double2, double2, double2, int, int, hipStream_t, bool);
+
+__global__ void zero_kernel(unsigned* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+template <class R>
+hipError_t launch_zero(unsigned* p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+template hipError_t launch_zero<float>(unsigned*, size_t, hipStream_t);
+
+}  // namespace pa

@@ -1,0 +1,62 @@
+// Launchers for the equiprop device kernels (defined in kernels.hip / mixed.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "device_common.hpp"
+
+namespace pa {
+
+constexpr int kMaxCheb = 48;        // Chebyshev coefficients carried in kernel arguments (M <= 41)
+constexpr int kMaxCoeffs = 256;     // operator coefficients per slice (controls + Magnus terms)
+
+template <class R> struct ChebArgs {
+    const cplx<R>* ops;    // (1 + ncoef) row-major d x d operators: H0, H_1.., Magnus terms
+    int ncoef;             // operators after H0
+    int d;
+    Controls<R> ctl;
+    int N;                 // slices
+    int M;                 // Chebyshev order (odd, >= 3)
+    R s;                   // 2 / Hnorm
+    R are[kMaxCheb], aim[kMaxCheb];   // a_k = (-i)^k J_k(tau Hnorm); are[0] holds a_0 - 1
+    cplx<R>* out;          // N x d x d, E_j = U_j - I
+};
+
+// Mixed precision: FP64 operators / coefficients; output E_j = U_j - I in complex64.
+struct MixedArgs {
+    const double2* ops;
+    int ncoef;
+    int d;
+    Controls<double> ctl;
+    int N;
+    int M;
+    double s;
+    double are[kMaxCheb], aim[kMaxCheb];   // are[0] holds a_0 - 1
+    float2* out;           // N x d x d, E_j = U_j - I
+};
+
+// Fused Chebyshev slice kernel, dims <= 64 (padded to 16 / 32 / 64).
+template <class R> hipError_t launch_cheb_fused(const ChebArgs<R>& a, hipStream_t st);
+hipError_t launch_cheb_mixed(const MixedArgs& a, hipStream_t st);
+bool mixed_supported(int d);
+
+// Pairwise product tree (dims <= 64) on deviations from identity, one launch per
+// block-reduction level.  leaves: N d x d matrices E_j = U_j - I, of cplx<R> or
+// (leaves_are_c64) complex64; scratch0 / scratch1: tree_scratch_nodes<R>(N) matrices each;
+// root receives the product U = I + E (not E).
+template <class R> int tree_scratch_nodes(int N);
+template <class R> hipError_t launch_tree(const void* leaves, bool leaves_are_c64, int N, int d, cplx<R>* scratch0,
+                                          cplx<R>* scratch1, cplx<R>* root, hipStream_t st);
+
+// Generic step-wise kernels (any dim).
+template <class R> hipError_t launch_assemble(const ChebArgs<R>& a, cplx<R>* X, hipStream_t st);
+// out_b = alpha * X_b + diag * I  (X may be null when alpha == 0)
+template <class R> hipError_t launch_fill(const cplx<R>* X, cplx<R> alpha, cplx<R> diag, int d, int batch,
+                                          cplx<R>* out, hipStream_t st);
+// C_b = alpha * A_b B_b + beta * Cin_b + diag * I (+ A_b + B_b when add_ab); strides in
+// matrices of d x d.
+template <class R> hipError_t launch_gemm(const cplx<R>* A, long sa, const cplx<R>* B, long sb,
+                                          const cplx<R>* Cin, long sc, cplx<R>* Cout, long so,
+                                          cplx<R> alpha, cplx<R> beta, cplx<R> diag, int d, int batch,
+                                          hipStream_t st, bool add_ab = false);
+template <class R> hipError_t launch_zero(unsigned* p, size_t n, hipStream_t st);
+
+}  // namespace pa

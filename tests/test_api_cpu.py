@@ -1,0 +1,136 @@
+"""C-ABI and host-logic tests that need no GPU: the library loads, exports every symbol
+declared in include/parament.h, and the error paths of the reference's test_error.py
+(which the host code resolves before any device work) behave the same."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import equiprop_oracle as eo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "parament.h")).read()
+    return sorted(set(re.findall(r"^LIBSPEC\s+[\w\s\*]*?\b(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from parament_amd import paramentlib
+    names = _declared_symbols()
+    assert len(names) >= 29
+    missing = [n for n in names if not hasattr(paramentlib.lib, n)]
+    assert not missing, missing
+
+
+def test_reference_symbols_are_declared():
+    # the 17 reference exports (parament.h:155-400) + device_info + OneNorm*
+    ref = ["Parament_create", "Parament_destroy", "Parament_setHamiltonian", "Parament_equiprop",
+           "Parament_selectIterationCycles_fp32", "Parament_setIterationCyclesManually",
+           "Parament_automaticIterationCycles", "Parament_peekAtLastError", "Parament_errorMessage",
+           "Parament_create_fp64", "Parament_destroy_fp64", "Parament_setHamiltonian_fp64",
+           "Parament_equiprop_fp64", "Parament_setIterationCyclesManually_fp64",
+           "Parament_automaticIterationCycles_fp64", "Parament_peekAtLastError_fp64",
+           "Parament_selectIterationCycles_fp64", "device_info", "OneNorm", "OneNorm_fp64"]
+    assert set(ref) <= set(_declared_symbols())
+
+
+def test_iteration_tables_agree_with_oracle():
+    from parament_amd import paramentlib
+    lib = paramentlib.lib
+    for x in np.r_[np.logspace(-6, 1.2, 300), [b for b, _ in eo.FP32_TABLE], [b for b, _ in eo.FP64_TABLE]]:
+        assert lib.Parament_selectIterationCycles_fp32(float(x), 1.0) == eo.select_iteration_cycles_fp32(x, 1.0)
+        assert lib.Parament_selectIterationCycles_fp64(float(x), 1.0) == eo.select_iteration_cycles_fp64(x, 1.0)
+
+
+def test_one_norm_agrees_with_oracle():
+    from parament_amd import paramentlib
+    rng = np.random.default_rng(0)
+    for d in (1, 3, 8):
+        m = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        assert abs(paramentlib.lib.OneNorm_fp64(np.ascontiguousarray(m.ravel()), d) - eo.one_norm(m)) < 1e-12
+        m32 = m.astype(np.complex64)
+        assert abs(paramentlib.lib.OneNorm(np.ascontiguousarray(m32.ravel()), d) - eo.one_norm(m32)) < 1e-5
+
+
+def test_error_messages_match_reference_strings():
+    from parament_amd import paramentlib
+    msg = lambda c: paramentlib.lib.Parament_errorMessage(c).decode()
+    assert msg(0) == "Success"
+    assert msg(50) == "Invalid value."
+    assert msg(70) == "Timestep too large"
+    assert msg(80) == "No hamiltonian set"
+    assert msg(90) == "Invalid quadrature selection."
+    assert msg(1000) == "Unknown error code"
+
+
+# ---- test_error.py of the reference (test_error.py:26-54); these resolve on the host ----
+
+def test_use_after_destruction():
+    import parament_amd as parament
+    context = parament.Parament()
+    context.destroy()
+    with pytest.raises(RuntimeError, match='Attempting to use a context that has been destroyed'):
+        context.set_hamiltonian(np.eye(2), np.eye(2))
+
+
+def test_propagate_no_hamiltonian():
+    import parament_amd as parament
+    with parament.Parament() as context:
+        with pytest.raises(RuntimeError, match='No hamiltonian set'):
+            context.equiprop(1.0)
+
+
+def test_invalid_quadrature_selection():
+    import parament_amd as parament
+    with parament.Parament() as context:
+        with pytest.raises(ValueError, match='Invalid quadrature selection'):
+            context.set_hamiltonian(np.eye(2), np.eye(2), use_magnus=True, quadrature_mode='none')
+        with pytest.raises(ValueError, match='Invalid quadrature selection'):
+            context.set_hamiltonian(np.eye(2), np.eye(2), use_magnus=True, quadrature_mode='midpoint')
+
+
+def test_unknown_quadrature_string():
+    import parament_amd as parament
+    with parament.Parament() as context:
+        with pytest.raises(ValueError, match='unknown quadrature mode selected'):
+            context.set_hamiltonian(np.eye(2), np.eye(2), quadrature_mode='trapezoid')
+
+
+def test_c_level_no_hamiltonian_code():
+    from parament_amd import paramentlib
+    lib = paramentlib.lib
+    h = ctypes.c_void_p()
+    assert lib.Parament_create_fp64(ctypes.byref(h)) == 0
+    out = np.zeros(4, np.complex128)
+    assert lib.Parament_equiprop_fp64(h, np.zeros(2, np.complex128), 1.0, 2, 1, out) == 80
+    assert lib.Parament_peekAtLastError_fp64(h) == 80
+    assert lib.Parament_destroy_fp64(h) == 0
+    assert lib.Parament_destroy_fp64(None) == 0
+
+
+def test_manual_cycles_validation():
+    import parament_amd as parament
+    with parament.Parament() as context:
+        for bad in (0, 1, 2, 4, 10, 49):
+            with pytest.raises(ValueError, match='Invalid value'):
+                context.set_iteration_cycles(bad)
+        context.set_iteration_cycles(9)
+        context.automatic_iteration_cycles()
+
+
+def test_precision_argument_checks():
+    import parament_amd as parament
+    with pytest.raises(ValueError):
+        parament.Parament('fp16')
+    with parament.Parament() as context:
+        with pytest.raises(ValueError):
+            context.set_precision_mode('mixed')
+    with parament.Parament('fp64') as context:
+        context.set_precision_mode('mixed')
+        context.set_precision_mode('native')
+        with pytest.raises(ValueError):
+            context.set_precision_mode('bf16')

@@ -1,0 +1,168 @@
+"""Parity of the HIP equiprop path with the oracle (FP64 restatement of the reference
+algorithm, same truncation M and quadrature) through the C ABI via the Parament class.
+
+Bar: relative Frobenius error <= 1e-5 for complex64 contexts and <= 1e-12 for complex128.
+The oracle for a complex64 run is evaluated on the complex64-rounded inputs, in FP64,
+with the FP32 iteration table."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import equiprop_oracle as eo
+from tests.helpers import as_c64_inputs, gue, random_hamiltonians, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-5, "fp64": 1e-12}
+
+
+def run_gpu(prec, h0, hks, c, dt, mode="none", magnus=False, path="auto", cycles=None, mixed=False):
+    import parament_amd as parament
+    with parament.Parament(prec) as p:
+        p.set_kernel_path(path)
+        if mixed:
+            p.set_precision_mode("mixed")
+        if cycles is not None:
+            p.set_iteration_cycles(cycles)
+        p.set_hamiltonian(h0, *hks, use_magnus=magnus, quadrature_mode=mode)
+        u = p.equiprop(dt, *c)
+        return u, p.iteration_cycles
+
+
+def oracle_for(prec, h0, hks, c, dt, mode="none", magnus=False, cycles=None):
+    if prec == "fp32":
+        h0, hks, c = as_c64_inputs(h0, hks, c)
+    return eo.equiprop(h0, hks, c, dt, mode=mode, use_magnus=magnus, precision=prec, cycles=cycles)
+
+
+def case(seed, d, amps, pts):
+    rng = np.random.default_rng(seed)
+    h = [gue(rng, d) for _ in range(amps + 1)]
+    c = rng.uniform(-1, 1, (amps, pts))
+    return h[0], h[1:], c
+
+
+# ---- the reference's own numerics tests (test_numerics.py), run through the HIP path ----
+
+@pytest.mark.parametrize("dim", [2, 16])
+def test_fp32_expm_scipy_random(dim):
+    h0, h1 = random_hamiltonians(dim)
+    u, _ = run_gpu("fp32", h0, [h1], np.zeros((1, 1)), 0.01)
+    assert np.sum(np.abs(u - scipy.linalg.expm(-1j * 0.01 * h0))) < np.finfo(np.float32).eps * dim * dim
+
+
+@pytest.mark.parametrize("dim", [2, 16])
+def test_fp64_expm_scipy_random(dim):
+    h0, h1 = random_hamiltonians(dim)
+    u, _ = run_gpu("fp64", h0, [h1], np.zeros((1, 1)), 0.01)
+    assert np.sum(np.abs(u - scipy.linalg.expm(-1j * 0.01 * h0))) < np.finfo(np.float64).eps * dim * dim
+
+
+@pytest.mark.parametrize("dim", [2, 4])
+def test_fp32_expm_debug(dim):
+    import parament_amd as parament
+    h0, _ = random_hamiltonians(dim)
+    u = parament.expm(h0)
+    assert np.sum(np.abs(u - scipy.linalg.expm(h0))) < np.finfo(np.float32).eps * dim * dim
+
+
+@pytest.mark.parametrize("dim", [2, 16])
+def test_fp32_multi_fields(dim, golden):
+    h0, h1 = random_hamiltonians(dim)
+    c = golden[f"multi_d{dim}_c"]
+    u, _ = run_gpu("fp32", h0, [h1, h0], c, 0.01)
+    assert np.linalg.norm(golden[f"multi_d{dim}_expected"] - u) < 1e-6
+
+
+def test_docstring_example(golden):
+    u, m = run_gpu("fp32", golden["doc_h0"], [golden["doc_h1"]], np.zeros((1, 1)), 1.0)
+    assert np.allclose(u, golden["doc_expected"], atol=1e-7)
+    assert m == eo.select_iteration_cycles_fp32(2.0, 1.0)  # Hnorm = |diag(1,-1)| + |sigma_x| = 2
+
+
+# ---- oracle parity across modes, dims, paths ----
+
+MODES = [("none", False), ("midpoint", False), ("simpson", False), ("simpson", True)]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("mode,magnus", MODES)
+@pytest.mark.parametrize("d", [2, 3, 8, 16, 17, 32, 33, 64])
+def test_parity_fused(prec, mode, magnus, d):
+    h0, hks, c = case(100 + d, d, 2, 65)
+    u, m = run_gpu(prec, h0, hks, c, 0.02, mode, magnus)
+    ref = oracle_for(prec, h0, hks, c, 0.02, mode, magnus)
+    assert rel_err(u, ref) < TOL[prec], (m,)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("mode,magnus", MODES)
+@pytest.mark.parametrize("d", [2, 8, 33, 72, 100])
+def test_parity_stepwise(prec, mode, magnus, d):
+    h0, hks, c = case(200 + d, d, 2, 33)
+    u, _ = run_gpu(prec, h0, hks, c, 0.02, mode, magnus, path="stepwise")
+    ref = oracle_for(prec, h0, hks, c, 0.02, mode, magnus)
+    assert rel_err(u, ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("mode,magnus", MODES)
+@pytest.mark.parametrize("pts", [1, 2, 3, 4, 5, 9, 10])
+def test_edge_pts(prec, mode, magnus, pts):
+    import parament_amd as parament
+    h0, hks, c = case(300 + pts, 4, 2, pts)
+    n = eo.slice_count(pts, eo._MODE_BY_NAME[mode], magnus)
+    if n == 0:
+        with pytest.raises(ValueError, match="Invalid value"):
+            run_gpu(prec, h0, hks, c, 0.05, mode, magnus)
+        return
+    u, _ = run_gpu(prec, h0, hks, c, 0.05, mode, magnus)
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 0.05, mode, magnus)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("cycles", [3, 5, 9, 21])
+def test_manual_cycles(prec, cycles):
+    h0, hks, c = case(400 + cycles, 8, 1, 40)
+    u, m = run_gpu(prec, h0, hks, c, 0.05, "midpoint", cycles=cycles)
+    assert m == cycles
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 0.05, "midpoint", cycles=cycles)) < TOL[prec]
+
+
+def test_fewer_amplitudes_than_hamiltonians():
+    h0, hks, c = case(500, 8, 3, 21)
+    u, _ = run_gpu("fp64", h0, hks, c[:1], 0.05, "simpson")
+    assert rel_err(u, eo.equiprop(h0, hks, c[:1], 0.05, mode="simpson")) < 1e-12
+
+
+def test_select_smaller_dt():
+    h0, hks, c = case(600, 4, 1, 8)
+    with pytest.raises(RuntimeError, match="Timestep too large"):
+        run_gpu("fp32", 30 * h0, [30 * hks[0]], c, 1.0)
+
+
+def test_context_reuse_grows_and_shrinks():
+    """Buffers grow with pts and are reused for smaller calls (parament.cpp:418 semantics)."""
+    import parament_amd as parament
+    h0, hks, c = case(700, 16, 2, 257)
+    with parament.Parament("fp64") as p:
+        p.set_hamiltonian(h0, *hks, quadrature_mode="simpson")
+        for pts in (17, 257, 33, 5):
+            u = p.equiprop(0.01, *c[:, :pts])
+            assert rel_err(u, eo.equiprop(h0, hks, c[:, :pts], 0.01, mode="simpson")) < 1e-12
+        # new Hamiltonian of a different dim on the same context
+        h0b, hksb, cb = case(701, 33, 1, 9)
+        p.set_hamiltonian(h0b, *hksb)
+        assert rel_err(p.equiprop(0.01, *cb), eo.equiprop(h0b, hksb, cb, 0.01)) < 1e-12
+
+
+# ---- reference-config-1 at full size: d = 8, 4096 steps, no quadrature ----
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_full_size_dim8_4096_steps(prec):
+    rng = np.random.default_rng(20260001)
+    h0, h1 = gue(rng, 8), gue(rng, 8)
+    c = rng.uniform(-1, 1, (1, 4096))
+    for path in ("auto", "stepwise"):
+        u, m = run_gpu(prec, h0, [h1], c, 1e-3, "none", path=path)
+        assert rel_err(u, oracle_for(prec, h0, [h1], c, 1e-3, "none")) < TOL[prec], (path, m)

@@ -1,0 +1,234 @@
+"""Pins the CPU oracle (oracle/equiprop_oracle.py) against the reference's known answers
+and against what the mathematics fixes.  CPU only."""
+import numpy as np
+import pytest
+import scipy.linalg
+from scipy.special import jv
+
+from oracle import equiprop_oracle as eo
+from oracle.expm_stepwise import expm_stepwise
+from tests.helpers import gue, random_hamiltonians, rel_err
+
+
+# ---------------- iteration tables (parament.cpp:723-766) ----------------
+
+@pytest.mark.parametrize("table,eps", [(eo.FP32_TABLE, np.finfo(np.float32).eps),
+                                       (eo.FP64_TABLE, np.finfo(np.float64).eps)])
+def test_tables_follow_truncation_rule(table, eps):
+    """Each threshold x_M is where the Chebyshev tail 2 sum_{k>M} |J_k(x)| reaches ~0.02-0.05 eps,
+    M runs over consecutive odd numbers from 3 and the thresholds increase: a wrong digit in
+    a threshold moves its tail out of the band."""
+    ms = [m for _, m in table]
+    assert ms == list(range(3, 3 + 2 * len(ms), 2))
+    xs = [x for x, _ in table]
+    assert all(a < b for a, b in zip(xs, xs[1:]))
+    for x, m in table:
+        tail = 2 * sum(abs(jv(k, x)) for k in range(m + 1, m + 60))
+        assert 0.015 * eps < tail < 0.06 * eps, (m, x, tail / eps)
+        prev = 2 * sum(abs(jv(k, x)) for k in range(m - 1, m + 60))
+        assert prev > 0.4 * eps  # M - 2 would not suffice
+
+
+def test_table_boundaries():
+    assert eo.select_iteration_cycles_fp32(1.0, 0.032516793) == 3
+    assert eo.select_iteration_cycles_fp32(1.0, 0.0325168) == 5
+    assert eo.select_iteration_cycles_fp32(2.0, 12.65570085 / 2) == 29
+    assert eo.select_iteration_cycles_fp32(1.0, 12.66) == -1
+    assert eo.select_iteration_cycles_fp64(1.0, 0.000213616) == 3
+    assert eo.select_iteration_cycles_fp64(1.0, 12.7584) == 41
+    assert eo.select_iteration_cycles_fp64(1.0, 12.76) == -1
+
+
+# ---------------- Bessel coefficients (mathhelper.cpp:35-80) ----------------
+
+@pytest.mark.parametrize("x", [1e-4, 0.0325, 0.3376, 1.0, 5.0, 12.7])
+def test_bessel_against_scipy_and_identities(x):
+    a = eo.bessel_coefficients(41, x)
+    k = np.arange(42)
+    assert np.allclose(np.abs(a), np.abs(jv(k, x)), rtol=1e-13, atol=1e-300)
+    # (-i)^k phase pattern
+    assert np.allclose(a[0::4].imag, 0) and np.allclose(a[1::4].real, 0)
+    # J_0 + 2 sum_{k>=1} J_{2k} = 1
+    j = jv(np.arange(80), x)
+    assert abs(j[0] + 2 * j[2::2].sum() - 1) < 1e-14
+    # generating function: exp(-i x cos t) = J_0 + 2 sum (-i)^k J_k cos(k t)
+    t = 0.7
+    series = eo.bessel_coefficients(60, x) * np.r_[1.0, 2 * np.cos(np.arange(1, 61) * t)]
+    assert abs(series.sum() - np.exp(-1j * x * np.cos(t))) < 1e-13
+
+
+# ---------------- Clenshaw (parament.cpp:569-652) ----------------
+
+def test_clenshaw_scalar_equals_chebyshev_series():
+    """On 1x1 'matrices' the recurrence must equal sum (2 - delta_k0) a_k T_k(y) exactly."""
+    hnorm, x = 2.5, 0.9
+    a = eo.bessel_coefficients(9, x)
+    ys = np.linspace(-1, 1, 11)
+    xs = (ys * hnorm).reshape(-1, 1, 1).astype(complex)
+    u = eo.clenshaw(xs, a, hnorm)[:, 0, 0]
+    t = np.cos(np.outer(np.arccos(ys), np.arange(10)))
+    direct = t @ (a * np.r_[1.0, 2 * np.ones(9)])
+    assert np.allclose(u, direct, atol=1e-14)
+    # and at this order the series is exp(-i x y) to truncation accuracy
+    assert np.allclose(u, np.exp(-1j * x * ys), atol=1e-8)
+
+
+@pytest.mark.parametrize("dim", [2, 4, 16])
+def test_clenshaw_matches_expm_reference_thresholds(dim):
+    """The reference's single-step test (test_numerics.py:32-97): sum |U - expm| < eps * d^2."""
+    h0, h1 = random_hamiltonians(dim)
+    for prec, eps in (("fp64", np.finfo(np.float64).eps), ("fp32", np.finfo(np.float32).eps)):
+        u = eo.equiprop(h0, [h1], np.zeros((1, 1)), 0.01, precision=prec)
+        assert np.abs(u - scipy.linalg.expm(-1j * 0.01 * h0)).sum() < eps * dim * dim
+
+
+def test_clenshaw_rejects_even_or_small_orders():
+    x = np.eye(2, dtype=complex)[None]
+    for m in (1, 2, 4):
+        with pytest.raises(ValueError):
+            eo.clenshaw(x, eo.bessel_coefficients(m, 0.1), 1.0)
+
+
+def test_debug_expm_trick(golden):
+    """parament.expm(m): H0 = 1j m, H1 = m, c = 0, dt = 1 gives exp(m) (debug_functions.py:22-31)."""
+    for dim in (2, 4):
+        h0 = golden[f"seed27_d{dim}_h0"]
+        u = eo.equiprop(1j * h0, [h0], np.zeros((1, 1)), 1.0, precision="fp32")
+        assert np.abs(u - golden[f"seed27_d{dim}_expm_debug"]).sum() < np.finfo(np.float32).eps * dim * dim
+
+
+def test_docstring_matrix(golden):
+    u = eo.equiprop(golden["doc_h0"], [golden["doc_h1"]], np.zeros((1, 1)), 1.0, precision="fp32")
+    assert np.allclose(u, golden["doc_expected"], atol=1e-8)
+
+
+# ---------------- product tree (parament.cpp:657-718) ----------------
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 13, 17, 33])
+def test_tree_equals_time_ordered_product(n):
+    rng = np.random.default_rng(n)
+    r = rng.normal(size=(n, 3, 3)) + 1j * rng.normal(size=(n, 3, 3))
+    seq = np.eye(3, dtype=complex)
+    for j in range(n):
+        seq = r[j] @ seq
+    assert rel_err(eo.pairwise_tree(r), seq) < 1e-12
+
+
+def test_tree_association_matches_aligned_blocks():
+    """Node (l, i) of the pairing covers slices [i 2^l, min((i+1) 2^l, N)): the tree of a
+    length-N list equals the tree of its aligned 8-blocks' trees, bit for bit."""
+    rng = np.random.default_rng(3)
+    n = 45
+    r = rng.normal(size=(n, 4, 4)) + 1j * rng.normal(size=(n, 4, 4))
+    blocks = np.stack([eo.pairwise_tree(r[i:i + 8]) for i in range(0, n, 8)])
+    assert np.array_equal(eo.pairwise_tree(blocks), eo.pairwise_tree(r))
+
+
+@pytest.mark.parametrize("dim", [2, 16])
+def test_time_ordering_reference_case(golden, dim):
+    """The reference's only ordering test (test_numerics.py:100-117), threshold 1e-6."""
+    h0, h1 = random_hamiltonians(dim)
+    c = golden[f"multi_d{dim}_c"]
+    u = eo.equiprop(h0, [h1, h0], c, 0.01, precision="fp32")
+    assert np.linalg.norm(u - golden[f"multi_d{dim}_expected"]) < 1e-6
+
+
+# ---------------- quadrature (control_expansion.cu:105-170) ----------------
+
+def test_slice_counts():
+    assert eo.slice_count(10, eo.QUAD_NONE) == 10
+    assert eo.slice_count(10, eo.QUAD_MIDPOINT) == 9
+    assert eo.slice_count(10, eo.QUAD_SIMPSON) == 4   # even pts drops the last sample
+    assert eo.slice_count(11, eo.QUAD_SIMPSON) == 5
+    assert eo.slice_count(2, eo.QUAD_SIMPSON) == 0
+    assert eo.slice_count(1, eo.QUAD_MIDPOINT) == 0
+
+
+def test_quadrature_closed_forms():
+    t = np.arange(9, dtype=float)
+    lin = np.stack([2 * t + 1, -t])
+    mid = eo.quadrature(lin, eo.QUAD_MIDPOINT)
+    assert np.allclose(mid, np.stack([2 * (t[:-1] + 0.5) + 1, -(t[:-1] + 0.5)]))
+    quad = np.stack([t ** 2])
+    simp = eo.quadrature(quad, eo.QUAD_SIMPSON)
+    # Simpson is exact for quadratics: mean over [2j, 2j+2] of t^2
+    exact = np.array([((2 * j + 2) ** 3 - (2 * j) ** 3) / 6 for j in range(4)])
+    assert np.allclose(simp[0], exact)
+    # even pts drops the last sample
+    assert eo.quadrature(np.ones((1, 10)), eo.QUAD_SIMPSON).shape == (1, 4)
+
+
+def test_simpson_constant_equals_none_at_twice_dt():
+    rng = np.random.default_rng(11)
+    h = [gue(rng, 4) for _ in range(3)]
+    c = np.array([[0.3], [-0.7]])
+    u_s = eo.equiprop(h[0], h[1:], np.repeat(c, 9, axis=1), 0.05, mode="simpson")
+    u_n = eo.equiprop(h[0], h[1:], np.repeat(c, 4, axis=1), 0.1, mode="none")
+    assert rel_err(u_s, u_n) < 1e-13
+
+
+# ---------------- Magnus (control_expansion.cu:27-60, parament.cpp:289-359) ----------------
+
+def test_pair_index_matches_reference_slots_for_three_controls():
+    # reference slot j + k - 1 for j < k (unique for amps <= 3)
+    for (j, k) in ((0, 1), (0, 2), (1, 2)):
+        assert eo.pair_index(j, k, 3) == j + k - 1
+    idx = sorted(eo.pair_index(j, k, 5) for j in range(5) for k in range(j + 1, 5))
+    assert idx == list(range(10))
+
+
+def test_magnus_commuting_case_reduces_to_simpson():
+    rng = np.random.default_rng(5)
+    q = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))[0]
+    diag = lambda: q @ np.diag(rng.normal(size=4)) @ q.conj().T
+    h = [diag() for _ in range(3)]
+    c = rng.uniform(-1, 1, (2, 17))
+    u_m = eo.equiprop(h[0], h[1:], c, 0.05, mode="simpson", use_magnus=True)
+    u_s = eo.equiprop(h[0], h[1:], c, 0.05, mode="simpson")
+    assert rel_err(u_m, u_s) < 1e-12
+
+
+def test_magnus_is_more_accurate_than_simpson():
+    rng = np.random.default_rng(1)
+    h0, h1, h2 = gue(rng, 4), gue(rng, 4), gue(rng, 4)
+    f = lambda t: np.stack([np.sin(3 * t + 0.3), np.cos(2 * t)])
+    dt, pts = 0.05, 41
+    c = f(np.arange(pts) * dt)
+    fine = 64
+    u = np.eye(4, dtype=complex)
+    for s in range((pts - 1) * fine):
+        cc = f((s + 0.5) * dt / fine)
+        u = scipy.linalg.expm(-1j * dt / fine * (h0 + cc[0] * h1 + cc[1] * h2)) @ u
+    e_s = np.linalg.norm(eo.equiprop(h0, [h1, h2], c, dt, mode="simpson") - u)
+    e_m = np.linalg.norm(eo.equiprop(h0, [h1, h2], c, dt, mode="simpson", use_magnus=True) - u)
+    assert e_m < 0.5 * e_s
+
+
+# ---------------- whole pipeline vs the stepwise expm oracle ----------------
+
+@pytest.mark.parametrize("mode,mag", [("none", False), ("midpoint", False), ("simpson", False), ("simpson", True)])
+def test_equiprop_matches_stepwise_expm(mode, mag):
+    rng = np.random.default_rng(2)
+    h = [gue(rng, 6) for _ in range(3)]
+    c = rng.uniform(-1, 1, (2, 33))
+    u = eo.equiprop(h[0], h[1:], c, 0.02, mode=mode, use_magnus=mag)
+    ref = expm_stepwise(h[0], h[1:], c, 0.02, mode=mode, use_magnus=mag)
+    assert rel_err(u, ref) < 1e-13
+
+
+def test_fewer_controls_than_hamiltonians_means_zero_amplitude():
+    rng = np.random.default_rng(4)
+    h = [gue(rng, 4) for _ in range(3)]
+    c = rng.uniform(-1, 1, (1, 9))
+    u1 = eo.equiprop(h[0], h[1:], c, 0.05, mode="simpson")
+    u2 = eo.equiprop(h[0], h[1:], np.vstack([c, np.zeros((1, 9))]), 0.05, mode="simpson")
+    assert np.array_equal(u1, u2)
+
+
+def test_golden_regression_vectors(golden):
+    hs, c = golden["mode_h"], golden["mode_c"]
+    for name, mode, mag in (("none", "none", False), ("midpoint", "midpoint", False),
+                            ("simpson", "simpson", False), ("magnus", "simpson", True)):
+        for prec in ("fp32", "fp64"):
+            u = eo.equiprop(hs[0], list(hs[1:]), c, 0.05, mode=mode, use_magnus=mag, precision=prec)
+            assert rel_err(u, golden[f"mode_{name}_{prec}"]) < 1e-14
