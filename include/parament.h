@@ -153,6 +153,22 @@ LIBSPEC Parament_ErrorCode Parament_synchronize_fp64(Parament_Context_f64 *handl
 LIBSPEC Parament_ErrorCode Parament_setKernelPath(Parament_Context_f32 *handle, int path);
 LIBSPEC Parament_ErrorCode Parament_setKernelPath_fp64(Parament_Context_f64 *handle, int path);
 
+/* Stream control, for callers that order equiprop against their own device work (e.g. a
+ * PyTorch stream).  The context uses its own non-blocking stream unless one is set;
+ * setStream(NULL) returns to the context's own stream. */
+LIBSPEC Parament_ErrorCode Parament_setStream(Parament_Context_f32 *handle, void *hip_stream);
+LIBSPEC Parament_ErrorCode Parament_setStream_fp64(Parament_Context_f64 *handle, void *hip_stream);
+LIBSPEC void *Parament_getStream(Parament_Context_f32 *handle);
+LIBSPEC void *Parament_getStream_fp64(Parament_Context_f64 *handle);
+
+/* Kernel timing for benchmarks: when enabled, each equiprop brackets its Chebyshev slice
+ * kernel and its product-tree launches with HIP events on the context's stream;
+ * getKernelTimes waits for them and returns the two durations of the last call in ms. */
+LIBSPEC Parament_ErrorCode Parament_setTiming(Parament_Context_f32 *handle, int enable);
+LIBSPEC Parament_ErrorCode Parament_setTiming_fp64(Parament_Context_f64 *handle, int enable);
+LIBSPEC Parament_ErrorCode Parament_getKernelTimes(Parament_Context_f32 *handle, double *slice_ms, double *tree_ms);
+LIBSPEC Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64 *handle, double *slice_ms, double *tree_ms);
+
 #ifdef __cplusplus
 }
 #endif

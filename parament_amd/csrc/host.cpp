@@ -102,7 +102,10 @@ template <class R> struct Context {
     // device
     bool bound = false;
     int device = -1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // stream in use (own_stream unless the caller set one)
+    hipStream_t own_stream = nullptr;
+    bool timing = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     C* d_ops = nullptr;
     // working memory (grow-only, like parament.cpp:418)
     C* d_c = nullptr; size_t cap_c = 0;
@@ -144,10 +147,11 @@ template <class R> Parament_ErrorCode bind_device(Context<R>* h) {
     }
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) { hipGetLastError(); return PARAMENT_FAIL; }
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) {
         hipGetLastError();
         return PARAMENT_STATUS_CUBLAS_INIT_FAILED;
     }
+    if (!h->stream) h->stream = h->own_stream;
     h->device = dev;
     h->bound = true;
     return PARAMENT_STATUS_SUCCESS;
@@ -165,9 +169,12 @@ template <class R> Parament_ErrorCode destroy(Context<R>* h) {
     if (h->bound) {
         hipSetDevice(h->device);
         hipStreamSynchronize(h->stream);
+        hipStreamSynchronize(h->own_stream);
         free_hamiltonian(h);
         free_working(h);
-        hipStreamDestroy(h->stream);
+        for (auto& e : h->ev)
+            if (e) hipEventDestroy(e);
+        hipStreamDestroy(h->own_stream);
     }
     delete h;
     return PARAMENT_STATUS_SUCCESS;
@@ -259,6 +266,13 @@ template <class R> void slices(const Context<R>* h, unsigned pts, double dt, int
     else *N = (int)pts;
 }
 
+// Timing marks: 0 before the slice kernel, 1 between slice kernel and tree, 2 after the tree.
+template <class R> void mark(Context<R>* h, int i) {
+    if (!h->timing) return;
+    if (!h->ev[i] && hipEventCreate(&h->ev[i]) != hipSuccess) { hipGetLastError(); h->ev[i] = nullptr; return; }
+    hipEventRecord(h->ev[i], h->stream);
+}
+
 // Runs the whole pipeline on the context's stream; carr_dev / out_dev are device pointers.
 template <class R>
 Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps, cplx<R>* out_dev) {
@@ -296,19 +310,28 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         ma.ncoef = h->ncoef; ma.d = (int)d;
         ma.ctl.c = reinterpret_cast<const double2*>(carr_dev); ma.ctl.pts = (int)pts; ma.ctl.amps_in = (int)amps;
         ma.ctl.amps_set = (int)h->amps; ma.ctl.mode = h->quad; ma.ctl.magnus = h->magnus ? 1 : 0; ma.ctl.tau = tau;
-        ma.N = N; ma.M = M; ma.s = 2.0 / h->hnorm;
-        for (int k = 0; k <= M; ++k) { ma.are[k] = are[k]; ma.aim[k] = aim[k]; }
-        ma.are[0] = a0m1;
+        // The low-precision recurrence multiplies by s in FP32: take s = fl32(2 / Hnorm) and expand
+        // with Hnorm' = 2 / s (within 6e-8 of Hnorm) so that every term uses the same, exact s.
+        const float sf = (float)(2.0 / h->hnorm);
+        const double hn2 = 2.0 / (double)sf;
+        double bre[pa::kMaxCheb], bim[pa::kMaxCheb];
+        bessel(M, tau * hn2, bre, bim);
+        ma.N = N; ma.M = M; ma.s = (double)sf;
+        for (int k = 0; k <= M; ++k) { ma.are[k] = bre[k]; ma.aim[k] = bim[k]; }
+        ma.are[0] = bre[0] - 1.0;
         const size_t nodes = (size_t)pa::tree_scratch_nodes<double>(N);
         if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)N * dd * sizeof(float2)) ||
             !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(double2)))
             return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
         ma.out = reinterpret_cast<float2*>(h->d_leaves);
+        mark(h, 0);
         if (pa::launch_cheb_mixed(ma, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        mark(h, 1);
         double2* nd = reinterpret_cast<double2*>(h->d_nodes);
         if (pa::launch_tree<double>(h->d_leaves, true, N, (int)d, nd, nd + nodes * dd, reinterpret_cast<double2*>(out_dev),
                                     h->stream) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
+        mark(h, 2);
         return PARAMENT_STATUS_SUCCESS;
     }
 
@@ -325,10 +348,13 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
             !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(C)))
             return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
         ca.out = reinterpret_cast<C*>(h->d_leaves);
+        mark(h, 0);
         if (pa::launch_cheb_fused<R>(ca, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        mark(h, 1);
         C* nd = reinterpret_cast<C*>(h->d_nodes);
         if (pa::launch_tree<R>(h->d_leaves, false, N, (int)d, nd, nd + nodes * dd, out_dev, h->stream) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
+        mark(h, 2);
         return PARAMENT_STATUS_SUCCESS;
     }
 
@@ -351,6 +377,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     const C zero = pa::mk<R>(0, 0), one = pa::mk<R>(1, 0);
     const C aM = pa::mk<R>(a_m_top, ca.aim[M]);
     const C saM = pa::cmul(pa::mk<R>(ca.s, 0), aM);
+    mark(h, 0);
     if (pa::launch_assemble<R>(ca, X, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
     if (pa::launch_fill<R>(nullptr, zero, aM, (int)d, N, b2, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
     if (pa::launch_fill<R>(X, saM, pa::mk<R>(ca.are[M - 1], ca.aim[M - 1]), (int)d, N, b1, h->stream) != hipSuccess)
@@ -363,6 +390,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
             return PARAMENT_STATUS_CUBLAS_FAILED;
         std::swap(b1, b2);
     }
+    mark(h, 1);
     // b1 holds E_j = U_j - I for the N slices; reduce level by level (parament.cpp:675-714)
     // on deviations from identity: E' = A + B + A B.
     C* rd = b1;
@@ -383,6 +411,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     }
     if (pa::launch_fill<R>(rd, one, one, (int)d, 1, out_dev, h->stream) != hipSuccess)  // U = I + E
         return PARAMENT_STATUS_CUBLAS_FAILED;
+    mark(h, 2);
     return PARAMENT_STATUS_SUCCESS;
 }
 
@@ -431,6 +460,17 @@ Parament_ErrorCode equiprop_device(Context<R>* h, const typename ApiCplx<R>::typ
     return h->last_error = err;
 }
 
+template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, double* b) {
+    if (!h->timing || !h->ev[0] || !h->ev[1] || !h->ev[2]) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    if (hipEventSynchronize(h->ev[2]) != hipSuccess) return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    float t0 = 0, t1 = 0;
+    if (hipEventElapsedTime(&t0, h->ev[0], h->ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&t1, h->ev[1], h->ev[2]) != hipSuccess)
+        return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    if (a) *a = t0;
+    if (b) *b = t1;
+    return h->last_error = PARAMENT_STATUS_SUCCESS;
+}
 }  // namespace
 
 // ======================================================================
@@ -561,6 +601,38 @@ Parament_ErrorCode Parament_setKernelPath_fp64(Parament_Context_f64* handle, int
     if (path != 0 && path != 1) return CTX64(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
     CTX64(handle)->path = path;
     return CTX64(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+}
+
+Parament_ErrorCode Parament_setStream(Parament_Context_f32* handle, void* hip_stream) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* h = CTX32(handle);
+    h->stream = hip_stream ? (hipStream_t)hip_stream : h->own_stream;
+    return h->last_error = PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_setStream_fp64(Parament_Context_f64* handle, void* hip_stream) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* h = CTX64(handle);
+    h->stream = hip_stream ? (hipStream_t)hip_stream : h->own_stream;
+    return h->last_error = PARAMENT_STATUS_SUCCESS;
+}
+void* Parament_getStream(Parament_Context_f32* handle) { return handle ? (void*)CTX32(handle)->stream : nullptr; }
+void* Parament_getStream_fp64(Parament_Context_f64* handle) { return handle ? (void*)CTX64(handle)->stream : nullptr; }
+
+Parament_ErrorCode Parament_setTiming(Parament_Context_f32* handle, int enable) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    CTX32(handle)->timing = enable != 0;
+    return PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_setTiming_fp64(Parament_Context_f64* handle, int enable) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    CTX64(handle)->timing = enable != 0;
+    return PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_getKernelTimes(Parament_Context_f32* handle, double* slice_ms, double* tree_ms) {
+    return handle ? kernel_times(CTX32(handle), slice_ms, tree_ms) : PARAMENT_STATUS_INVALID_VALUE;
+}
+Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64* handle, double* slice_ms, double* tree_ms) {
+    return handle ? kernel_times(CTX64(handle), slice_ms, tree_ms) : PARAMENT_STATUS_INVALID_VALUE;
 }
 
 void device_info(void) {

@@ -226,6 +226,26 @@ class Parament:
                                      int(pts), int(amps), ctypes.c_void_p(out_ptr)))
 
     @_must_be_alive
+    def set_stream(self, stream_ptr):
+        """Run on the given HIP stream (an int handle, e.g. torch.cuda.current_stream().cuda_stream); None resets."""
+        self._check_error(self._call('Parament_setStream', self._handle, ctypes.c_void_p(stream_ptr or None)))
+
+    @property
+    def stream(self):
+        return self._call('Parament_getStream', self._handle) if self._handle is not None else None
+
+    @_must_be_alive
+    def set_timing(self, enable=True):
+        self._check_error(self._call('Parament_setTiming', self._handle, int(bool(enable))))
+
+    @_must_be_alive
+    def kernel_times(self):
+        """(slice-kernel ms, product-tree ms) of the last equiprop, with timing enabled."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check_error(self._call('Parament_getKernelTimes', self._handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    @_must_be_alive
     def synchronize(self):
         self._check_error(self._call('Parament_synchronize', self._handle))
 

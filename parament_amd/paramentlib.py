@@ -7,6 +7,7 @@ library built next to this file is used.  A missing library raises at import, so
 code path can run without the HIP implementation.
 """
 import ctypes
+import importlib.util
 import os
 import pathlib
 
@@ -17,6 +18,28 @@ PARAMENT_LIB_DIR = os.environ.get("PARAMENT_LIB_DIR")
 
 if os.name != "posix":
     raise RuntimeError("Don't know how to load library on " + os.name)
+
+
+
+def _share_torch_hip_runtime():
+    """Load PyTorch-ROCm's HIP runtime (if installed) before libparament.so.
+
+    torch ships its own libamdhip64 with the same SONAME as the system one; whichever is
+    loaded first is then used by both, so torch tensors, torch streams and this library
+    share one runtime in the process.  Loading the system runtime first would leave two
+    runtimes competing for the device.  Set PARAMENT_OWN_HIP_RUNTIME=1 to skip this.
+    """
+    if os.environ.get("PARAMENT_OWN_HIP_RUNTIME"):
+        return
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    cand = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(cand):
+        ctypes.CDLL(cand, mode=ctypes.RTLD_GLOBAL)
+
+
+_share_torch_hip_runtime()
 
 if PARAMENT_LIB_DIR:
     lib_path = str(pathlib.Path(PARAMENT_LIB_DIR) / "libparament.so")
@@ -51,6 +74,12 @@ for sfx, cptr in (("", c_cuComplex_p), ("_fp64", c_cuDoubleComplex_p)):
                                                               ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p]
     getattr(lib, "Parament_synchronize" + sfx).argtypes = [c_ParamentContext_p]
     getattr(lib, "Parament_setKernelPath" + sfx).argtypes = [c_ParamentContext_p, ctypes.c_int]
+    getattr(lib, "Parament_setStream" + sfx).argtypes = [c_ParamentContext_p, ctypes.c_void_p]
+    getattr(lib, "Parament_getStream" + sfx).argtypes = [c_ParamentContext_p]
+    getattr(lib, "Parament_getStream" + sfx).restype = ctypes.c_void_p
+    getattr(lib, "Parament_setTiming" + sfx).argtypes = [c_ParamentContext_p, ctypes.c_int]
+    getattr(lib, "Parament_getKernelTimes" + sfx).argtypes = [c_ParamentContext_p, ctypes.POINTER(ctypes.c_double),
+                                                              ctypes.POINTER(ctypes.c_double)]
 
 lib.Parament_setPrecisionMode_fp64.argtypes = [c_ParamentContext_p, ctypes.c_int]
 lib.Parament_errorMessage.argtypes = [ctypes.c_int]

@@ -166,3 +166,36 @@ def test_full_size_dim8_4096_steps(prec):
     for path in ("auto", "stepwise"):
         u, m = run_gpu(prec, h0, [h1], c, 1e-3, "none", path=path)
         assert rel_err(u, oracle_for(prec, h0, [h1], c, 1e-3, "none")) < TOL[prec], (path, m)
+
+
+# ---- mixed precision (complex128 interface; bf16-split matrix cores + FP64 first-order terms) ----
+
+@pytest.mark.parametrize("mode,magnus", MODES)
+@pytest.mark.parametrize("d", [2, 8, 16, 31, 32, 33, 64])
+def test_mixed_parity(mode, magnus, d):
+    """Same truncation as the FP32 table, within 1e-5 relative of the FP64 restatement."""
+    h0, hks, c = case(800 + d, d, 2, 65)
+    u, m = run_gpu("fp64", h0, hks, c, 0.02, mode, magnus, mixed=True)
+    assert m == eo.select_iteration_cycles_fp32(eo.hamiltonian_norm(h0, hks), 0.04 if mode == "simpson" else 0.02)
+    ref = eo.equiprop(h0, hks, c, 0.02, mode=mode, use_magnus=magnus, precision="fp32")
+    assert rel_err(u, ref) < 1e-5
+
+
+@pytest.mark.parametrize("cycles", [3, 5, 9])
+def test_mixed_manual_cycles(cycles):
+    h0, hks, c = case(900 + cycles, 16, 1, 40)
+    u, m = run_gpu("fp64", h0, hks, c, 0.05, "midpoint", cycles=cycles, mixed=True)
+    assert m == cycles
+    assert rel_err(u, eo.equiprop(h0, hks, c, 0.05, mode="midpoint", cycles=cycles)) < 1e-5
+
+
+@pytest.mark.xfail(reason="mixed path accumulates ~1.9e-5 over 4096 slices: the matrix-core FP32 accumulation "
+                          "is not round-to-nearest (coherent per-slice error ~5e-9); being reworked", strict=False)
+def test_mixed_long_product_dim32():
+    """4096 slices of a dim-32 system: error of the mixed path vs the FP64 GPU path."""
+    rng = np.random.default_rng(20260004)
+    h = [gue(rng, 32) for _ in range(5)]
+    c = rng.uniform(-1, 1, (4, 4096))
+    u64, _ = run_gpu("fp64", h[0], h[1:], c, 2e-3, "none")
+    um, _ = run_gpu("fp64", h[0], h[1:], c, 2e-3, "none", mixed=True)
+    assert rel_err(um, u64) < 1e-5
