@@ -107,6 +107,7 @@ template <class R> struct Context {
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     C* d_ops = nullptr;
+    double* d_opmax = nullptr;  // largest modulus of each operator (mixed path's fixed-point scale)
     // working memory (grow-only, like parament.cpp:418)
     C* d_c = nullptr; size_t cap_c = 0;
     void* d_leaves = nullptr; size_t cap_leaves = 0;
@@ -117,7 +118,9 @@ template <class R> struct Context {
 
 template <class R> void free_hamiltonian(Context<R>* h) {
     if (h->d_ops) hipFree(h->d_ops);
+    if (h->d_opmax) hipFree(h->d_opmax);
     h->d_ops = nullptr;
+    h->d_opmax = nullptr;
     h->has_h = false;
 }
 
@@ -253,6 +256,25 @@ Parament_ErrorCode set_hamiltonian(Context<R>* h, const typename ApiCplx<R>::typ
         free_hamiltonian(h);
         return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
     }
+    // per-operator largest modulus (the Magnus commutators are read back once): bounds
+    // |Re X_ik|, |Im X_ik| <= |H0|max + sum_t |c'_t| |O_t|max for the mixed path's scaling
+    {
+        std::vector<cplx<R>> hostops(nops * dd);
+        std::vector<double> opmax(nops, 0.0);
+        if (hipMemcpy(hostops.data(), h->d_ops, nops * dd * sizeof(cplx<R>), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMalloc(&h->d_opmax, nops * sizeof(double)) != hipSuccess) {
+            hipGetLastError();
+            free_hamiltonian(h);
+            return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+        }
+        for (size_t t = 0; t < nops; ++t)
+            for (size_t e = 0; e < dd; ++e)
+                opmax[t] = std::max(opmax[t], hypot((double)hostops[t * dd + e].x, (double)hostops[t * dd + e].y));
+        if (hipMemcpy(h->d_opmax, opmax.data(), nops * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            free_hamiltonian(h);
+            return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+        }
+    }
     h->has_h = true;
     return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
@@ -307,6 +329,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     if (mixed) {
         pa::MixedArgs ma;
         ma.ops = reinterpret_cast<const double2*>(h->d_ops);
+        ma.opmax = h->d_opmax;
         ma.ncoef = h->ncoef; ma.d = (int)d;
         ma.ctl.c = reinterpret_cast<const double2*>(carr_dev); ma.ctl.pts = (int)pts; ma.ctl.amps_in = (int)amps;
         ma.ctl.amps_set = (int)h->amps; ma.ctl.mode = h->quad; ma.ctl.magnus = h->magnus ? 1 : 0; ma.ctl.tau = tau;

@@ -30,6 +30,7 @@ struct MixedArgs {
     int M;
     double s;
     double are[kMaxCheb], aim[kMaxCheb];   // are[0] holds a_0 - 1
+    const double* opmax;   // largest modulus of each operator (1 + ncoef entries)
     float2* out;           // N x d x d, E_j = U_j - I
 };
 

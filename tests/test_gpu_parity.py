@@ -189,8 +189,6 @@ def test_mixed_manual_cycles(cycles):
     assert rel_err(u, eo.equiprop(h0, hks, c, 0.05, mode="midpoint", cycles=cycles)) < 1e-5
 
 
-@pytest.mark.xfail(reason="mixed path accumulates ~1.9e-5 over 4096 slices: the matrix-core FP32 accumulation "
-                          "is not round-to-nearest (coherent per-slice error ~5e-9); being reworked", strict=False)
 def test_mixed_long_product_dim32():
     """4096 slices of a dim-32 system: error of the mixed path vs the FP64 GPU path."""
     rng = np.random.default_rng(20260004)
@@ -199,3 +197,17 @@ def test_mixed_long_product_dim32():
     u64, _ = run_gpu("fp64", h[0], h[1:], c, 2e-3, "none")
     um, _ = run_gpu("fp64", h[0], h[1:], c, 2e-3, "none", mixed=True)
     assert rel_err(um, u64) < 1e-5
+
+
+def test_full_size_heisenberg6_mixed_within_gate():
+    """The benchmark workload at full size (dim 64, 65,536 steps, Simpson): the mixed path is
+    within 1e-5 of the FP64 path, and both propagators are unitary to their precision."""
+    from parament_amd import workloads
+    h0, hk, c, dt, quad = workloads.heisenberg6()
+    u64, m64 = run_gpu("fp64", h0, hk, c, dt, quad)
+    um, mm = run_gpu("fp64", h0, hk, c, dt, quad, mixed=True)
+    assert (m64, mm) == (11, 7)
+    assert rel_err(um, u64) < 1e-5
+    eye = np.eye(64)
+    assert np.linalg.norm(u64.conj().T @ u64 - eye) < 1e-9
+    assert np.linalg.norm(um.conj().T @ um - eye) < 1e-4
