@@ -97,6 +97,24 @@ def select_iteration_cycles_fp64(hnorm: float, dt: float) -> int:
     return _select(FP64_TABLE, hnorm, dt)
 
 
+def chebyshev_tail(m: int, x: float) -> float:
+    """2 sum_{k>m} |J_k(x)|: spectral-norm bound of the truncation error of one slice."""
+    return 2.0 * sum(abs(jn(k, x)) for k in range(m + 1, m + 41))
+
+
+def select_iteration_cycles_mixed(hnorm: float, tau: float, n_slices: int) -> int:
+    """Order of this build's mixed-precision path (no reference counterpart): the FP32 table's
+    M, raised by 2 until n_slices * chebyshev_tail(M) <= 2e-6 (systematic truncation error
+    summed over the product)."""
+    m = select_iteration_cycles_fp32(hnorm, tau)
+    if m < 3:
+        return m
+    x = hnorm * tau
+    while m + 2 < 48 and n_slices * chebyshev_tail(m, x) > 2e-6:
+        m += 2
+    return m
+
+
 def one_norm(m: np.ndarray) -> float:
     """Max over rows of the summed moduli of a row-major matrix (mathhelper.cpp:82-114).
 

@@ -81,6 +81,21 @@ void bessel(int m, double x, double* re, double* im) {
     }
 }
 
+// Chebyshev order of the mixed-precision path.  The FP32 table bounds the per-slice tail
+// 2 sum_{k>M} |J_k(x)| at ~0.05 eps32; that truncation error is the same for every slice and
+// adds up over the product, so the mixed path (whose target is 1e-5 against FP64) also
+// requires N * tail(M) <= 2e-6, raising M by 2 until it holds (never below the FP32 choice).
+double cheb_tail(int m, double x) {
+    double t = 0;
+    for (int k = m + 1; k <= m + 40; ++k) t += fabs(jn(k, x));
+    return 2 * t;
+}
+int mixed_order(int m_fp32, double x, int n_slices) {
+    int m = m_fp32;
+    while (m + 2 < 48 && (double)n_slices * cheb_tail(m, x) > 2e-6) m += 2;
+    return m;
+}
+
 // ----- context -----
 
 template <class R> struct Context {
@@ -304,11 +319,12 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     slices(h, pts, dt, &N, &tau);
     if (N <= 0) return PARAMENT_STATUS_INVALID_VALUE;
 
+    const bool mixed_req = (sizeof(R) == 8) && h->precision == PARAMENT_PRECISION_MIXED;
     int M = h->mmax;
     if (!h->manual) {
-        M = (sizeof(R) == 4 || h->precision == PARAMENT_PRECISION_MIXED) ? select_fp32(h->hnorm, tau)
-                                                                         : select_fp64(h->hnorm, tau);
+        M = (sizeof(R) == 4 || mixed_req) ? select_fp32(h->hnorm, tau) : select_fp64(h->hnorm, tau);
         if (M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
+        if (mixed_req) M = mixed_order(M, tau * h->hnorm, N);
     }
     if (M >= pa::kMaxCheb) return PARAMENT_STATUS_INVALID_VALUE;
     h->last_m = M;
@@ -320,7 +336,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     const unsigned d = h->dim;
     const size_t dd = (size_t)d * d;
     const bool fused = (d <= 64) && h->path == 0;
-    const bool mixed = (sizeof(R) == 8) && h->precision == PARAMENT_PRECISION_MIXED && fused && pa::mixed_supported((int)d);
+    const bool mixed = mixed_req && fused && pa::mixed_supported((int)d);
 
     pa::Controls<R> ctl;
     ctl.c = carr_dev; ctl.pts = (int)pts; ctl.amps_in = (int)amps; ctl.amps_set = (int)h->amps;

@@ -68,11 +68,12 @@ __device__ __forceinline__ void split_fixed_sr(double x, double scale, float u, 
     s0 = (__bf16)(float)r0; s1 = (__bf16)(float)(r1 * (1.0 / 256.0)); s2 = (__bf16)(float)(r2 * (1.0 / 65536.0));
 }
 
-// Reproducible uniform [0, 1) from (slice, element, part): a 32-bit integer hash.
-__device__ __forceinline__ float hash_uniform(unsigned j, unsigned e, unsigned part) {
-    unsigned h = j * 0x9E3779B1u ^ (e * 0x85EBCA77u + part * 0xC2B2AE3Du);
+// Reproducible 32 random bits from (slice, element pair): an integer hash.  Each byte gives
+// one dither value (k + 1/2) / 256, k uniform in 0..255 (mean 1/2: unbiased rounding).
+__device__ __forceinline__ unsigned hash_bits(unsigned j, unsigned e) {
+    unsigned h = j * 0x9E3779B1u ^ (e * 0x85EBCA77u + 0xC2B2AE3Du);
     h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
-    return (float)(h >> 8) * (1.0f / 16777216.0f);
+    return h;
 }
 
 // Exponent e with m < 2^e for the largest modulus m of a matrix (0 for a zero matrix).
@@ -196,16 +197,6 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
         }
         return eb;
     };
-    // accumulators -> value: both carry the operand scales 2^(ex - 8) 2^(eb - 8)
-    auto combine = [&](const LTile& t, int ex, int eb, MTile& out) {
-        const float f = ldexpf(1.f, ex + eb - 16);
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            out.re[v] = (t.re[0][v] + t.re[1][v]) * f;
-            out.im[v] = (t.im[0][v] + t.im[1][v]) * f;
-        }
-    };
-
     for (int j = blockIdx.x; j < a.N; j += gridDim.x) {
         for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
         __syncthreads();
@@ -215,19 +206,29 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
         for (int t = 0; t < a.ncoef; ++t) bound += hypot(coef[t].x, coef[t].y) * a.opmax[t + 1];
         const int ex = scale_exp(bound);
         const double xsc = ldexp(1.0, 8 - ex);
-        for (int e = tid; e < D * D; e += 64 * G) {
-            const int i = e / D, k = e % D;
-            double2 x = mk<double>(0, 0);
-            if (i < d && k < d) x = x64(i, k);
+        typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+        for (int e2 = tid; e2 < D * D / 2; e2 += 64 * G) {
+            const int i = (2 * e2) / D, k = (2 * e2) % D;   // elements (i, k), (i, k + 1)
+            double2 x0 = mk<double>(0, 0), x1 = mk<double>(0, 0);
+            if (i < d && k < d) x0 = x64(i, k);
+            if (i < d && k + 1 < d) x1 = x64(i, k + 1);
+            const unsigned hsh = hash_bits(j, e2);
+            const float u[4] = {(float)(hsh & 0xFFu) * (1.f / 256.f) + (1.f / 512.f),
+                                (float)((hsh >> 8) & 0xFFu) * (1.f / 256.f) + (1.f / 512.f),
+                                (float)((hsh >> 16) & 0xFFu) * (1.f / 256.f) + (1.f / 512.f),
+                                (float)(hsh >> 24) * (1.f / 256.f) + (1.f / 512.f)};
+            bf16x2 p[6];
             __bf16 s0, s1, s2;
-            split_fixed_sr(x.x, xsc, hash_uniform(j, e, 0), s0, s1, s2);
-            Xp[0 * PL + i * S + k] = s0; Xp[2 * PL + i * S + k] = s1; Xp[4 * PL + i * S + k] = s2;
-            split_fixed_sr(x.y, xsc, hash_uniform(j, e, 1), s0, s1, s2);
-            Xp[1 * PL + i * S + k] = s0; Xp[3 * PL + i * S + k] = s1; Xp[5 * PL + i * S + k] = s2;
+            split_fixed_sr(x0.x, xsc, u[0], s0, s1, s2); p[0][0] = s0; p[2][0] = s1; p[4][0] = s2;
+            split_fixed_sr(x0.y, xsc, u[1], s0, s1, s2); p[1][0] = s0; p[3][0] = s1; p[5][0] = s2;
+            split_fixed_sr(x1.x, xsc, u[2], s0, s1, s2); p[0][1] = s0; p[2][1] = s1; p[4][1] = s2;
+            split_fixed_sr(x1.y, xsc, u[3], s0, s1, s2); p[1][1] = s0; p[3][1] = s1; p[5][1] = s2;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) *reinterpret_cast<bf16x2*>(Xp + q * PL + i * S + k) = p[q];
         }
         __syncthreads();
 
-        MTile b1, b2, c;
+        MTile b1, b2;
         LTile acc;
         {
             const float aMr = (float)a.are[M], aMi = (float)a.aim[M];
@@ -253,25 +254,38 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
         int eb = write_b(b1);
         __syncthreads();
 
-        // b_k = s X b_{k+1} - b_{k+2} + a_k I for k = M-2 .. 2, then b_1' (no a_1 I).  The
-        // diagonal update adds a_k in FP64: a rounded a_k would be the same error in every slice.
+        // b_k = s X b_{k+1} - b_{k+2} + a_k I for k = M-2 .. 2, then b_1' (no a_1 I).
+        // For k = 2, 3 the diagonal update adds a_k in FP64: a rounded a_k would put the same
+        // error into every slice (|a_2|, |a_3| ~ (tau |H|)^2 / 8, (tau |H|)^3 / 48); for k >= 4 the
+        // FP32 rounding of a_k is below 2^-24 |a_4| and negligible over any product length.
         for (int k = M - 2; k >= 1; --k) {
             wave_cgemm_exact<D>(Xp, Bp, row0, col0, acc);
-            combine(acc, ex, eb, c);
-            const double akr = (k >= 2) ? a.are[k] : 0.0, aki = (k >= 2) ? a.aim[k] : 0.0;
+            const float sc = sf * ldexpf(1.f, ex + eb - 16);   // s times the operand scales
             const int col = mcol(col0);
+            if (k == 2 || k == 3) {
+                const double akr = a.are[k], aki = a.aim[k];
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                float nr, ni;
-                if (mrow(row0, v) == col) {
-                    nr = (float)((double)(sf * c.re[v]) - (double)b2.re[v] + akr);
-                    ni = (float)((double)(sf * c.im[v]) - (double)b2.im[v] + aki);
-                } else {
-                    nr = sf * c.re[v] - b2.re[v];
-                    ni = sf * c.im[v] - b2.im[v];
+                for (int v = 0; v < 16; ++v) {
+                    const float cr = sc * (acc.re[0][v] + acc.re[1][v]);
+                    const float ci = sc * (acc.im[0][v] + acc.im[1][v]);
+                    float nr = cr - b2.re[v], ni = ci - b2.im[v];
+                    if (mrow(row0, v) == col) {
+                        nr = (float)((double)cr - (double)b2.re[v] + akr);
+                        ni = (float)((double)ci - (double)b2.im[v] + aki);
+                    }
+                    b2.re[v] = b1.re[v]; b2.im[v] = b1.im[v];
+                    b1.re[v] = nr; b1.im[v] = ni;
                 }
-                b2.re[v] = b1.re[v]; b2.im[v] = b1.im[v];
-                b1.re[v] = nr; b1.im[v] = ni;
+            } else {
+                const float akr = (k >= 2) ? (float)a.are[k] : 0.f, aki = (k >= 2) ? (float)a.aim[k] : 0.f;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const bool dg = mrow(row0, v) == col;
+                    const float nr = sc * (acc.re[0][v] + acc.re[1][v]) - b2.re[v] + (dg ? akr : 0.f);
+                    const float ni = sc * (acc.im[0][v] + acc.im[1][v]) - b2.im[v] + (dg ? aki : 0.f);
+                    b2.re[v] = b1.re[v]; b2.im[v] = b1.im[v];
+                    b1.re[v] = nr; b1.im[v] = ni;
+                }
             }
             wave_max(b1);
             __syncthreads();

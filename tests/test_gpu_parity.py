@@ -176,8 +176,10 @@ def test_mixed_parity(mode, magnus, d):
     """Same truncation as the FP32 table, within 1e-5 relative of the FP64 restatement."""
     h0, hks, c = case(800 + d, d, 2, 65)
     u, m = run_gpu("fp64", h0, hks, c, 0.02, mode, magnus, mixed=True)
-    assert m == eo.select_iteration_cycles_fp32(eo.hamiltonian_norm(h0, hks), 0.04 if mode == "simpson" else 0.02)
-    ref = eo.equiprop(h0, hks, c, 0.02, mode=mode, use_magnus=magnus, precision="fp32")
+    tau = 0.04 if mode == "simpson" else 0.02
+    n = eo.slice_count(65, eo._MODE_BY_NAME[mode], magnus)
+    assert m == eo.select_iteration_cycles_mixed(eo.hamiltonian_norm(h0, hks), tau, n)
+    ref = eo.equiprop(h0, hks, c, 0.02, mode=mode, use_magnus=magnus, cycles=m)
     assert rel_err(u, ref) < 1e-5
 
 

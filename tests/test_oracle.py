@@ -232,3 +232,17 @@ def test_golden_regression_vectors(golden):
         for prec in ("fp32", "fp64"):
             u = eo.equiprop(hs[0], list(hs[1:]), c, 0.05, mode=mode, use_magnus=mag, precision=prec)
             assert rel_err(u, golden[f"mode_{name}_{prec}"]) < 1e-14
+
+
+def test_mixed_order_bounds_accumulated_truncation():
+    """The mixed path's order keeps N * tail(M) <= 2e-6 and never goes below the FP32 table."""
+    for x, n in ((0.0308, 4096), (0.3376, 32767), (0.3376, 10), (1.5, 100000), (0.0005, 1)):
+        m32 = eo.select_iteration_cycles_fp32(x, 1.0)
+        m = eo.select_iteration_cycles_mixed(x, 1.0, n)
+        assert m >= m32 and m % 2 == 1
+        assert n * eo.chebyshev_tail(m, x) <= 2e-6
+        if m > m32:
+            assert n * eo.chebyshev_tail(m - 2, x) > 2e-6
+    # the case that motivated it: dim-32 GUE sum, 4096 slices at x = 0.0308 -> M = 5 (FP32 table: 3)
+    assert eo.select_iteration_cycles_mixed(0.0308, 1.0, 4096) == 5
+    assert eo.select_iteration_cycles_mixed(0.3376, 1.0, 32767) == 7
