@@ -169,6 +169,19 @@ LIBSPEC Parament_ErrorCode Parament_setTiming_fp64(Parament_Context_f64 *handle,
 LIBSPEC Parament_ErrorCode Parament_getKernelTimes(Parament_Context_f32 *handle, double *slice_ms, double *tree_ms);
 LIBSPEC Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64 *handle, double *slice_ms, double *tree_ms);
 
+/* Adjoint (GRAPE) gradient of the gate fidelity F = |Tr(target^H U)|^2 / dim^2 of the
+ * propagator U that Parament_equiprop_fp64 computes, with respect to every control sample
+ * (carr layout, real controls: the imaginary parts must be zero).  grad receives amps x pts
+ * doubles dF/dc[k*pts + j] (quadrature chain rule included), fidelity (optional) receives F.
+ * For callers optimising controls (GRAPE loops, the PyTorch autograd wrapper).  Complex128
+ * contexts, dim <= 32, no Magnus.  The Device variant takes device pointers and is
+ * asynchronous on the context's stream. */
+LIBSPEC Parament_ErrorCode Parament_gateFidelityGradient_fp64(Parament_Context_f64 *handle, const cuDoubleComplex *carr,
+    double dt, unsigned int pts, unsigned int amps, const cuDoubleComplex *target, double *fidelity, double *grad);
+LIBSPEC Parament_ErrorCode Parament_gateFidelityGradientDevice_fp64(Parament_Context_f64 *handle,
+    const cuDoubleComplex *carr_dev, double dt, unsigned int pts, unsigned int amps, const cuDoubleComplex *target_dev,
+    double *fidelity_dev, double *grad_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,6 +129,12 @@ template <class R> struct Context {
     void* d_nodes = nullptr; size_t cap_nodes = 0;
     C* d_gen[3] = {nullptr, nullptr, nullptr}; size_t cap_gen = 0;
     void* d_root = nullptr; size_t cap_root = 0;  // d x d result of the last equiprop
+    // gradient: tree levels V, prefixes P, costates L, per-slice traces, target, trace g
+    void* d_gV = nullptr; size_t cap_gV = 0;
+    void* d_gP = nullptr; size_t cap_gP = 0;
+    void* d_gL = nullptr; size_t cap_gL = 0;
+    void* d_gdg = nullptr; size_t cap_gdg = 0;
+    void* d_gaux = nullptr; size_t cap_gaux = 0;
 };
 
 template <class R> void free_hamiltonian(Context<R>* h) {
@@ -146,6 +152,8 @@ template <class R> void free_working(Context<R>* h) {
     for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
     h->cap_gen = 0;
     hipFree(h->d_root); h->d_root = nullptr; h->cap_root = 0;
+    for (void** p : {&h->d_gV, &h->d_gP, &h->d_gL, &h->d_gdg, &h->d_gaux}) { hipFree(*p); *p = nullptr; }
+    h->cap_gV = h->cap_gP = h->cap_gL = h->cap_gdg = h->cap_gaux = 0;
 }
 
 // Grow a device buffer to at least `bytes` (contents are not preserved).
@@ -499,6 +507,115 @@ Parament_ErrorCode equiprop_device(Context<R>* h, const typename ApiCplx<R>::typ
     return h->last_error = err;
 }
 
+
+// ----- adjoint gate-fidelity gradient (grad.hip) -----
+// All device pointers; target is d x d row-major, grad receives amps x pts doubles (d F / d c,
+// real controls), fid (optional) receives F = |Tr(U_t^H U)|^2 / d^2.
+Parament_ErrorCode gradient(Context<double>* h, const double2* carr_dev, double dt, unsigned pts, unsigned amps,
+                            const double2* target_dev, double* fid_dev, double* grad_dev) {
+    if (!h->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (h->magnus || !pa::grad_supported((int)h->dim) || h->ncoef > 64) return PARAMENT_STATUS_INVALID_VALUE;
+    if (amps == 0 || amps > h->amps) return PARAMENT_STATUS_INVALID_VALUE;
+    if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
+    int N = 0;
+    double tau = 0;
+    slices(h, pts, dt, &N, &tau);
+    if (N <= 0) return PARAMENT_STATUS_INVALID_VALUE;
+    int M = h->mmax;
+    if (!h->manual) {
+        M = select_fp64(h->hnorm, tau);
+        if (M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
+    }
+    if (M >= pa::kMaxCheb) return PARAMENT_STATUS_INVALID_VALUE;
+    h->last_m = M;
+    double are[pa::kMaxCheb], aim[pa::kMaxCheb];
+    bessel(M, tau * h->hnorm, are, aim);
+    const unsigned d = h->dim;
+    const size_t dd = (size_t)d * d;
+
+    pa::ChebArgs<double> ca;
+    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = (int)d; ca.N = N; ca.M = M; ca.s = 2.0 / h->hnorm;
+    ca.ctl.c = carr_dev; ca.ctl.pts = (int)pts; ca.ctl.amps_in = (int)amps; ca.ctl.amps_set = (int)h->amps;
+    ca.ctl.mode = h->quad; ca.ctl.magnus = 0; ca.ctl.tau = tau;
+    for (int k = 0; k <= M; ++k) { ca.are[k] = are[k]; ca.aim[k] = aim[k]; }
+    ca.are[0] = are[0] - 1.0;  // the slice kernel writes E_j = U_j - I
+
+    std::vector<int> n_lvl(1, N);
+    std::vector<size_t> off(1, 0);
+    while (n_lvl.back() > 1) { off.push_back(off.back() + (size_t)n_lvl.back()); n_lvl.push_back((n_lvl.back() + 1) / 2); }
+    const size_t total = off.back() + 1;
+    const int top = (int)n_lvl.size() - 1;
+    if (!grow(&h->d_gV, &h->cap_gV, total * dd * sizeof(double2)) ||
+        !grow(&h->d_gP, &h->cap_gP, total * dd * sizeof(double2)) ||
+        !grow(&h->d_gL, &h->cap_gL, total * dd * sizeof(double2)) ||
+        !grow(&h->d_gdg, &h->cap_gdg, (size_t)h->ncoef * N * sizeof(double2)) ||
+        !grow(&h->d_gaux, &h->cap_gaux, 4 * sizeof(double2)))
+        return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    double2* V = reinterpret_cast<double2*>(h->d_gV);
+    double2* P = reinterpret_cast<double2*>(h->d_gP);
+    double2* L = reinterpret_cast<double2*>(h->d_gL);
+    double2* dg = reinterpret_cast<double2*>(h->d_gdg);
+    double2* gsum = reinterpret_cast<double2*>(h->d_gaux);
+    hipStream_t st = h->stream;
+    const double2 zero = pa::mk<double>(0, 0), one = pa::mk<double>(1, 0);
+
+    ca.out = V;
+    mark(h, 0);
+    if (pa::launch_cheb_fused<double>(ca, st) != hipSuccess || pa::launch_add_identity(V, (int)d, N, st) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    for (int l = 1; l <= top; ++l)
+        if (pa::launch_pair_level(V + off[l - 1] * dd, n_lvl[l - 1], (int)d, V + off[l] * dd, st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+    mark(h, 1);
+    if (pa::launch_fill<double>(nullptr, zero, one, (int)d, 1, P + off[top] * dd, st) != hipSuccess ||
+        pa::launch_conj_transpose(target_dev, (int)d, L + off[top] * dd, st) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    for (int l = top; l >= 1; --l)
+        if (pa::launch_down_level(V + off[l - 1] * dd, n_lvl[l - 1], (int)d, P + off[l] * dd, L + off[l] * dd,
+                                  P + off[l - 1] * dd, L + off[l - 1] * dd, st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (pa::launch_grad_slices(ca, P, L, dg, st) != hipSuccess ||
+        pa::launch_target_trace(target_dev, V + off[top] * dd, (int)d, gsum, st) != hipSuccess ||
+        pa::launch_grad_finish(dg, gsum, N, (int)pts, (int)amps, h->quad, (int)d, grad_dev, fid_dev, st) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    mark(h, 2);
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+Parament_ErrorCode gradient_host(Context<double>* h, const cuDoubleComplex* carr, double dt, unsigned pts, unsigned amps,
+                                 const cuDoubleComplex* target, double* fidelity, double* grad) {
+    if (!h) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (!carr || !target || !grad) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    for (size_t i = 0; i < (size_t)amps * pts; ++i)
+        if (carr[i].y != 0.0) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;  // real controls only
+    Parament_ErrorCode err = ensure_common(h, amps);
+    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
+    const size_t dd = (size_t)h->dim * h->dim;
+    const size_t cbytes = (size_t)amps * pts * sizeof(double2);
+    // staging: controls, target, grad (amps*pts doubles), fidelity
+    const size_t need = cbytes + dd * sizeof(double2) + (size_t)amps * pts * sizeof(double) + 2 * sizeof(double);
+    if (!grow(reinterpret_cast<void**>(&h->d_c), &h->cap_c, need)) return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    char* base = reinterpret_cast<char*>(h->d_c);
+    double2* dc = reinterpret_cast<double2*>(base);
+    double2* dt_ = reinterpret_cast<double2*>(base + cbytes);
+    double* dgrad = reinterpret_cast<double*>(base + cbytes + dd * sizeof(double2));
+    double* dfid = dgrad + (size_t)amps * pts;
+    if (hipMemcpyAsync(dc, carr, cbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(dt_, target, dd * sizeof(double2), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    err = gradient(h, dc, dt, pts, amps, dt_, dfid, dgrad);
+    if (err == PARAMENT_STATUS_SUCCESS) {
+        if (hipMemcpyAsync(grad, dgrad, (size_t)amps * pts * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            (fidelity && hipMemcpyAsync(fidelity, dfid, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess) ||
+            hipStreamSynchronize(h->stream) != hipSuccess)
+            err = PARAMENT_STATUS_CUBLAS_FAILED;
+    } else {
+        hipStreamSynchronize(h->stream);
+    }
+    return h->last_error = err;
+}
+
 template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, double* b) {
     if (!h->timing || !h->ev[0] || !h->ev[1] || !h->ev[2]) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
     if (hipEventSynchronize(h->ev[2]) != hipSuccess) return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
@@ -672,6 +789,25 @@ Parament_ErrorCode Parament_getKernelTimes(Parament_Context_f32* handle, double*
 }
 Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64* handle, double* slice_ms, double* tree_ms) {
     return handle ? kernel_times(CTX64(handle), slice_ms, tree_ms) : PARAMENT_STATUS_INVALID_VALUE;
+}
+
+Parament_ErrorCode Parament_gateFidelityGradient_fp64(Parament_Context_f64* handle, const cuDoubleComplex* carr, double dt,
+                                                    unsigned int pts, unsigned int amps, const cuDoubleComplex* target,
+                                                    double* fidelity, double* grad) {
+    return gradient_host(CTX64(handle), carr, dt, pts, amps, target, fidelity, grad);
+}
+Parament_ErrorCode Parament_gateFidelityGradientDevice_fp64(Parament_Context_f64* handle, const cuDoubleComplex* carr_dev,
+                                                          double dt, unsigned int pts, unsigned int amps,
+                                                          const cuDoubleComplex* target_dev, double* fidelity_dev,
+                                                          double* grad_dev) {
+    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* h = CTX64(handle);
+    if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (!carr_dev || !target_dev || !grad_dev) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    Parament_ErrorCode err = ensure_common(h, amps);
+    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
+    return h->last_error = gradient(h, reinterpret_cast<const double2*>(carr_dev), dt, pts, amps,
+                                    reinterpret_cast<const double2*>(target_dev), fidelity_dev, grad_dev);
 }
 
 void device_info(void) {

@@ -4,21 +4,6 @@
 
 namespace pa {
 
-// LDS row padding (elements) of the A-operand and B-operand planes; chosen so the
-// 16x16x4 operand reads of a half-wave fall into distinct banks.
-template <class R> struct Pad;
-template <> struct Pad<float> { static constexpr int A = 2, B = 16; };
-template <> struct Pad<double> { static constexpr int A = 1, B = 8; };
-
-// Work split of one slice: D = 64 uses 4 waves (one 32x32 quadrant each); D = 32 and
-// D = 16 use a single wave owning the whole (32x32 or 16x16) matrix.
-template <int D> struct Shape {
-    static constexpr int G = (D == 64) ? 4 : 1;
-    static constexpr int T = (D == 16) ? 1 : 2;
-    static __device__ __forceinline__ int row0(int w) { return D == 64 ? 32 * (w >> 1) : 0; }
-    static __device__ __forceinline__ int col0(int w) { return D == 64 ? 32 * (w & 1) : 0; }
-};
-
 // ---------------------------------------------------------------------------------------
 // Fused slice kernel: for each slice j, assemble X_j = H0 + sum_t c'_tj O_t in LDS, run the
 // Clenshaw recurrence of parament.cpp:569-652 with all states on chip, and write the slice

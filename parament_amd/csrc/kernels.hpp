@@ -60,4 +60,16 @@ template <class R> hipError_t launch_gemm(const cplx<R>* A, long sa, const cplx<
                                           hipStream_t st, bool add_ab = false);
 template <class R> hipError_t launch_zero(unsigned* p, size_t n, hipStream_t st);
 
+// Adjoint gate-fidelity gradient (grad.hip), complex128, dims <= 32.
+bool grad_supported(int d);
+hipError_t launch_add_identity(double2* m, int d, int n, hipStream_t st);
+hipError_t launch_pair_level(const double2* in, int n_in, int d, double2* out, hipStream_t st);
+hipError_t launch_down_level(const double2* Vc, int n_child, int d, const double2* Pp, const double2* Lp, double2* Pc,
+                             double2* Lc, hipStream_t st);
+hipError_t launch_conj_transpose(const double2* t, int d, double2* out, hipStream_t st);
+hipError_t launch_grad_slices(const ChebArgs<double>& a, const double2* P, const double2* L, double2* dg, hipStream_t st);
+hipError_t launch_target_trace(const double2* t, const double2* u, int d, double2* out, hipStream_t st);
+hipError_t launch_grad_finish(const double2* dg, const double2* g, int N, int pts, int amps, int mode, int d, double* grad,
+                              double* fid, hipStream_t st);
+
 }  // namespace pa

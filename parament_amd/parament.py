@@ -226,6 +226,43 @@ class Parament:
                                      int(pts), int(amps), ctypes.c_void_p(out_ptr)))
 
     @_must_be_alive
+    def gate_fidelity_gradient(self, dt, *carr, target):
+        """F = |Tr(target^H U)|^2 / dim^2 and dF/dc for every control sample (adjoint method).
+
+        Complex128 ('fp64') contexts, real controls, dim <= 32, no Magnus.  Returns
+        ``(F, grad)`` with ``grad`` shaped like the stacked controls, ``(amps, pts)``.
+        """
+        if not self._use_doubles:
+            raise ValueError("gradients need an 'fp64' context")
+        if self.amps < 0:
+            raise RuntimeError("No hamiltonian set")
+        amps = len(carr)
+        if amps > self.amps:
+            raise ValueError(f'Got {amps} amplitude arrays, but there are only {self.amps} Hamiltonians.')
+        pts = np.shape(carr[0])[0]
+        if any(np.shape(carri) != (pts,) for carri in carr):
+            raise ValueError("All amplitude arrays must have the same length.")
+        c = np.asarray(carr)
+        if np.iscomplexobj(c) and np.any(c.imag != 0):
+            raise ValueError("gradients need real control amplitudes")
+        c = np.ascontiguousarray(np.ravel(c.real.astype(np.complex128), order='C'))
+        t = np.ascontiguousarray(np.ravel(np.asarray(target, dtype=np.complex128), order='C'))
+        if t.size != self.dim ** 2:
+            raise ValueError("target must be a dim x dim matrix")
+        fid = ctypes.c_double()
+        grad = np.zeros(amps * pts, dtype=np.float64)
+        self._check_error(self._lib.Parament_gateFidelityGradient_fp64(self._handle, c, float(dt), int(pts), int(amps),
+                                                                      t, ctypes.byref(fid), grad))
+        return fid.value, grad.reshape(amps, pts)
+
+    @_must_be_alive
+    def gate_fidelity_gradient_device(self, dt, carr_ptr, pts, amps, target_ptr, fid_ptr, grad_ptr):
+        """Device-pointer variant (asynchronous on the context's stream)."""
+        self._check_error(self._lib.Parament_gateFidelityGradientDevice_fp64(
+            self._handle, ctypes.c_void_p(carr_ptr), float(dt), int(pts), int(amps), ctypes.c_void_p(target_ptr),
+            ctypes.c_void_p(fid_ptr) if fid_ptr else None, ctypes.c_void_p(grad_ptr)))
+
+    @_must_be_alive
     def set_stream(self, stream_ptr):
         """Run on the given HIP stream (an int handle, e.g. torch.cuda.current_stream().cuda_stream); None resets."""
         self._check_error(self._call('Parament_setStream', self._handle, ctypes.c_void_p(stream_ptr or None)))

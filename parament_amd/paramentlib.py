@@ -81,6 +81,13 @@ for sfx, cptr in (("", c_cuComplex_p), ("_fp64", c_cuDoubleComplex_p)):
     getattr(lib, "Parament_getKernelTimes" + sfx).argtypes = [c_ParamentContext_p, ctypes.POINTER(ctypes.c_double),
                                                               ctypes.POINTER(ctypes.c_double)]
 
+lib.Parament_gateFidelityGradient_fp64.argtypes = [c_ParamentContext_p, c_cuDoubleComplex_p, ctypes.c_double,
+                                                   ctypes.c_uint, ctypes.c_uint, c_cuDoubleComplex_p,
+                                                   ctypes.POINTER(ctypes.c_double),
+                                                   np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+lib.Parament_gateFidelityGradientDevice_fp64.argtypes = [c_ParamentContext_p, ctypes.c_void_p, ctypes.c_double,
+                                                         ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
+                                                         ctypes.c_void_p, ctypes.c_void_p]
 lib.Parament_setPrecisionMode_fp64.argtypes = [c_ParamentContext_p, ctypes.c_int]
 lib.Parament_errorMessage.argtypes = [ctypes.c_int]
 lib.Parament_errorMessage.restype = ctypes.c_char_p

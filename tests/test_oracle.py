@@ -246,3 +246,34 @@ def test_mixed_order_bounds_accumulated_truncation():
     # the case that motivated it: dim-32 GUE sum, 4096 slices at x = 0.0308 -> M = 5 (FP32 table: 3)
     assert eo.select_iteration_cycles_mixed(0.0308, 1.0, 4096) == 5
     assert eo.select_iteration_cycles_mixed(0.3376, 1.0, 32767) == 7
+
+
+# ---------------- gradient oracle (oracle/grape_oracle.py) ----------------
+
+def test_frechet_block_matches_scipy_expm_frechet():
+    from oracle.grape_oracle import frechet_block
+    rng = np.random.default_rng(9)
+    a = -1j * 0.3 * gue(rng, 5)
+    e = -1j * 0.3 * gue(rng, 5)
+    ref = scipy.linalg.expm_frechet(a, e, compute_expm=False)
+    assert np.allclose(frechet_block(a, e), ref, atol=1e-13)
+
+
+@pytest.mark.parametrize("mode", ["none", "midpoint", "simpson"])
+def test_grape_oracle_matches_central_differences(mode):
+    from oracle.grape_oracle import fidelity, fidelity_gradient
+    from parament_amd.workloads import haar_unitary
+    rng = np.random.default_rng(21)
+    h = [gue(rng, 3) for _ in range(3)]
+    c = rng.uniform(-1, 1, (2, 9))
+    t = haar_unitary(rng, 3)
+    f, grad = fidelity_gradient(h[0], h[1:], c, 0.2, mode, t)
+    assert abs(f - fidelity(h[0], h[1:], c, 0.2, mode, t)) < 1e-13
+    hstep = 1e-6
+    for k in range(2):
+        for p in range(9):
+            cp, cm = c.copy(), c.copy()
+            cp[k, p] += hstep
+            cm[k, p] -= hstep
+            fd = (fidelity(h[0], h[1:], cp, 0.2, mode, t) - fidelity(h[0], h[1:], cm, 0.2, mode, t)) / (2 * hstep)
+            assert abs(fd - grad[k, p]) < 1e-8, (k, p, fd, grad[k, p])
