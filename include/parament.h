@@ -19,6 +19,7 @@
 extern "C" {
 #else
 #include <stdbool.h>
+#include <stddef.h>
 #endif
 
 #if defined(__GNUC__)
@@ -154,8 +155,10 @@ LIBSPEC Parament_ErrorCode Parament_setKernelPath(Parament_Context_f32 *handle, 
 LIBSPEC Parament_ErrorCode Parament_setKernelPath_fp64(Parament_Context_f64 *handle, int path);
 
 /* Stream control, for callers that order equiprop against their own device work (e.g. a
- * PyTorch stream).  The context uses its own non-blocking stream unless one is set;
- * setStream(NULL) returns to the context's own stream. */
+ * PyTorch stream).  A context works on its own non-blocking stream (created by the first
+ * setHamiltonian; getStream returns it) until setStream is called; setStream(NULL) selects
+ * the device's null stream, and passing back the handle getStream returned earlier restores
+ * the context's own stream. */
 LIBSPEC Parament_ErrorCode Parament_setStream(Parament_Context_f32 *handle, void *hip_stream);
 LIBSPEC Parament_ErrorCode Parament_setStream_fp64(Parament_Context_f64 *handle, void *hip_stream);
 LIBSPEC void *Parament_getStream(Parament_Context_f32 *handle);
@@ -170,17 +173,39 @@ LIBSPEC Parament_ErrorCode Parament_getKernelTimes(Parament_Context_f32 *handle,
 LIBSPEC Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64 *handle, double *slice_ms, double *tree_ms);
 
 /* Adjoint (GRAPE) gradient of the gate fidelity F = |Tr(target^H U)|^2 / dim^2 of the
- * propagator U that Parament_equiprop_fp64 computes, with respect to every control sample
+ * propagator U that Parament_equiprop[_fp64] computes, with respect to every control sample
  * (carr layout, real controls: the imaginary parts must be zero).  grad receives amps x pts
- * doubles dF/dc[k*pts + j] (quadrature chain rule included), fidelity (optional) receives F.
- * For callers optimising controls (GRAPE loops, the PyTorch autograd wrapper).  Complex128
- * contexts, dim <= 32, no Magnus.  The Device variant takes device pointers and is
- * asynchronous on the context's stream. */
+ * values dF/dc[k*pts + j] (quadrature chain rule included), fidelity (optional) receives F.
+ * For callers optimising controls (GRAPE loops, the PyTorch autograd wrapper).  dim <= 32,
+ * no Magnus.  The Device variants take device pointers and are asynchronous on the
+ * context's stream; they read only the real parts of carr_dev. */
+LIBSPEC Parament_ErrorCode Parament_gateFidelityGradient(Parament_Context_f32 *handle, const cuComplex *carr,
+    double dt, unsigned int pts, unsigned int amps, const cuComplex *target, float *fidelity, float *grad);
 LIBSPEC Parament_ErrorCode Parament_gateFidelityGradient_fp64(Parament_Context_f64 *handle, const cuDoubleComplex *carr,
     double dt, unsigned int pts, unsigned int amps, const cuDoubleComplex *target, double *fidelity, double *grad);
+LIBSPEC Parament_ErrorCode Parament_gateFidelityGradientDevice(Parament_Context_f32 *handle,
+    const cuComplex *carr_dev, double dt, unsigned int pts, unsigned int amps, const cuComplex *target_dev,
+    float *fidelity_dev, float *grad_dev);
 LIBSPEC Parament_ErrorCode Parament_gateFidelityGradientDevice_fp64(Parament_Context_f64 *handle,
     const cuDoubleComplex *carr_dev, double dt, unsigned int pts, unsigned int amps, const cuDoubleComplex *target_dev,
     double *fidelity_dev, double *grad_dev);
+
+/* Memory policy of the gradient (no reference counterpart; for long control sequences).
+ *   segment_slices = 0: keep the whole product tree of the N slices (default; fastest).
+ *   segment_slices = 1: checkpoint at segments of 2^ceil(log2 sqrt(N)) slices.
+ *   segment_slices >= 2: checkpoint at segments of that many slices (rounded up to a power of two).
+ * With checkpointing only the segment products and the tree above them stay in memory; the
+ * slices are processed in chunks of chunk_slices (0: max(segment, 2048); rounded up to a
+ * multiple of the segment) whose propagators are computed again in the backward pass, so
+ * the working set is O(N / segment + chunk) matrices for the cost of one more forward pass.
+ * Results are identical with and without checkpointing.  getGradientWorkspace returns the
+ * device bytes the last gradient call worked in. */
+LIBSPEC Parament_ErrorCode Parament_setGradientCheckpointing(Parament_Context_f32 *handle, unsigned int segment_slices,
+    unsigned int chunk_slices);
+LIBSPEC Parament_ErrorCode Parament_setGradientCheckpointing_fp64(Parament_Context_f64 *handle,
+    unsigned int segment_slices, unsigned int chunk_slices);
+LIBSPEC size_t Parament_getGradientWorkspace(Parament_Context_f32 *handle);
+LIBSPEC size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64 *handle);
 
 #ifdef __cplusplus
 }

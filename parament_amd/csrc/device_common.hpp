@@ -176,13 +176,13 @@ template <class R> struct Pad;
 template <> struct Pad<float> { static constexpr int A = 2, B = 16; };
 template <> struct Pad<double> { static constexpr int A = 1, B = 8; };
 
-// Work split of one d x d product: D = 64 uses 4 waves (one 32x32 quadrant each); D = 32 and
-// D = 16 use a single wave owning the whole (32x32 or 16x16) matrix.
+// Work split of one d x d product: D = 64 and D = 32 use 4 waves, each owning one quadrant
+// (32x32 or 16x16); D = 16 uses a single wave owning the whole matrix.
 template <int D> struct Shape {
-    static constexpr int G = (D == 64) ? 4 : 1;
-    static constexpr int T = (D == 16) ? 1 : 2;
-    static __device__ __forceinline__ int row0(int w) { return D == 64 ? 32 * (w >> 1) : 0; }
-    static __device__ __forceinline__ int col0(int w) { return D == 64 ? 32 * (w & 1) : 0; }
+    static constexpr int G = (D == 16) ? 1 : 4;
+    static constexpr int T = (D == 64) ? 2 : 1;
+    static __device__ __forceinline__ int row0(int w) { return D == 16 ? 0 : (D / 2) * (w >> 1); }
+    static __device__ __forceinline__ int col0(int w) { return D == 16 ? 0 : (D / 2) * (w & 1); }
 };
 
 }  // namespace pa

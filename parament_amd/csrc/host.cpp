@@ -119,6 +119,7 @@ template <class R> struct Context {
     int device = -1;
     hipStream_t stream = nullptr;      // stream in use (own_stream unless the caller set one)
     hipStream_t own_stream = nullptr;
+    bool user_stream = false;          // setStream was called: `stream` is the caller's (NULL = null stream)
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     C* d_ops = nullptr;
@@ -129,12 +130,14 @@ template <class R> struct Context {
     void* d_nodes = nullptr; size_t cap_nodes = 0;
     C* d_gen[3] = {nullptr, nullptr, nullptr}; size_t cap_gen = 0;
     void* d_root = nullptr; size_t cap_root = 0;  // d x d result of the last equiprop
-    // gradient: tree levels V, prefixes P, costates L, per-slice traces, target, trace g
+    // gradient: leaves E, tree levels V, costates W, per-slice traces, trace g; tree layout policy
+    void* d_gE = nullptr; size_t cap_gE = 0;
     void* d_gV = nullptr; size_t cap_gV = 0;
-    void* d_gP = nullptr; size_t cap_gP = 0;
-    void* d_gL = nullptr; size_t cap_gL = 0;
+    void* d_gW = nullptr; size_t cap_gW = 0;
     void* d_gdg = nullptr; size_t cap_gdg = 0;
     void* d_gaux = nullptr; size_t cap_gaux = 0;
+    unsigned ckpt_seg = 0, ckpt_chunk = 0;
+    size_t grad_workspace = 0;  // device bytes the last gradient call worked in
 };
 
 template <class R> void free_hamiltonian(Context<R>* h) {
@@ -152,8 +155,8 @@ template <class R> void free_working(Context<R>* h) {
     for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
     h->cap_gen = 0;
     hipFree(h->d_root); h->d_root = nullptr; h->cap_root = 0;
-    for (void** p : {&h->d_gV, &h->d_gP, &h->d_gL, &h->d_gdg, &h->d_gaux}) { hipFree(*p); *p = nullptr; }
-    h->cap_gV = h->cap_gP = h->cap_gL = h->cap_gdg = h->cap_gaux = 0;
+    for (void** p : {&h->d_gE, &h->d_gV, &h->d_gW, &h->d_gdg, &h->d_gaux}) { hipFree(*p); *p = nullptr; }
+    h->cap_gE = h->cap_gV = h->cap_gW = h->cap_gdg = h->cap_gaux = 0;
 }
 
 // Grow a device buffer to at least `bytes` (contents are not preserved).
@@ -177,7 +180,7 @@ template <class R> Parament_ErrorCode bind_device(Context<R>* h) {
         hipGetLastError();
         return PARAMENT_STATUS_CUBLAS_INIT_FAILED;
     }
-    if (!h->stream) h->stream = h->own_stream;
+    if (!h->user_stream) h->stream = h->own_stream;
     h->device = dev;
     h->bound = true;
     return PARAMENT_STATUS_SUCCESS;
@@ -509,12 +512,56 @@ Parament_ErrorCode equiprop_device(Context<R>* h, const typename ApiCplx<R>::typ
 
 
 // ----- adjoint gate-fidelity gradient (grad.hip) -----
-// All device pointers; target is d x d row-major, grad receives amps x pts doubles (d F / d c,
-// real controls), fid (optional) receives F = |Tr(U_t^H U)|^2 / d^2.
-Parament_ErrorCode gradient(Context<double>* h, const double2* carr_dev, double dt, unsigned pts, unsigned amps,
-                            const double2* target_dev, double* fid_dev, double* grad_dev) {
+// Layout of one gradient call over the pairwise tree.  Level l has n[l] = ceil(N / 2^l) nodes,
+// level `top` (>= 1) is the root.  Levels seg .. top are kept for the whole call (the "top"
+// arrays: V = node products, W = costates); levels 1 .. seg-1 live in chunk arrays that are
+// reused by consecutive chunks of `chunk` slices (a multiple of 2^seg).  seg = top with one
+// chunk keeps the whole tree; seg ~ log2(sqrt N) is the checkpointed layout: the first pass
+// stores only the level-seg products, the second pass recomputes each chunk's slices.
+struct GradPlan {
+    int N = 0, top = 1, seg = 1, chunk = 1, nchunks = 1;
+    std::vector<int> n;
+    std::vector<size_t> coff, toff;   // matrix offsets of chunk levels (1..seg-1) and top levels (seg..top)
+    size_t chunk_mats = 0, top_mats = 0;
+};
+
+GradPlan grad_plan(int N, unsigned seg_req, unsigned chunk_req) {
+    GradPlan p;
+    p.N = N;
+    while ((1L << p.top) < N) ++p.top;
+    for (int l = 0; l <= p.top; ++l) p.n.push_back((int)(((long)N + (1L << l) - 1) >> l));
+    int seg = p.top;
+    if (seg_req == 1) {
+        int b = 1;
+        while ((1L << (2 * b)) < N) ++b;   // 2^b >= sqrt(N)
+        seg = std::min(b, p.top);
+    } else if (seg_req >= 2) {
+        int b = 1;
+        while ((1L << b) < (long)seg_req) ++b;
+        seg = std::min(b, p.top);
+    }
+    const long B = 1L << seg;
+    long chunk = chunk_req ? (long)chunk_req : std::max<long>(B, 2048);
+    chunk = (chunk + B - 1) / B * B;
+    if (chunk >= N) { seg = p.top; chunk = N; }
+    p.seg = seg;
+    p.chunk = (int)chunk;
+    p.nchunks = (int)((N + chunk - 1) / chunk);
+    p.coff.assign(p.top + 1, 0);
+    p.toff.assign(p.top + 1, 0);
+    for (int l = 1; l < seg; ++l) { p.coff[l] = p.chunk_mats; p.chunk_mats += (size_t)((chunk + (1L << l) - 1) >> l); }
+    for (int l = seg; l <= p.top; ++l) { p.toff[l] = p.top_mats; p.top_mats += (size_t)p.n[l]; }
+    return p;
+}
+
+// All device pointers; target is d x d row-major, grad receives amps x pts values of dF/dc
+// (real controls: only the real parts of carr are read), fid (optional) F = |Tr(U_t^H U)|^2 / d^2.
+template <class R>
+Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps,
+                            const cplx<R>* target_dev, R* fid_dev, R* grad_dev) {
+    typedef cplx<R> C;
     if (!h->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
-    if (h->magnus || !pa::grad_supported((int)h->dim) || h->ncoef > 64) return PARAMENT_STATUS_INVALID_VALUE;
+    if (h->magnus || !pa::grad_supported((int)h->dim) || h->ncoef > pa::kGradCoeffs) return PARAMENT_STATUS_INVALID_VALUE;
     if (amps == 0 || amps > h->amps) return PARAMENT_STATUS_INVALID_VALUE;
     if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
     int N = 0;
@@ -523,97 +570,140 @@ Parament_ErrorCode gradient(Context<double>* h, const double2* carr_dev, double 
     if (N <= 0) return PARAMENT_STATUS_INVALID_VALUE;
     int M = h->mmax;
     if (!h->manual) {
-        M = select_fp64(h->hnorm, tau);
+        M = sizeof(R) == 4 ? select_fp32(h->hnorm, tau) : select_fp64(h->hnorm, tau);
         if (M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
     }
     if (M >= pa::kMaxCheb) return PARAMENT_STATUS_INVALID_VALUE;
     h->last_m = M;
     double are[pa::kMaxCheb], aim[pa::kMaxCheb];
     bessel(M, tau * h->hnorm, are, aim);
-    const unsigned d = h->dim;
+    const int d = (int)h->dim;
     const size_t dd = (size_t)d * d;
 
-    pa::ChebArgs<double> ca;
-    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = (int)d; ca.N = N; ca.M = M; ca.s = 2.0 / h->hnorm;
+    pa::ChebArgs<R> ca;
+    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = d; ca.M = M; ca.s = (R)(2.0 / h->hnorm);
     ca.ctl.c = carr_dev; ca.ctl.pts = (int)pts; ca.ctl.amps_in = (int)amps; ca.ctl.amps_set = (int)h->amps;
-    ca.ctl.mode = h->quad; ca.ctl.magnus = 0; ca.ctl.tau = tau;
-    for (int k = 0; k <= M; ++k) { ca.are[k] = are[k]; ca.aim[k] = aim[k]; }
-    ca.are[0] = are[0] - 1.0;  // the slice kernel writes E_j = U_j - I
+    ca.ctl.mode = h->quad; ca.ctl.magnus = 0; ca.ctl.tau = (R)tau;
+    for (int k = 0; k <= M; ++k) { ca.are[k] = (R)are[k]; ca.aim[k] = (R)aim[k]; }
+    ca.are[0] = (R)(are[0] - 1.0);  // slices are written as E_j = U_j - I
 
-    std::vector<int> n_lvl(1, N);
-    std::vector<size_t> off(1, 0);
-    while (n_lvl.back() > 1) { off.push_back(off.back() + (size_t)n_lvl.back()); n_lvl.push_back((n_lvl.back() + 1) / 2); }
-    const size_t total = off.back() + 1;
-    const int top = (int)n_lvl.size() - 1;
-    if (!grow(&h->d_gV, &h->cap_gV, total * dd * sizeof(double2)) ||
-        !grow(&h->d_gP, &h->cap_gP, total * dd * sizeof(double2)) ||
-        !grow(&h->d_gL, &h->cap_gL, total * dd * sizeof(double2)) ||
-        !grow(&h->d_gdg, &h->cap_gdg, (size_t)h->ncoef * N * sizeof(double2)) ||
-        !grow(&h->d_gaux, &h->cap_gaux, 4 * sizeof(double2)))
+    const GradPlan p = grad_plan(N, h->ckpt_seg, h->ckpt_chunk);
+    const size_t mats_vw = p.chunk_mats + p.top_mats;
+    const size_t bytes_e = (size_t)p.chunk * dd * sizeof(C), bytes_vw = mats_vw * dd * sizeof(C);
+    const size_t bytes_dg = (size_t)h->ncoef * N * sizeof(C);
+    if (!grow(&h->d_gE, &h->cap_gE, bytes_e) || !grow(&h->d_gV, &h->cap_gV, bytes_vw) ||
+        !grow(&h->d_gW, &h->cap_gW, bytes_vw) || !grow(&h->d_gdg, &h->cap_gdg, bytes_dg) ||
+        !grow(&h->d_gaux, &h->cap_gaux, 4 * sizeof(C)))
         return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
-    double2* V = reinterpret_cast<double2*>(h->d_gV);
-    double2* P = reinterpret_cast<double2*>(h->d_gP);
-    double2* L = reinterpret_cast<double2*>(h->d_gL);
-    double2* dg = reinterpret_cast<double2*>(h->d_gdg);
-    double2* gsum = reinterpret_cast<double2*>(h->d_gaux);
+    h->grad_workspace = bytes_e + 2 * bytes_vw + bytes_dg + 4 * sizeof(C);
+    C* E = reinterpret_cast<C*>(h->d_gE);
+    C* V = reinterpret_cast<C*>(h->d_gV);
+    C* W = reinterpret_cast<C*>(h->d_gW);
+    C* dg = reinterpret_cast<C*>(h->d_gdg);
+    C* gsum = reinterpret_cast<C*>(h->d_gaux);
+    auto vc = [&](int l) { return V + p.coff[l] * dd; };
+    auto wc = [&](int l) { return W + p.coff[l] * dd; };
+    auto vt = [&](int l) { return V + (p.chunk_mats + p.toff[l]) * dd; };
+    auto wt = [&](int l) { return W + (p.chunk_mats + p.toff[l]) * dd; };
     hipStream_t st = h->stream;
-    const double2 zero = pa::mk<double>(0, 0), one = pa::mk<double>(1, 0);
+    auto count = [](int cnt, int l) { return (int)(((long)cnt + (1L << l) - 1) >> l); };
 
-    ca.out = V;
+    // Slices of chunk c and their products up to level `upto` (level seg goes to the top arrays).
+    auto forward_chunk = [&](int c, int upto) -> bool {
+        const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
+        ca.j0 = j0; ca.N = cnt; ca.out = E;
+        if (pa::launch_cheb_fused<R>(ca, st) != hipSuccess) return false;
+        for (int l = 1; l <= upto; ++l) {
+            const C* in = (l == 1) ? E : vc(l - 1);
+            C* out = (l == p.seg) ? vt(p.seg) + (size_t)(j0 >> p.seg) * dd : vc(l);
+            if (pa::launch_up_level<R>(in, count(cnt, l - 1), d, out, l == 1, st) != hipSuccess) return false;
+        }
+        return true;
+    };
+
     mark(h, 0);
-    if (pa::launch_cheb_fused<double>(ca, st) != hipSuccess || pa::launch_add_identity(V, (int)d, N, st) != hipSuccess)
-        return PARAMENT_STATUS_CUBLAS_FAILED;
-    for (int l = 1; l <= top; ++l)
-        if (pa::launch_pair_level(V + off[l - 1] * dd, n_lvl[l - 1], (int)d, V + off[l] * dd, st) != hipSuccess)
+    for (int c = 0; c < p.nchunks; ++c)
+        if (!forward_chunk(c, p.seg)) return PARAMENT_STATUS_CUBLAS_FAILED;
+    for (int l = p.seg + 1; l <= p.top; ++l)
+        if (pa::launch_up_level<R>(vt(l - 1), p.n[l - 1], d, vt(l), false, st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (pa::launch_target_trace<R>(target_dev, vt(p.top), d, gsum, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 1);
-    if (pa::launch_fill<double>(nullptr, zero, one, (int)d, 1, P + off[top] * dd, st) != hipSuccess ||
-        pa::launch_conj_transpose(target_dev, (int)d, L + off[top] * dd, st) != hipSuccess)
-        return PARAMENT_STATUS_CUBLAS_FAILED;
-    for (int l = top; l >= 1; --l)
-        if (pa::launch_down_level(V + off[l - 1] * dd, n_lvl[l - 1], (int)d, P + off[l] * dd, L + off[l] * dd,
-                                  P + off[l - 1] * dd, L + off[l - 1] * dd, st) != hipSuccess)
+    if (pa::launch_conj_transpose<R>(target_dev, d, wt(p.top), st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    for (int l = p.top - 1; l >= p.seg; --l)
+        if (pa::launch_down_level<R>(vt(l), p.n[l], d, wt(l + 1), wt(l), st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (pa::launch_grad_slices(ca, P, L, dg, st) != hipSuccess ||
-        pa::launch_target_trace(target_dev, V + off[top] * dd, (int)d, gsum, st) != hipSuccess ||
-        pa::launch_grad_finish(dg, gsum, N, (int)pts, (int)amps, h->quad, (int)d, grad_dev, fid_dev, st) != hipSuccess)
+    for (int c = 0; c < p.nchunks; ++c) {
+        const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
+        if (p.nchunks > 1 && !forward_chunk(c, p.seg - 1)) return PARAMENT_STATUS_CUBLAS_FAILED;
+        for (int l = p.seg - 1; l >= 1; --l) {
+            const C* parent = (l + 1 == p.seg) ? wt(p.seg) + (size_t)(j0 >> p.seg) * dd : wc(l + 1);
+            if (pa::launch_down_level<R>(vc(l), count(cnt, l), d, parent, wc(l), st) != hipSuccess)
+                return PARAMENT_STATUS_CUBLAS_FAILED;
+        }
+        const C* W1 = (p.seg == 1) ? wt(1) + (size_t)(j0 >> 1) * dd : wc(1);
+        ca.j0 = j0; ca.N = cnt;
+        if (pa::launch_grad_slices<R>(ca, E, W1, dg, N, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    }
+    if (pa::launch_grad_finish<R>(dg, gsum, N, (int)pts, (int)amps, h->quad, d, grad_dev, fid_dev, st) != hipSuccess)
         return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 2);
     return PARAMENT_STATUS_SUCCESS;
 }
 
-Parament_ErrorCode gradient_host(Context<double>* h, const cuDoubleComplex* carr, double dt, unsigned pts, unsigned amps,
-                                 const cuDoubleComplex* target, double* fidelity, double* grad) {
+template <class R>
+Parament_ErrorCode gradient_host(Context<R>* h, const typename ApiCplx<R>::type* carr, double dt, unsigned pts,
+                                 unsigned amps, const typename ApiCplx<R>::type* target, R* fidelity, R* grad) {
+    typedef cplx<R> C;
     if (!h) return PARAMENT_STATUS_INVALID_VALUE;
     if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
     if (!carr || !target || !grad) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
     for (size_t i = 0; i < (size_t)amps * pts; ++i)
-        if (carr[i].y != 0.0) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;  // real controls only
+        if (carr[i].y != 0) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;  // real controls only
     Parament_ErrorCode err = ensure_common(h, amps);
     if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
     const size_t dd = (size_t)h->dim * h->dim;
-    const size_t cbytes = (size_t)amps * pts * sizeof(double2);
-    // staging: controls, target, grad (amps*pts doubles), fidelity
-    const size_t need = cbytes + dd * sizeof(double2) + (size_t)amps * pts * sizeof(double) + 2 * sizeof(double);
+    const size_t cbytes = (size_t)amps * pts * sizeof(C);
+    // staging: controls, target, grad (amps * pts values), fidelity
+    const size_t need = cbytes + dd * sizeof(C) + ((size_t)amps * pts + 2) * sizeof(R);
     if (!grow(reinterpret_cast<void**>(&h->d_c), &h->cap_c, need)) return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
     char* base = reinterpret_cast<char*>(h->d_c);
-    double2* dc = reinterpret_cast<double2*>(base);
-    double2* dt_ = reinterpret_cast<double2*>(base + cbytes);
-    double* dgrad = reinterpret_cast<double*>(base + cbytes + dd * sizeof(double2));
-    double* dfid = dgrad + (size_t)amps * pts;
+    C* dc = reinterpret_cast<C*>(base);
+    C* dtgt = reinterpret_cast<C*>(base + cbytes);
+    R* dgrad = reinterpret_cast<R*>(base + cbytes + dd * sizeof(C));
+    R* dfid = dgrad + (size_t)amps * pts;
     if (hipMemcpyAsync(dc, carr, cbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(dt_, target, dd * sizeof(double2), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        hipMemcpyAsync(dtgt, target, dd * sizeof(C), hipMemcpyHostToDevice, h->stream) != hipSuccess)
         return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
-    err = gradient(h, dc, dt, pts, amps, dt_, dfid, dgrad);
+    err = gradient(h, dc, dt, pts, amps, dtgt, dfid, dgrad);
     if (err == PARAMENT_STATUS_SUCCESS) {
-        if (hipMemcpyAsync(grad, dgrad, (size_t)amps * pts * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-            (fidelity && hipMemcpyAsync(fidelity, dfid, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess) ||
+        if (hipMemcpyAsync(grad, dgrad, (size_t)amps * pts * sizeof(R), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            (fidelity && hipMemcpyAsync(fidelity, dfid, sizeof(R), hipMemcpyDeviceToHost, h->stream) != hipSuccess) ||
             hipStreamSynchronize(h->stream) != hipSuccess)
             err = PARAMENT_STATUS_CUBLAS_FAILED;
     } else {
         hipStreamSynchronize(h->stream);
     }
     return h->last_error = err;
+}
+
+template <class R>
+Parament_ErrorCode gradient_device(Context<R>* h, const typename ApiCplx<R>::type* carr_dev, double dt, unsigned pts,
+                                   unsigned amps, const typename ApiCplx<R>::type* target_dev, R* fid_dev, R* grad_dev) {
+    if (!h) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (!carr_dev || !target_dev || !grad_dev) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    Parament_ErrorCode err = ensure_common(h, amps);
+    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
+    return h->last_error = gradient(h, reinterpret_cast<const cplx<R>*>(carr_dev), dt, pts, amps,
+                                    reinterpret_cast<const cplx<R>*>(target_dev), fid_dev, grad_dev);
+}
+
+template <class R> Parament_ErrorCode set_checkpointing(Context<R>* h, unsigned seg, unsigned chunk) {
+    if (!h) return PARAMENT_STATUS_INVALID_VALUE;
+    h->ckpt_seg = seg;
+    h->ckpt_chunk = chunk;
+    return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
 
 template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, double* b) {
@@ -762,13 +852,15 @@ Parament_ErrorCode Parament_setKernelPath_fp64(Parament_Context_f64* handle, int
 Parament_ErrorCode Parament_setStream(Parament_Context_f32* handle, void* hip_stream) {
     if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
     auto* h = CTX32(handle);
-    h->stream = hip_stream ? (hipStream_t)hip_stream : h->own_stream;
+    h->stream = (hipStream_t)hip_stream;
+    h->user_stream = true;
     return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
 Parament_ErrorCode Parament_setStream_fp64(Parament_Context_f64* handle, void* hip_stream) {
     if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
     auto* h = CTX64(handle);
-    h->stream = hip_stream ? (hipStream_t)hip_stream : h->own_stream;
+    h->stream = (hipStream_t)hip_stream;
+    h->user_stream = true;
     return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
 void* Parament_getStream(Parament_Context_f32* handle) { return handle ? (void*)CTX32(handle)->stream : nullptr; }
@@ -791,24 +883,37 @@ Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64* handle, do
     return handle ? kernel_times(CTX64(handle), slice_ms, tree_ms) : PARAMENT_STATUS_INVALID_VALUE;
 }
 
+Parament_ErrorCode Parament_gateFidelityGradient(Parament_Context_f32* handle, const cuComplex* carr, double dt,
+                                               unsigned int pts, unsigned int amps, const cuComplex* target,
+                                               float* fidelity, float* grad) {
+    return gradient_host<float>(CTX32(handle), carr, dt, pts, amps, target, fidelity, grad);
+}
 Parament_ErrorCode Parament_gateFidelityGradient_fp64(Parament_Context_f64* handle, const cuDoubleComplex* carr, double dt,
                                                     unsigned int pts, unsigned int amps, const cuDoubleComplex* target,
                                                     double* fidelity, double* grad) {
-    return gradient_host(CTX64(handle), carr, dt, pts, amps, target, fidelity, grad);
+    return gradient_host<double>(CTX64(handle), carr, dt, pts, amps, target, fidelity, grad);
+}
+Parament_ErrorCode Parament_gateFidelityGradientDevice(Parament_Context_f32* handle, const cuComplex* carr_dev, double dt,
+                                                     unsigned int pts, unsigned int amps, const cuComplex* target_dev,
+                                                     float* fidelity_dev, float* grad_dev) {
+    return gradient_device<float>(CTX32(handle), carr_dev, dt, pts, amps, target_dev, fidelity_dev, grad_dev);
 }
 Parament_ErrorCode Parament_gateFidelityGradientDevice_fp64(Parament_Context_f64* handle, const cuDoubleComplex* carr_dev,
                                                           double dt, unsigned int pts, unsigned int amps,
                                                           const cuDoubleComplex* target_dev, double* fidelity_dev,
                                                           double* grad_dev) {
-    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
-    auto* h = CTX64(handle);
-    if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
-    if (!carr_dev || !target_dev || !grad_dev) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
-    Parament_ErrorCode err = ensure_common(h, amps);
-    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
-    return h->last_error = gradient(h, reinterpret_cast<const double2*>(carr_dev), dt, pts, amps,
-                                    reinterpret_cast<const double2*>(target_dev), fidelity_dev, grad_dev);
+    return gradient_device<double>(CTX64(handle), carr_dev, dt, pts, amps, target_dev, fidelity_dev, grad_dev);
 }
+Parament_ErrorCode Parament_setGradientCheckpointing(Parament_Context_f32* handle, unsigned int segment_slices,
+                                                     unsigned int chunk_slices) {
+    return set_checkpointing(CTX32(handle), segment_slices, chunk_slices);
+}
+Parament_ErrorCode Parament_setGradientCheckpointing_fp64(Parament_Context_f64* handle, unsigned int segment_slices,
+                                                          unsigned int chunk_slices) {
+    return set_checkpointing(CTX64(handle), segment_slices, chunk_slices);
+}
+size_t Parament_getGradientWorkspace(Parament_Context_f32* handle) { return handle ? CTX32(handle)->grad_workspace : 0; }
+size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64* handle) { return handle ? CTX64(handle)->grad_workspace : 0; }
 
 void device_info(void) {
     int n = 0;

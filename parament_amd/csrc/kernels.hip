@@ -27,7 +27,7 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void cheb_fused_kernel(ChebArgs<R
     const R s = a.s;
 
     for (int j = blockIdx.x; j < a.N; j += gridDim.x) {
-        for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
+        for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, a.j0 + j);
         __syncthreads();
         for (int e = tid; e < D * D; e += 64 * G) {
             const int i = e / D, k = e % D;

@@ -7,17 +7,19 @@ namespace pa {
 
 constexpr int kMaxCheb = 48;        // Chebyshev coefficients carried in kernel arguments (M <= 41)
 constexpr int kMaxCoeffs = 256;     // operator coefficients per slice (controls + Magnus terms)
+constexpr int kGradCoeffs = 64;     // controls per slice of the gradient kernels
 
 template <class R> struct ChebArgs {
     const cplx<R>* ops;    // (1 + ncoef) row-major d x d operators: H0, H_1.., Magnus terms
     int ncoef;             // operators after H0
     int d;
     Controls<R> ctl;
-    int N;                 // slices
+    int N;                 // slices computed by this launch: j0 .. j0 + N - 1
     int M;                 // Chebyshev order (odd, >= 3)
     R s;                   // 2 / Hnorm
     R are[kMaxCheb], aim[kMaxCheb];   // a_k = (-i)^k J_k(tau Hnorm); are[0] holds a_0 - 1
-    cplx<R>* out;          // N x d x d, E_j = U_j - I
+    cplx<R>* out;          // N x d x d, E_j = U_j - I (out[0] is slice j0)
+    int j0 = 0;            // first slice
 };
 
 // Mixed precision: FP64 operators / coefficients; output E_j = U_j - I in complex64.
@@ -60,16 +62,19 @@ template <class R> hipError_t launch_gemm(const cplx<R>* A, long sa, const cplx<
                                           hipStream_t st, bool add_ab = false);
 template <class R> hipError_t launch_zero(unsigned* p, size_t n, hipStream_t st);
 
-// Adjoint gate-fidelity gradient (grad.hip), complex128, dims <= 32.
+// Adjoint gate-fidelity gradient (grad.hip), dims <= 32.  Tree levels are stored as
+// consecutive d x d matrices; see grad.hip for the scan.
 bool grad_supported(int d);
-hipError_t launch_add_identity(double2* m, int d, int n, hipStream_t st);
-hipError_t launch_pair_level(const double2* in, int n_in, int d, double2* out, hipStream_t st);
-hipError_t launch_down_level(const double2* Vc, int n_child, int d, const double2* Pp, const double2* Lp, double2* Pc,
-                             double2* Lc, hipStream_t st);
-hipError_t launch_conj_transpose(const double2* t, int d, double2* out, hipStream_t st);
-hipError_t launch_grad_slices(const ChebArgs<double>& a, const double2* P, const double2* L, double2* dg, hipStream_t st);
-hipError_t launch_target_trace(const double2* t, const double2* u, int d, double2* out, hipStream_t st);
-hipError_t launch_grad_finish(const double2* dg, const double2* g, int N, int pts, int amps, int mode, int d, double* grad,
-                              double* fid, hipStream_t st);
+template <class R> hipError_t launch_up_level(const cplx<R>* in, int n_in, int d, cplx<R>* out, bool leaves_eform,
+                                              hipStream_t st);
+template <class R> hipError_t launch_down_level(const cplx<R>* V, int n_child, int d, const cplx<R>* Wp, cplx<R>* Wc,
+                                                hipStream_t st);
+template <class R> hipError_t launch_conj_transpose(const cplx<R>* t, int d, cplx<R>* out, hipStream_t st);
+template <class R> hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx<R>* W1,
+                                                 cplx<R>* dg, int dg_stride, hipStream_t st);
+template <class R> hipError_t launch_target_trace(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* out,
+                                                  hipStream_t st);
+template <class R> hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pts, int amps,
+                                                 int mode, int d, R* grad, R* fid, hipStream_t st);
 
 }  // namespace pa

@@ -229,11 +229,9 @@ class Parament:
     def gate_fidelity_gradient(self, dt, *carr, target):
         """F = |Tr(target^H U)|^2 / dim^2 and dF/dc for every control sample (adjoint method).
 
-        Complex128 ('fp64') contexts, real controls, dim <= 32, no Magnus.  Returns
-        ``(F, grad)`` with ``grad`` shaped like the stacked controls, ``(amps, pts)``.
+        Real controls, dim <= 32, no Magnus.  Returns ``(F, grad)`` with ``grad`` shaped like
+        the stacked controls, ``(amps, pts)``, in the context's precision.
         """
-        if not self._use_doubles:
-            raise ValueError("gradients need an 'fp64' context")
         if self.amps < 0:
             raise RuntimeError("No hamiltonian set")
         amps = len(carr)
@@ -245,26 +243,45 @@ class Parament:
         c = np.asarray(carr)
         if np.iscomplexobj(c) and np.any(c.imag != 0):
             raise ValueError("gradients need real control amplitudes")
-        c = np.ascontiguousarray(np.ravel(c.real.astype(np.complex128), order='C'))
-        t = np.ascontiguousarray(np.ravel(np.asarray(target, dtype=np.complex128), order='C'))
+        ctype = np.complex128 if self._use_doubles else np.complex64
+        rtype = np.float64 if self._use_doubles else np.float32
+        c = np.ascontiguousarray(np.ravel(c.real.astype(ctype), order='C'))
+        t = np.ascontiguousarray(np.ravel(np.asarray(target, dtype=ctype), order='C'))
         if t.size != self.dim ** 2:
             raise ValueError("target must be a dim x dim matrix")
-        fid = ctypes.c_double()
-        grad = np.zeros(amps * pts, dtype=np.float64)
-        self._check_error(self._lib.Parament_gateFidelityGradient_fp64(self._handle, c, float(dt), int(pts), int(amps),
-                                                                      t, ctypes.byref(fid), grad))
+        fid = ctypes.c_double() if self._use_doubles else ctypes.c_float()
+        grad = np.zeros(amps * pts, dtype=rtype)
+        self._check_error(self._call('Parament_gateFidelityGradient', self._handle, c, float(dt), int(pts), int(amps), t,
+                                     ctypes.byref(fid), grad))
         return fid.value, grad.reshape(amps, pts)
 
     @_must_be_alive
     def gate_fidelity_gradient_device(self, dt, carr_ptr, pts, amps, target_ptr, fid_ptr, grad_ptr):
-        """Device-pointer variant (asynchronous on the context's stream)."""
-        self._check_error(self._lib.Parament_gateFidelityGradientDevice_fp64(
-            self._handle, ctypes.c_void_p(carr_ptr), float(dt), int(pts), int(amps), ctypes.c_void_p(target_ptr),
-            ctypes.c_void_p(fid_ptr) if fid_ptr else None, ctypes.c_void_p(grad_ptr)))
+        """Device-pointer variant (asynchronous on the context's stream); reads the real parts of
+        the complex control array at ``carr_ptr``."""
+        self._check_error(self._call('Parament_gateFidelityGradientDevice', self._handle, ctypes.c_void_p(carr_ptr), float(dt),
+                                     int(pts), int(amps), ctypes.c_void_p(target_ptr),
+                                     ctypes.c_void_p(fid_ptr) if fid_ptr else None, ctypes.c_void_p(grad_ptr)))
+
+    @_must_be_alive
+    def set_gradient_checkpointing(self, segment=1, chunk=0):
+        """Gradient memory policy: ``segment=0`` keeps the whole product tree (default of a new
+        context); ``segment=1`` checkpoints at ~sqrt(N)-slice segments; larger values set the
+        segment length.  ``chunk`` is the number of slices recomputed per pass (0: automatic)."""
+        self._check_error(self._call('Parament_setGradientCheckpointing', self._handle, int(segment), int(chunk)))
+
+    @property
+    @_must_be_alive
+    def gradient_workspace(self):
+        """Device bytes the last gradient call worked in."""
+        return int(self._call('Parament_getGradientWorkspace', self._handle))
 
     @_must_be_alive
     def set_stream(self, stream_ptr):
-        """Run on the given HIP stream (an int handle, e.g. torch.cuda.current_stream().cuda_stream); None resets."""
+        """Run on the given HIP stream (an int handle, e.g. torch.cuda.current_stream().cuda_stream).
+
+        0 / None is the device's null stream (PyTorch's default stream); passing back a value
+        read from ``stream`` before restores the context's own stream."""
         self._check_error(self._call('Parament_setStream', self._handle, ctypes.c_void_p(stream_ptr or None)))
 
     @property

@@ -81,13 +81,18 @@ for sfx, cptr in (("", c_cuComplex_p), ("_fp64", c_cuDoubleComplex_p)):
     getattr(lib, "Parament_getKernelTimes" + sfx).argtypes = [c_ParamentContext_p, ctypes.POINTER(ctypes.c_double),
                                                               ctypes.POINTER(ctypes.c_double)]
 
-lib.Parament_gateFidelityGradient_fp64.argtypes = [c_ParamentContext_p, c_cuDoubleComplex_p, ctypes.c_double,
-                                                   ctypes.c_uint, ctypes.c_uint, c_cuDoubleComplex_p,
-                                                   ctypes.POINTER(ctypes.c_double),
-                                                   np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
-lib.Parament_gateFidelityGradientDevice_fp64.argtypes = [c_ParamentContext_p, ctypes.c_void_p, ctypes.c_double,
-                                                         ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
-                                                         ctypes.c_void_p, ctypes.c_void_p]
+for sfx, cptr, real in (("", c_cuComplex_p, np.float32), ("_fp64", c_cuDoubleComplex_p, np.float64)):
+    rptr = ctypes.POINTER(ctypes.c_float if real is np.float32 else ctypes.c_double)
+    getattr(lib, "Parament_gateFidelityGradient" + sfx).argtypes = [
+        c_ParamentContext_p, cptr, ctypes.c_double, ctypes.c_uint, ctypes.c_uint, cptr, rptr,
+        np.ctypeslib.ndpointer(real, flags="C_CONTIGUOUS")]
+    getattr(lib, "Parament_gateFidelityGradientDevice" + sfx).argtypes = [
+        c_ParamentContext_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_void_p]
+    getattr(lib, "Parament_setGradientCheckpointing" + sfx).argtypes = [c_ParamentContext_p, ctypes.c_uint,
+                                                                        ctypes.c_uint]
+    getattr(lib, "Parament_getGradientWorkspace" + sfx).argtypes = [c_ParamentContext_p]
+    getattr(lib, "Parament_getGradientWorkspace" + sfx).restype = ctypes.c_size_t
 lib.Parament_setPrecisionMode_fp64.argtypes = [c_ParamentContext_p, ctypes.c_int]
 lib.Parament_errorMessage.argtypes = [ctypes.c_int]
 lib.Parament_errorMessage.restype = ctypes.c_char_p
