@@ -11,7 +11,9 @@ whose propagator is within 1e-5 relative Frobenius error of the FP64 path's.
 ``roofline`` describes that path's Chebyshev slice kernel (the dominant kernel), timed
 with HIP events on the stream it runs on; ``cpu_baseline`` times the numpy FP64
 restatement of the reference algorithm (oracle/) on a bounded sample of the same
-workload on the host cores.
+workload on the host cores.  ``grape_grads_per_s`` is the rate of complete FP64 gradients
+(F and every dF/dc) on the GRAPE workload (dim 32, 4 controls, 4096 steps); ``grape`` holds
+both precisions, both memory layouts and the gradient / forward cost ratio.
 """
 from __future__ import annotations
 
@@ -71,6 +73,57 @@ def run_path(name, prec, mixed, h0, hk, c, dt, quad, steps, warmup):
     ctx.destroy()
     return {"name": name, "s_per_call": t, "u": u, "M": m, "slice_ms": float(np.median(ks)),
             "tree_ms": float(np.median(kt))}
+
+
+def grape_bench(steps, warmup):
+    """GRAPE workload: dim 32, 4 GUE controls, 4096 steps, dt 2e-3, Haar target.  Per precision:
+    wall time of one equipropDevice call (forward) and of one gateFidelityGradientDevice call
+    (F and all 4 x 4096 derivatives), with the whole product tree kept (default) and with
+    sqrt(N) checkpointing."""
+    import torch
+    import parament_amd as parament
+    from parament_amd import workloads
+    h0, hk, c, dt, quad, target = workloads.gue32_grape()
+    amps, pts = c.shape
+    d = h0.shape[0]
+    dev = torch.device("cuda:0")
+    res = {}
+    grads = {}
+    for prec, ct, rt in (("fp64", torch.complex128, torch.float64), ("fp32", torch.complex64, torch.float32)):
+        carr = torch.tensor(c, dtype=ct, device=dev).contiguous()
+        tgt = torch.tensor(target, dtype=ct, device=dev).contiguous()
+        out = torch.empty(d * d, dtype=ct, device=dev)
+        g = torch.empty((amps, pts), dtype=rt, device=dev)
+        f = torch.empty(1, dtype=rt, device=dev)
+        ctx = parament.Parament(prec)
+        ctx.set_hamiltonian(h0, *hk, quadrature_mode=quad)
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+
+        def timed(call):
+            for _ in range(warmup):
+                call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                call()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / steps
+
+        t_fwd = timed(lambda: ctx.equiprop_device(dt, carr.data_ptr(), pts, amps, out.data_ptr()))
+        entry = {"forward_ms": t_fwd * 1e3, "M": ctx.iteration_cycles}
+        for label, seg in (("full_tree", 0), ("sqrt_checkpoint", 1)):
+            ctx.set_gradient_checkpointing(seg, 0)
+            t_grad = timed(lambda: ctx.gate_fidelity_gradient_device(dt, carr.data_ptr(), pts, amps, tgt.data_ptr(),
+                                                                     f.data_ptr(), g.data_ptr()))
+            entry[label] = {"grad_ms": t_grad * 1e3, "grads_per_s": 1.0 / t_grad, "grad_over_forward": t_grad / t_fwd,
+                            "workspace_mb": ctx.gradient_workspace / 2 ** 20}
+        grads[prec] = g.double().cpu().numpy()
+        entry["fidelity"] = float(f.item())
+        res[prec] = entry
+        ctx.destroy()
+    res["fp32_grad_err_vs_fp64"] = float(np.abs(grads["fp32"] - grads["fp64"]).max() / np.abs(grads["fp64"]).max())
+    res["workload"] = "gue32_grape: dim 32, 4 controls, 4096 steps, dt 2e-3, no quadrature, Haar target"
+    return res
 
 
 def cpu_baseline(h0, hk, c, dt, quad, sample_pts):
@@ -137,8 +190,10 @@ def main():
                      "kernel": "Chebyshev slice kernel", "flops_per_launch": fl},
         "speedup_mixed_vs_fp64": paths["mixed"]["steps_per_s"] / paths["fp64"]["steps_per_s"],
         "paths": paths,
-        "grape_grads_per_s": None,
     }
+    grape = grape_bench(args.steps, args.warmup)
+    line["grape_grads_per_s"] = grape["fp64"]["full_tree"]["grads_per_s"]
+    line["grape"] = grape
     if not args.no_cpu:
         ucpu, cb = cpu_baseline(h0, hk, c, dt, quad, args.cpu_sample_pts)
         line["cpu_baseline"] = cb

@@ -179,10 +179,10 @@ template <> struct Pad<double> { static constexpr int A = 1, B = 8; };
 // Work split of one d x d product: D = 64 and D = 32 use 4 waves, each owning one quadrant
 // (32x32 or 16x16); D = 16 uses a single wave owning the whole matrix.
 template <int D> struct Shape {
-    static constexpr int G = (D == 16) ? 1 : 4;
-    static constexpr int T = (D == 64) ? 2 : 1;
-    static __device__ __forceinline__ int row0(int w) { return D == 16 ? 0 : (D / 2) * (w >> 1); }
-    static __device__ __forceinline__ int col0(int w) { return D == 16 ? 0 : (D / 2) * (w & 1); }
+    static constexpr int G = (D == 16) ? 1 : (D == 32 ? 4 : 16);
+    static constexpr int T = 1;
+    static __device__ __forceinline__ int row0(int w) { return D == 16 ? 0 : (D == 32 ? 16 * (w >> 1) : 16 * (w >> 2)); }
+    static __device__ __forceinline__ int col0(int w) { return D == 16 ? 0 : (D == 32 ? 16 * (w & 1) : 16 * (w & 3)); }
 };
 
 }  // namespace pa
