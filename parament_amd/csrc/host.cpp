@@ -123,7 +123,6 @@ template <class R> struct Context {
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     C* d_ops = nullptr;
-    double* d_opmax = nullptr;  // largest modulus of each operator (mixed path's fixed-point scale)
     // working memory (grow-only, like parament.cpp:418)
     C* d_c = nullptr; size_t cap_c = 0;
     void* d_leaves = nullptr; size_t cap_leaves = 0;
@@ -142,9 +141,7 @@ template <class R> struct Context {
 
 template <class R> void free_hamiltonian(Context<R>* h) {
     if (h->d_ops) hipFree(h->d_ops);
-    if (h->d_opmax) hipFree(h->d_opmax);
     h->d_ops = nullptr;
-    h->d_opmax = nullptr;
     h->has_h = false;
 }
 
@@ -282,25 +279,6 @@ Parament_ErrorCode set_hamiltonian(Context<R>* h, const typename ApiCplx<R>::typ
         free_hamiltonian(h);
         return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
     }
-    // per-operator largest modulus (the Magnus commutators are read back once): bounds
-    // |Re X_ik|, |Im X_ik| <= |H0|max + sum_t |c'_t| |O_t|max for the mixed path's scaling
-    {
-        std::vector<cplx<R>> hostops(nops * dd);
-        std::vector<double> opmax(nops, 0.0);
-        if (hipMemcpy(hostops.data(), h->d_ops, nops * dd * sizeof(cplx<R>), hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMalloc(&h->d_opmax, nops * sizeof(double)) != hipSuccess) {
-            hipGetLastError();
-            free_hamiltonian(h);
-            return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
-        }
-        for (size_t t = 0; t < nops; ++t)
-            for (size_t e = 0; e < dd; ++e)
-                opmax[t] = std::max(opmax[t], hypot((double)hostops[t * dd + e].x, (double)hostops[t * dd + e].y));
-        if (hipMemcpy(h->d_opmax, opmax.data(), nops * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-            free_hamiltonian(h);
-            return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
-        }
-    }
     h->has_h = true;
     return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
@@ -356,7 +334,6 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     if (mixed) {
         pa::MixedArgs ma;
         ma.ops = reinterpret_cast<const double2*>(h->d_ops);
-        ma.opmax = h->d_opmax;
         ma.ncoef = h->ncoef; ma.d = (int)d;
         ma.ctl.c = reinterpret_cast<const double2*>(carr_dev); ma.ctl.pts = (int)pts; ma.ctl.amps_in = (int)amps;
         ma.ctl.amps_set = (int)h->amps; ma.ctl.mode = h->quad; ma.ctl.magnus = h->magnus ? 1 : 0; ma.ctl.tau = tau;
@@ -369,8 +346,11 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         ma.N = N; ma.M = M; ma.s = (double)sf;
         for (int k = 0; k <= M; ++k) { ma.are[k] = bre[k]; ma.aim[k] = bim[k]; }
         ma.are[0] = bre[0] - 1.0;
-        const size_t nodes = (size_t)pa::tree_scratch_nodes<double>(N);
-        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)N * dd * sizeof(float2)) ||
+        // the slice kernel folds the lowest F tree levels; the FP64 tree takes its level-F nodes
+        const int F = pa::mixed_fused_levels(N);
+        const int n_nodes = (N + (1 << F) - 1) >> F;
+        const size_t nodes = (size_t)pa::tree_scratch_nodes<double>(n_nodes);
+        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)n_nodes * dd * sizeof(float2)) ||
             !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(double2)))
             return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
         ma.out = reinterpret_cast<float2*>(h->d_leaves);
@@ -378,8 +358,8 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         if (pa::launch_cheb_mixed(ma, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
         mark(h, 1);
         double2* nd = reinterpret_cast<double2*>(h->d_nodes);
-        if (pa::launch_tree<double>(h->d_leaves, true, N, (int)d, nd, nd + nodes * dd, reinterpret_cast<double2*>(out_dev),
-                                    h->stream) != hipSuccess)
+        if (pa::launch_tree<double>(h->d_leaves, true, n_nodes, (int)d, nd, nd + nodes * dd,
+                                    reinterpret_cast<double2*>(out_dev), h->stream) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         mark(h, 2);
         return PARAMENT_STATUS_SUCCESS;

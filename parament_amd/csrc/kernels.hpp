@@ -22,7 +22,8 @@ template <class R> struct ChebArgs {
     int j0 = 0;            // first slice
 };
 
-// Mixed precision: FP64 operators / coefficients; output E_j = U_j - I in complex64.
+// Mixed precision: FP64 operators / coefficients; output: the level-F nodes E = U - I of the
+// pairwise product tree (F = mixed_fused_levels(N)) in complex64.
 struct MixedArgs {
     const double2* ops;
     int ncoef;
@@ -32,14 +33,14 @@ struct MixedArgs {
     int M;
     double s;
     double are[kMaxCheb], aim[kMaxCheb];   // are[0] holds a_0 - 1
-    const double* opmax;   // largest modulus of each operator (1 + ncoef entries)
-    float2* out;           // N x d x d, E_j = U_j - I
+    float2* out;           // ceil(N / 2^F) x d x d
 };
 
 // Fused Chebyshev slice kernel, dims <= 64 (padded to 16 / 32 / 64).
 template <class R> hipError_t launch_cheb_fused(const ChebArgs<R>& a, hipStream_t st);
 hipError_t launch_cheb_mixed(const MixedArgs& a, hipStream_t st);
 bool mixed_supported(int d);
+int mixed_fused_levels(int N);   // tree levels the mixed slice kernel folds in
 
 // Pairwise product tree (dims <= 64) on deviations from identity, one launch per
 // block-reduction level.  leaves: N d x d matrices E_j = U_j - I, of cplx<R> or

@@ -1,100 +1,76 @@
-// Mixed-precision Chebyshev slice kernel (gfx950 bf16 matrix cores + FP64 first-order terms).
+// Mixed-precision Chebyshev slice kernel (gfx950 bf16 matrix cores + FP64 first-order terms),
+// with the lowest levels of the pairwise product tree fused in.
 //
 // The slice propagator is split as (parament.cpp:569-652 recurrence, rearranged exactly):
 //     U_j - I = (a_0 - 1) I + a_1 s X_j + [ s X_j b_1' - 2 b_2 ],   b_1' = s X_j b_2 - b_3
 // where b_k are the Clenshaw states of the same degree-M Chebyshev series (b_1 = b_1' + a_1 I).
 // The first two terms carry the O(tau |H|) part of the propagator; they are formed in FP64
 // from the FP64 slice Hamiltonian, because any rounding there recurs identically in every
-// slice and accumulates linearly over the product.  The bracket is O((tau |H|)^2) and is
-// evaluated by the recurrence in FP32 arithmetic whose GEMMs run on the bf16 matrix cores.
+// slice and accumulates linearly over the product.  For the same reason the diagonal
+// coefficients a_2, a_3 are added in FP64.  The bracket is O((tau |H|)^2) and is evaluated by
+// the recurrence in FP32 arithmetic whose GEMMs run on the bf16 matrix cores.
 //
-// Exact matrix-core accumulation: each GEMM operand is scaled by a power of two and split in
-// fixed point, x 2^(8-e) = s0 + s1 + s2 + r with s0 an integer (|s0| <= 256), s1 = r1 2^-8,
-// s2 = r2 2^-16 (|r1|, |r2| <= 128 integers), all exact in bf16 and |r| <= 2^-17.  Products
-// s0 t0 are integers below 2^16 and their sum over K = 64 (real and imaginary parts of the 4M
-// form combined) stays below 2^24: that accumulator is exact on the matrix cores regardless of
-// how the instruction rounds.  The five products with a non-leading slice (s0 t1, s1 t0, s0 t2,
-// s1 t1, s2 t0) share a second accumulator whose sum needs more than 24 bits only below
-// 2^-31 of the leading one, so whatever rounding the instruction applies stays there.  The two
-// accumulators are recombined with IEEE arithmetic: the remaining roundings are the operand
-// splits and the FP32 epilogues, all round-to-nearest.
-// The slice result E_j is stored as complex64 for the FP64 product tree.
+// Split products: every FP32 operand x is written as three bf16 numbers, s0 = bf16(x),
+// s1 = bf16(x - s0), s2 = bf16(x - s0 - s1) (round to nearest; both differences are exact in
+// FP32), so x = s0 + s1 + s2 within 2^-27 |x|.  A product is the sum of the six slice products
+// s_p t_q with p + q <= 2 (the three dropped ones are below 2^-26 of the leading one), each
+// exact in FP32 and accumulated by the matrix cores in FP32: the leading products in one
+// accumulator, the five smaller ones in a second, added at the end.  This is an FP32-accurate
+// complex GEMM at six bf16 products per real product.
+//
+// Tree levels: work-group w computes the slices [w 2^F, (w+1) 2^F) one after the other and
+// multiplies them on chip in the reference's pairing (later slice on the left), on deviations
+// from identity: (I + A)(I + B) = I + (A + B + A B), the product A B being a split product.  A
+// slice beyond N is a zero node, and combining with it returns the other node unchanged, as
+// the reference's odd-carry rule does.  The node E = U - I of level F is stored as complex64
+// for the FP64 levels above.
+//
+// Work split: one work-group per block of slices, one wave per 16x32 output tile (8 waves at
+// dim 64, two per SIMD inside the one-work-group-per-CU LDS budget; 2 waves at dim 32), so that
+// matrix-core, vector and LDS work of different waves overlap; the two 16x16 blocks of a wave
+// share each A-operand fragment, which keeps the LDS reads per matrix-core operation low.
 #include "kernels.hpp"
 
 namespace pa {
 
-typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 bool mixed_supported(int d) { return d >= 1 && d <= 64; }
 
+// Wave tiles of 16 x (16 TN): TN column blocks share each A fragment (register blocking).
 template <int D> struct MShape {
-    static constexpr int W = D / 32;         // wave tiles per side
-    static constexpr int G = W * W;          // waves per slice
+    static constexpr int TN = 2;             // 16-column blocks per wave
+    static constexpr int WR = D / 16;        // wave rows
+    static constexpr int WC = D / (16 * TN); // wave columns
+    static constexpr int G = WR * WC;        // waves per slice
     static constexpr int S = D + 8;          // plane row stride (bf16 elements), 16-byte rows
     static constexpr int PL = D * S;         // plane size
 };
 
-// Fixed-point split of a = x 2^(8-e) (|x| < 2^e, so |a| < 256) into three bf16 numbers with
-// a = s0 + s1 + s2 + O(2^-17): s0 = r0, s1 = r1 2^-8, s2 = r2 2^-16 with integers
-// |r0| <= 256, |r1|, |r2| <= 128 (each exactly representable in bf16).
-__device__ __forceinline__ void split_fixed(double x, double scale, __bf16& s0, __bf16& s1, __bf16& s2) {
-    const double a = x * scale;                 // |a| <= 256
-    const double r0 = rint(a);
-    const double b = (a - r0) * 256.0;          // exact, |b| <= 128
-    const double r1 = rint(b);
-    const double r2 = rint((b - r1) * 256.0);   // |r2| <= 128
-    s0 = (__bf16)(float)r0; s1 = (__bf16)(float)(r1 * (1.0 / 256.0)); s2 = (__bf16)(float)(r2 * (1.0 / 65536.0));
-}
-__device__ __forceinline__ void split_fixed(float x, float scale, __bf16& s0, __bf16& s1, __bf16& s2) {
-    const float a = x * scale;
-    const float r0 = rintf(a);
-    const float b = (a - r0) * 256.f;
-    const float r1 = rintf(b);
-    const float r2 = rintf((b - r1) * 256.f);
-    s0 = (__bf16)r0; s1 = (__bf16)(r1 * (1.f / 256.f)); s2 = (__bf16)(r2 * (1.f / 65536.f));
+// Three-way bf16 split of two FP32 values (one packed conversion per slice pair).
+__device__ __forceinline__ void split3(float x0, float x1, bf16x2& s0, bf16x2& s1, bf16x2& s2) {
+    s0[0] = (__bf16)x0; s0[1] = (__bf16)x1;
+    const float r0 = x0 - (float)s0[0], r1 = x1 - (float)s0[1];
+    s1[0] = (__bf16)r0; s1[1] = (__bf16)r1;
+    const float q0 = r0 - (float)s1[0], q1 = r1 - (float)s1[1];
+    s2[0] = (__bf16)q0; s2[1] = (__bf16)q1;
 }
 
-// Same split with the last slice rounded stochastically: r2 = floor(b + u), u uniform in
-// [0, 1), so the representation error has zero mean.  Used for the slice Hamiltonian, whose
-// constant entries would otherwise carry the same rounding error into every slice.
-__device__ __forceinline__ void split_fixed_sr(double x, double scale, float u, __bf16& s0, __bf16& s1, __bf16& s2) {
-    const double a = x * scale;
-    const double r0 = rint(a);
-    const double b = (a - r0) * 256.0;
-    const double r1 = rint(b);
-    const double r2 = floor((b - r1) * 256.0 + (double)u);
-    s0 = (__bf16)(float)r0; s1 = (__bf16)(float)(r1 * (1.0 / 256.0)); s2 = (__bf16)(float)(r2 * (1.0 / 65536.0));
-}
+// Accumulator element v of a 16x16x32 tile: row 4*(lane/16) + v, column lane%16.
+__device__ __forceinline__ int mrow(int row0, int v) { return row0 + 4 * ((threadIdx.x & 63) >> 4) + v; }
+__device__ __forceinline__ int mcol(int col0, int n) { return col0 + 16 * n + (threadIdx.x & 15); }
 
-// Reproducible 32 random bits from (slice, element pair): an integer hash.  Each byte gives
-// one dither value (k + 1/2) / 256, k uniform in 0..255 (mean 1/2: unbiased rounding).
-__device__ __forceinline__ unsigned hash_bits(unsigned j, unsigned e) {
-    unsigned h = j * 0x9E3779B1u ^ (e * 0x85EBCA77u + 0xC2B2AE3Du);
-    h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
-    return h;
-}
-
-// Exponent e with m < 2^e for the largest modulus m of a matrix (0 for a zero matrix).
-__device__ __forceinline__ int scale_exp(double m) {
-    int e = 0;
-    if (m > 0) frexp(m, &e);   // m = f 2^e, f in [0.5, 1)
-    return e;                   // |x| <= m < 2^e
-}
-
-// Accumulator element v of a 32x32x16 tile: row 8*(v/4) + 4*(lane/32) + v%4, column lane%32.
-__device__ __forceinline__ int mrow(int row0, int v) { return row0 + 8 * (v >> 2) + 4 * ((threadIdx.x & 63) >> 5) + (v & 3); }
-__device__ __forceinline__ int mcol(int col0) { return col0 + (threadIdx.x & 31); }
-
-struct MTile {
-    f32x16 re, im;
+template <int TN> struct MTile {
+    f32x4v re[TN], im[TN];
 };
 
-// Accumulators of one 32x32 complex tile: [0] = leading-slice products (exact), [1] = the
-// products involving a non-leading slice (carry their own 2^-8 / 2^-16 scales).
-struct LTile {
-    f32x16 re[2], im[2];
+// Accumulators of the wave's complex tiles: [L][n], L = 0 leading-slice products, L = 1 the
+// products involving a smaller slice.
+template <int TN> struct LTile {
+    f32x4v re[2][TN], im[2][TN];
 };
 
 __device__ __forceinline__ bf16x8 negate(bf16x8 v) {
@@ -104,224 +80,371 @@ __device__ __forceinline__ bf16x8 negate(bf16x8 v) {
     return __builtin_bit_cast(bf16x8, u);
 }
 
-// C = X * B on the wave's 32x32 tile with exact level accumulation.  X slice planes are
-// row-major (A operand), B slice planes transposed (Bt[col][k]); plane p = 2 * slice + {0: re, 1: im}.
+// C = A * B on the wave's 16 x 16TN tile.  A slice planes are row-major, B slice planes
+// transposed (Bt[col][k]); plane 2 * slice + {0: re, 1: im}.  Operand fragments of the
+// 16x16x32 instruction: lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7].
 template <int D>
-__device__ __forceinline__ void wave_cgemm_exact(const __bf16* __restrict__ Xp, const __bf16* __restrict__ Bp,
-                                                 int row0, int col0, LTile& c) {
-    constexpr int S = MShape<D>::S, PL = MShape<D>::PL;
+__device__ __forceinline__ void wave_cgemm_split(const __bf16* __restrict__ Ap, const __bf16* __restrict__ Bp,
+                                                 int row0, int col0, LTile<MShape<D>::TN>& c) {
+    constexpr int S = MShape<D>::S, PL = MShape<D>::PL, TN = MShape<D>::TN;
     const int lane = threadIdx.x & 63;
-    const int r = lane & 31, kh = lane >> 5;
+    const int r = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int L = 0; L < 2; ++L)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) { c.re[L][v] = 0.f; c.im[L][v] = 0.f; }
+        for (int n = 0; n < TN; ++n)
 #pragma unroll
-    for (int ks = 0; ks < D / 16; ++ks) {
-        const int ka = (row0 + r) * S + 16 * ks + 8 * kh;
-        const int kb = (col0 + r) * S + 16 * ks + 8 * kh;
-        bf16x8 Ar[3], Ai[3], An[3], Br[3], Bi[3];
+            for (int v = 0; v < 4; ++v) { c.re[L][n][v] = 0.f; c.im[L][n][v] = 0.f; }
+#pragma unroll 1
+    for (int ks = 0; ks < D / 32; ++ks) {
+        const int ka = (row0 + r) * S + 32 * ks + 8 * kq;
+        bf16x8 Br[TN][3], Bi[TN][3];
+#pragma unroll
+        for (int n = 0; n < TN; ++n) {
+            const int kb = (col0 + 16 * n + r) * S + 32 * ks + 8 * kq;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                Br[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q) * PL + kb);
+                Bi[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q + 1) * PL + kb);
+            }
+        }
+        // A slice p against the B slices q <= 2 - p (slice pairs with p + q <= 2)
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-            Ar[p] = *reinterpret_cast<const bf16x8*>(Xp + (2 * p) * PL + ka);
-            Ai[p] = *reinterpret_cast<const bf16x8*>(Xp + (2 * p + 1) * PL + ka);
-            An[p] = negate(Ai[p]);
-            Br[p] = *reinterpret_cast<const bf16x8*>(Bp + (2 * p) * PL + kb);
-            Bi[p] = *reinterpret_cast<const bf16x8*>(Bp + (2 * p + 1) * PL + kb);
-        }
-        // slice pairs (p, q) with p + q = L <= 2
+            const bf16x8 Ar = *reinterpret_cast<const bf16x8*>(Ap + (2 * p) * PL + ka);
+            const bf16x8 Ai = *reinterpret_cast<const bf16x8*>(Ap + (2 * p + 1) * PL + ka);
+            const bf16x8 An = negate(Ai);
 #pragma unroll
-        for (int t = 0; t < 6; ++t) {
-            const int p = (t == 0) ? 0 : (t == 1) ? 0 : (t == 2) ? 1 : (t == 3) ? 0 : (t == 4) ? 1 : 2;
-            const int q = (t == 0) ? 0 : (t == 1) ? 1 : (t == 2) ? 0 : (t == 3) ? 2 : (t == 4) ? 1 : 0;
-            const int L = (p + q) > 0 ? 1 : 0;
-            c.re[L] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ar[p], Br[q], c.re[L], 0, 0, 0);
-            c.im[L] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ar[p], Bi[q], c.im[L], 0, 0, 0);
-            c.re[L] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(An[p], Bi[q], c.re[L], 0, 0, 0);
-            c.im[L] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ai[p], Br[q], c.im[L], 0, 0, 0);
+            for (int n = 0; n < TN; ++n)
+#pragma unroll
+                for (int q = 0; q + p <= 2; ++q) {
+                    const int L = (p + q) > 0 ? 1 : 0;
+                    c.re[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Br[n][q], c.re[L][n], 0, 0, 0);
+                    c.im[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Bi[n][q], c.im[L][n], 0, 0, 0);
+                    c.re[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(An, Bi[n][q], c.re[L][n], 0, 0, 0);
+                    c.im[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, Br[n][q], c.im[L][n], 0, 0, 0);
+                }
         }
     }
 }
 
-template <int D>
+// Tile (accumulator layout) -> split planes: transposed (B operand, Bt[col][row]) or
+// row-major (A operand, A[row][col]).  Each lane holds rows 4(l/16) .. +3 of one column per
+// block.
+template <int D, bool TRANSPOSED>
+__device__ __forceinline__ void write_split(const MTile<MShape<D>::TN>& b, __bf16* __restrict__ P, int row0, int col0) {
+    constexpr int S = MShape<D>::S, PL = MShape<D>::PL, TN = MShape<D>::TN;
+    const int rbase = mrow(row0, 0);
+#pragma unroll
+    for (int n = 0; n < TN; ++n) {
+        const int col = mcol(col0, n);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const f32x4v& x = c ? b.im[n] : b.re[n];
+            bf16x2 h0, m0, l0, h1, m1, l1;
+            split3(x[0], x[1], h0, m0, l0);
+            split3(x[2], x[3], h1, m1, l1);
+            if (TRANSPOSED) {
+                const bf16x4 h = {h0[0], h0[1], h1[0], h1[1]}, m = {m0[0], m0[1], m1[0], m1[1]},
+                             l = {l0[0], l0[1], l1[0], l1[1]};
+                *reinterpret_cast<bf16x4*>(P + (0 + c) * PL + col * S + rbase) = h;
+                *reinterpret_cast<bf16x4*>(P + (2 + c) * PL + col * S + rbase) = m;
+                *reinterpret_cast<bf16x4*>(P + (4 + c) * PL + col * S + rbase) = l;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int o = (rbase + q) * S + col;
+                    const bf16x2& hh = (q < 2) ? h0 : h1;
+                    const bf16x2& mm = (q < 2) ? m0 : m1;
+                    const bf16x2& ll = (q < 2) ? l0 : l1;
+                    P[(0 + c) * PL + o] = hh[q & 1];
+                    P[(2 + c) * PL + o] = mm[q & 1];
+                    P[(4 + c) * PL + o] = ll[q & 1];
+                }
+            }
+        }
+    }
+}
+
+// A slice coefficient in two halves, so that its control samples can be loaded long before
+// the value is needed: load the samples, then form exactly what slice_coefficient() returns
+// (non-Magnus modes).
+struct RawCoef {
+    double2 c0, c1, c2;
+};
+__device__ __forceinline__ RawCoef coef_load(const Controls<double>& q, int t, int j) {
+    RawCoef r;
+    if (q.mode == QUAD_NONE) { r.c0 = sample(q, t, j); r.c1 = r.c2 = mk<double>(0, 0); }
+    else if (q.mode == QUAD_MIDPOINT) { r.c0 = sample(q, t, j); r.c1 = sample(q, t, j + 1); r.c2 = mk<double>(0, 0); }
+    else { r.c0 = sample(q, t, 2 * j); r.c1 = sample(q, t, 2 * j + 1); r.c2 = sample(q, t, 2 * j + 2); }
+    return r;
+}
+__device__ __forceinline__ double2 coef_finish(const Controls<double>& q, const RawCoef& r) {
+    if (q.mode == QUAD_NONE) return r.c0;
+    if (q.mode == QUAD_MIDPOINT) {
+        const double2 s = cadd(r.c0, r.c1);
+        return mk<double>(0.5 * s.x, 0.5 * s.y);
+    }
+    const double2 s = cadd(cadd(r.c0, mk<double>(4.0 * r.c1.x, 4.0 * r.c1.y)), r.c2);
+    const double sixth = 1.0 / 6.0;
+    return mk<double>(s.x * sixth, s.y * sixth);
+}
+
+template <int D, int F>
 __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs a) {
-    constexpr int G = MShape<D>::G, S = MShape<D>::S, PL = MShape<D>::PL, W = MShape<D>::W;
+    constexpr int G = MShape<D>::G, S = MShape<D>::S, PL = MShape<D>::PL, WC = MShape<D>::WC;
+    constexpr int TN = MShape<D>::TN;
+    constexpr int NB = 1 << F;
+    typedef MTile<TN> Tile;
     __shared__ __attribute__((aligned(16))) __bf16 Xp[6 * PL];
     __shared__ __attribute__((aligned(16))) __bf16 Bp[6 * PL];
+    __shared__ float2 Rx[D * (D + 2)];   // X_j - (s0 + s1 + s2), the FP64 residual of the split
     __shared__ double2 coef[kMaxCoeffs];
-    __shared__ float red[G];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kh = lane >> 5;
-    const int row0 = 32 * (wave / W), col0 = 32 * (wave % W);
+    constexpr int SR = D + 2;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int row0 = 16 * (wave / WC), col0 = 16 * TN * (wave % WC);
     const int d = a.d, dd = d * d, M = a.M;
     const float sf = (float)a.s;  // exact: the host picks Hnorm so that 2/Hnorm is a float
+    const int n_nodes = (a.N + NB - 1) >> F;
 
-    auto x64 = [&](int i, int k) -> double2 {
-        double2 x = a.ops[i * d + k];
-        for (int t = 0; t < a.ncoef; ++t) x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + i * d + k]));
-        return x;
-    };
-    // b (accumulator layout, FP32) -> transposed slice planes Bt[col][row], scaled by the
-    // work-group max published in red[] before the caller's barrier; returns the exponent.
-    auto wave_max = [&](const MTile& b) {
-        float m = 0.f;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) m = fmaxf(m, fmaxf(fabsf(b.re[v]), fabsf(b.im[v])));
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        if (lane == 0) red[wave] = m;
-    };
-    auto write_b = [&](const MTile& b) -> int {
-        float m = red[0];
-        for (int w = 1; w < G; ++w) m = fmaxf(m, red[w]);
-        const int eb = scale_exp((double)m);
-        const float sc = ldexpf(1.f, 8 - eb);   // maps |b| < 2^eb into |.| < 256
-        const int col = mcol(col0);
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int rbase = row0 + 8 * g + 4 * kh;
-            bf16x4 h[2], m1[2], l[2];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                __bf16 t0, t1, t2;
-                split_fixed(b.re[4 * g + q], sc, t0, t1, t2);
-                h[0][q] = t0; m1[0][q] = t1; l[0][q] = t2;
-                split_fixed(b.im[4 * g + q], sc, t0, t1, t2);
-                h[1][q] = t0; m1[1][q] = t1; l[1][q] = t2;
-            }
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                *reinterpret_cast<bf16x4*>(Bp + (0 + c) * PL + col * S + rbase) = h[c];
-                *reinterpret_cast<bf16x4*>(Bp + (2 + c) * PL + col * S + rbase) = m1[c];
-                *reinterpret_cast<bf16x4*>(Bp + (4 + c) * PL + col * S + rbase) = l[c];
-            }
-        }
-        return eb;
-    };
-    for (int j = blockIdx.x; j < a.N; j += gridDim.x) {
-        for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
+    // Coefficients of the next slice are prefetched during the current one when every
+    // coefficient has its own thread and the mode has no Magnus terms.
+    const bool prefetch = !a.ctl.magnus && a.ncoef <= 64 * G;
+    bool coef_ready = false;   // coef[] already holds the coefficients of the slice about to run
+
+    // E_j = U_j - I of slice j into e (accumulator layout, FP32); jn is the slice this
+    // work-group runs next (or -1).
+    auto slice = [&](int j, int jn, Tile& e) {
+        if (!coef_ready)
+            for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
         __syncthreads();
-        // X_j in FP64, split in fixed point against the bound |X| <= |H0| + sum_t |c'_t| |O_t|
-        // (per-operator largest moduli from the host), last slice rounded stochastically.
-        double bound = a.opmax[0];
-        for (int t = 0; t < a.ncoef; ++t) bound += hypot(coef[t].x, coef[t].y) * a.opmax[t + 1];
-        const int ex = scale_exp(bound);
-        const double xsc = ldexp(1.0, 8 - ex);
-        typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-        for (int e2 = tid; e2 < D * D / 2; e2 += 64 * G) {
+        // X_j = H0 + sum_t c'_t O_t in FP64: each thread issues the operator loads of an element
+        // pair together before the arithmetic, so their latencies overlap.
+#pragma unroll 1
+        for (int it = 0; it < D * D / 2 / (64 * G); ++it) {
+            const int e2 = tid + it * 64 * G;
             const int i = (2 * e2) / D, k = (2 * e2) % D;   // elements (i, k), (i, k + 1)
+            const bool v0 = i < d && k < d, v1 = i < d && k + 1 < d;
             double2 x0 = mk<double>(0, 0), x1 = mk<double>(0, 0);
-            if (i < d && k < d) x0 = x64(i, k);
-            if (i < d && k + 1 < d) x1 = x64(i, k + 1);
-            const unsigned hsh = hash_bits(j, e2);
-            const float u[4] = {(float)(hsh & 0xFFu) * (1.f / 256.f) + (1.f / 512.f),
-                                (float)((hsh >> 8) & 0xFFu) * (1.f / 256.f) + (1.f / 512.f),
-                                (float)((hsh >> 16) & 0xFFu) * (1.f / 256.f) + (1.f / 512.f),
-                                (float)(hsh >> 24) * (1.f / 256.f) + (1.f / 512.f)};
+            if (v0) x0 = a.ops[i * d + k];
+            if (v1) x1 = a.ops[i * d + k + 1];
+            for (int t0 = 0; t0 < a.ncoef; t0 += 4) {
+                double2 o0[4], o1[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const size_t base = (size_t)(t0 + q + 1) * dd + i * d + k;
+                    o0[q] = (v0 && t0 + q < a.ncoef) ? a.ops[base] : mk<double>(0, 0);
+                    o1[q] = (v1 && t0 + q < a.ncoef) ? a.ops[base + 1] : mk<double>(0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (t0 + q < a.ncoef) {
+                        x0 = cadd(x0, cmul(coef[t0 + q], o0[q]));
+                        x1 = cadd(x1, cmul(coef[t0 + q], o1[q]));
+                    }
+            }
             bf16x2 p[6];
-            __bf16 s0, s1, s2;
-            split_fixed_sr(x0.x, xsc, u[0], s0, s1, s2); p[0][0] = s0; p[2][0] = s1; p[4][0] = s2;
-            split_fixed_sr(x0.y, xsc, u[1], s0, s1, s2); p[1][0] = s0; p[3][0] = s1; p[5][0] = s2;
-            split_fixed_sr(x1.x, xsc, u[2], s0, s1, s2); p[0][1] = s0; p[2][1] = s1; p[4][1] = s2;
-            split_fixed_sr(x1.y, xsc, u[3], s0, s1, s2); p[1][1] = s0; p[3][1] = s1; p[5][1] = s2;
+            split3((float)x0.x, (float)x1.x, p[0], p[2], p[4]);
+            split3((float)x0.y, (float)x1.y, p[1], p[3], p[5]);
 #pragma unroll
             for (int q = 0; q < 6; ++q) *reinterpret_cast<bf16x2*>(Xp + q * PL + i * S + k) = p[q];
+            // residuals for the FP64 first-order term (the sum of the three slices is exact in FP64)
+            auto res = [&](double x, int h, int c) -> float {
+                return (float)(x - (((double)(float)p[4 + c][h] + (double)(float)p[2 + c][h]) + (double)(float)p[c][h]));
+            };
+            Rx[i * SR + k] = make_float2(res(x0.x, 0, 0), res(x0.y, 0, 1));
+            Rx[i * SR + k + 1] = make_float2(res(x1.x, 1, 0), res(x1.y, 1, 1));
         }
         __syncthreads();
+        // coef[] is free from here on: start loading the next slice's control samples
+        const bool pre = prefetch && jn >= 0 && jn < a.N;
+        RawCoef raw;
+        if (pre && tid < a.ncoef) raw = coef_load(a.ctl, tid, jn);
 
-        MTile b1, b2;
-        LTile acc;
+        Tile b1, b2;
+        LTile<TN> acc;
         {
             const float aMr = (float)a.are[M], aMi = (float)a.aim[M];
             const float aM1r = (float)a.are[M - 1], aM1i = (float)a.aim[M - 1];
-            const float xinv = ldexpf(1.f, ex - 8);
-            const int col = mcol(col0);
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int row = mrow(row0, v);
-                const bool dg = row == col;
-                const int o = row * S + col;
-                const float xr = (((float)Xp[4 * PL + o] + (float)Xp[2 * PL + o]) + (float)Xp[o]) * xinv;
-                const float xi = (((float)Xp[5 * PL + o] + (float)Xp[3 * PL + o]) + (float)Xp[PL + o]) * xinv;
-                b2.re[v] = dg ? aMr : 0.f;
-                b2.im[v] = dg ? aMi : 0.f;
-                // s a_M X + a_{M-1} I
-                b1.re[v] = sf * (aMr * xr - aMi * xi) + (dg ? aM1r : 0.f);
-                b1.im[v] = sf * (aMr * xi + aMi * xr) + (dg ? aM1i : 0.f);
+            for (int n = 0; n < TN; ++n) {
+                const int col = mcol(col0, n);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = mrow(row0, v);
+                    const bool dg = row == col;
+                    const int o = row * S + col;
+                    const float xr = ((float)Xp[4 * PL + o] + (float)Xp[2 * PL + o]) + (float)Xp[o];
+                    const float xi = ((float)Xp[5 * PL + o] + (float)Xp[3 * PL + o]) + (float)Xp[PL + o];
+                    b2.re[n][v] = dg ? aMr : 0.f;
+                    b2.im[n][v] = dg ? aMi : 0.f;
+                    // s a_M X + a_{M-1} I
+                    b1.re[n][v] = sf * (aMr * xr - aMi * xi) + (dg ? aM1r : 0.f);
+                    b1.im[n][v] = sf * (aMr * xi + aMi * xr) + (dg ? aM1i : 0.f);
+                }
             }
         }
-        wave_max(b1);
-        __syncthreads();
-        int eb = write_b(b1);
+        write_split<D, true>(b1, Bp, row0, col0);
         __syncthreads();
 
-        // b_k = s X b_{k+1} - b_{k+2} + a_k I for k = M-2 .. 2, then b_1' (no a_1 I).
-        // For k = 2, 3 the diagonal update adds a_k in FP64: a rounded a_k would put the same
-        // error into every slice (|a_2|, |a_3| ~ (tau |H|)^2 / 8, (tau |H|)^3 / 48); for k >= 4 the
-        // FP32 rounding of a_k is below 2^-24 |a_4| and negligible over any product length.
-        for (int k = M - 2; k >= 1; --k) {
-            wave_cgemm_exact<D>(Xp, Bp, row0, col0, acc);
-            const float sc = sf * ldexpf(1.f, ex + eb - 16);   // s times the operand scales
-            const int col = mcol(col0);
+        // b_k = s X b_{k+1} - b_{k+2} + a_k I for k = M-2 .. 2, then b_1' (no a_1 I).  One step:
+        // bc = b_{k+1} (staged in Bp), bp = b_{k+2} -> b_k, staged for the next step.
+        auto step = [&](const Tile& bc, Tile& bp, int k) {
+            wave_cgemm_split<D>(Xp, Bp, row0, col0, acc);
             if (k == 2 || k == 3) {
                 const double akr = a.are[k], aki = a.aim[k];
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const float cr = sc * (acc.re[0][v] + acc.re[1][v]);
-                    const float ci = sc * (acc.im[0][v] + acc.im[1][v]);
-                    float nr = cr - b2.re[v], ni = ci - b2.im[v];
-                    if (mrow(row0, v) == col) {
-                        nr = (float)((double)cr - (double)b2.re[v] + akr);
-                        ni = (float)((double)ci - (double)b2.im[v] + aki);
+                for (int n = 0; n < TN; ++n)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const float cr = sf * (acc.re[0][n][v] + acc.re[1][n][v]);
+                        const float ci = sf * (acc.im[0][n][v] + acc.im[1][n][v]);
+                        float nr = cr - bp.re[n][v], ni = ci - bp.im[n][v];
+                        if (mrow(row0, v) == mcol(col0, n)) {
+                            nr = (float)((double)cr - (double)bp.re[n][v] + akr);
+                            ni = (float)((double)ci - (double)bp.im[n][v] + aki);
+                        }
+                        bp.re[n][v] = nr; bp.im[n][v] = ni;
                     }
-                    b2.re[v] = b1.re[v]; b2.im[v] = b1.im[v];
-                    b1.re[v] = nr; b1.im[v] = ni;
-                }
             } else {
                 const float akr = (k >= 2) ? (float)a.are[k] : 0.f, aki = (k >= 2) ? (float)a.aim[k] : 0.f;
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const bool dg = mrow(row0, v) == col;
-                    const float nr = sc * (acc.re[0][v] + acc.re[1][v]) - b2.re[v] + (dg ? akr : 0.f);
-                    const float ni = sc * (acc.im[0][v] + acc.im[1][v]) - b2.im[v] + (dg ? aki : 0.f);
-                    b2.re[v] = b1.re[v]; b2.im[v] = b1.im[v];
-                    b1.re[v] = nr; b1.im[v] = ni;
-                }
-            }
-            wave_max(b1);
-            __syncthreads();
-            eb = write_b(b1);
-            __syncthreads();
-        }
-        // R = s X b_1' - 2 b_2 ;  E = (a_0 - 1) I + a_1 s X + R  (FP64)
-        wave_cgemm_exact<D>(Xp, Bp, row0, col0, acc);
-        {
-            const double a0m1 = a.are[0];
-            const double2 a1s = mk<double>(a.are[1] * a.s, a.aim[1] * a.s);
-            const double f = ldexp(1.0, ex + eb - 16);
-            float2* out = a.out + (size_t)j * dd;
-            const int col = mcol(col0);
+                for (int n = 0; n < TN; ++n)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
+                    for (int v = 0; v < 4; ++v) {
+                        const bool dg = mrow(row0, v) == mcol(col0, n);
+                        bp.re[n][v] = sf * (acc.re[0][n][v] + acc.re[1][n][v]) - bp.re[n][v] + (dg ? akr : 0.f);
+                        bp.im[n][v] = sf * (acc.im[0][n][v] + acc.im[1][n][v]) - bp.im[n][v] + (dg ? aki : 0.f);
+                    }
+            }
+            __syncthreads();
+            write_split<D, true>(bp, Bp, row0, col0);
+            __syncthreads();
+        };
+        // M is odd: the M - 2 steps are pairs k, k - 1 (k = M-2, M-4, .., 3) and a last k = 1.
+        for (int k = M - 2; k >= 2; k -= 2) {
+            step(b1, b2, k);        // b2 <- b_k
+            step(b2, b1, k - 1);    // b1 <- b_{k-1}
+        }
+        step(b1, b2, 1);            // b2 <- b_1', b1 = b_2
+        if (pre && tid < a.ncoef) coef[tid] = coef_finish(a.ctl, raw);
+        coef_ready = pre;
+        // R = s X b_1' - 2 b_2 ;  E = (a_0 - 1) I + a_1 s X + R  (FP64)
+        wave_cgemm_split<D>(Xp, Bp, row0, col0, acc);
+        const double a0m1 = a.are[0];
+        const double2 a1s = mk<double>(a.are[1] * a.s, a.aim[1] * a.s);
+#pragma unroll
+        for (int n = 0; n < TN; ++n) {
+            const int col = mcol(col0, n);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
                 const int row = mrow(row0, v);
+                double er = 0.0, ei = 0.0;
                 if (row < d && col < d) {
-                    const double cr = ((double)acc.re[0][v] + (double)acc.re[1][v]) * f;
-                    const double ci = ((double)acc.im[0][v] + (double)acc.im[1][v]) * f;
-                    const double rr = a.s * cr - 2.0 * (double)b2.re[v];
-                    const double ri = a.s * ci - 2.0 * (double)b2.im[v];
-                    const double2 t = cmul(a1s, x64(row, col));
-                    out[row * d + col] = make_float2((float)(t.x + rr + (row == col ? a0m1 : 0.0)), (float)(t.y + ri));
+                    const double cr = (double)acc.re[0][n][v] + (double)acc.re[1][n][v];
+                    const double ci = (double)acc.im[0][n][v] + (double)acc.im[1][n][v];
+                    // X_j in FP64 from its split and residual
+                    const int o = row * S + col;
+                    const float2 rx = Rx[row * SR + col];
+                    const double xr = (((double)(float)Xp[4 * PL + o] + (double)(float)Xp[2 * PL + o]) +
+                                       (double)(float)Xp[o]) + (double)rx.x;
+                    const double xi = (((double)(float)Xp[5 * PL + o] + (double)(float)Xp[3 * PL + o]) +
+                                       (double)(float)Xp[PL + o]) + (double)rx.y;
+                    const double2 t = cmul(a1s, mk<double>(xr, xi));
+                    er = t.x + (a.s * cr - 2.0 * (double)b1.re[n][v]) + (row == col ? a0m1 : 0.0);
+                    ei = t.y + (a.s * ci - 2.0 * (double)b1.im[n][v]);
                 }
+                e.re[n][v] = (float)er;
+                e.im[n][v] = (float)ei;
             }
         }
         __syncthreads();
+    };
+
+    // out = A + B + A B for the later node A and the earlier node B (split product).
+    auto combine = [&](const Tile& A, const Tile& B, Tile& out) {
+        write_split<D, false>(A, Xp, row0, col0);
+        write_split<D, true>(B, Bp, row0, col0);
+        __syncthreads();
+        LTile<TN> acc;
+        wave_cgemm_split<D>(Xp, Bp, row0, col0, acc);
+#pragma unroll
+        for (int n = 0; n < TN; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                out.re[n][v] = (A.re[n][v] + B.re[n][v]) + (acc.re[0][n][v] + acc.re[1][n][v]);
+                out.im[n][v] = (A.im[n][v] + B.im[n][v]) + (acc.im[0][n][v] + acc.im[1][n][v]);
+            }
+        __syncthreads();
+    };
+
+    for (int w = blockIdx.x; w < n_nodes; w += gridDim.x) {
+        Tile stk[F > 0 ? F : 1];
+        Tile node;
+#pragma unroll
+        for (int p = 0; p < NB; ++p) {
+            const int j = w * NB + p;
+            if (j >= a.N) break;   // the rest of the block are zero nodes
+            Tile e;
+            // next slice of this work-group: j + 1 inside the block, else the first of its next block
+            const int jn = (p + 1 < NB && j + 1 < a.N) ? j + 1 : ((w + (int)gridDim.x) * NB < a.N ? (w + (int)gridDim.x) * NB : -1);
+            slice(j, jn, e);
+            // binary counter over p: merge completed siblings (later node on the left)
+            int l = 0;
+#pragma unroll
+            for (; l < F; ++l) {
+                if (((p >> l) & 1) == 0) { stk[l] = e; break; }
+                Tile m;
+                combine(e, stk[l], m);
+                e = m;
+            }
+            if (l == F) node = e;
+        }
+        // a ragged block: the pending nodes stk[l], l a set bit of the live count, fold from the
+        // lowest level up (each is earlier than everything folded before it)
+        const int live = min(a.N - w * NB, NB);
+        if (live < NB) {
+            int have = 0;
+#pragma unroll
+            for (int l = 0; l < F; ++l) {
+                if ((live >> l) & 1) {
+                    if (!have) { node = stk[l]; have = 1; }
+                    else { Tile m; combine(node, stk[l], m); node = m; }
+                }
+            }
+        }
+        float2* out = a.out + (size_t)w * dd;
+#pragma unroll
+        for (int n = 0; n < TN; ++n) {
+            const int col = mcol(col0, n);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = mrow(row0, v);
+                if (row < d && col < d) out[row * d + col] = make_float2(node.re[n][v], node.im[n][v]);
+            }
+        }
     }
 }
+
+int mixed_fused_levels(int N) { return N >= 4096 ? 3 : (N >= 1024 ? 2 : (N >= 256 ? 1 : 0)); }
 
 hipError_t launch_cheb_mixed(const MixedArgs& a, hipStream_t st) {
     if (a.d <= 0 || a.d > 64 || a.N <= 0 || a.M < 3 || a.M >= kMaxCheb || a.ncoef > kMaxCoeffs)
         return hipErrorInvalidValue;
-    if (a.d <= 32) hipLaunchKernelGGL((cheb_mixed_kernel<32>), dim3(a.N), dim3(64 * MShape<32>::G), 0, st, a);
-    else hipLaunchKernelGGL((cheb_mixed_kernel<64>), dim3(a.N), dim3(64 * MShape<64>::G), 0, st, a);
+    const int F = mixed_fused_levels(a.N);
+    const int grid = (a.N + (1 << F) - 1) >> F;
+    auto go = [&](auto kern, int g) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * g), 0, st, a); };
+    if (a.d <= 32) {
+        if (F == 3) go(cheb_mixed_kernel<32, 3>, MShape<32>::G);
+        else if (F == 2) go(cheb_mixed_kernel<32, 2>, MShape<32>::G);
+        else if (F == 1) go(cheb_mixed_kernel<32, 1>, MShape<32>::G);
+        else go(cheb_mixed_kernel<32, 0>, MShape<32>::G);
+    } else {
+        if (F == 3) go(cheb_mixed_kernel<64, 3>, MShape<64>::G);
+        else if (F == 2) go(cheb_mixed_kernel<64, 2>, MShape<64>::G);
+        else if (F == 1) go(cheb_mixed_kernel<64, 1>, MShape<64>::G);
+        else go(cheb_mixed_kernel<64, 0>, MShape<64>::G);
+    }
     return hipGetLastError();
 }
 

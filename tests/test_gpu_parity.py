@@ -183,6 +183,17 @@ def test_mixed_parity(mode, magnus, d):
     assert rel_err(u, ref) < 1e-5
 
 
+@pytest.mark.parametrize("d,n", [(d, n) for d in (8, 64) for n in (255, 256, 257, 259, 1024, 1025, 1026, 1027, 1029)]
+                         + [(8, n) for n in (4096, 4097, 4098, 4099, 4101, 4103, 4105)] + [(64, 4099), (64, 4103)])
+def test_mixed_fused_tree_levels(d, n):
+    """The mixed kernel folds the lowest tree levels (1 from 256 slices, 2 from 1024, 3 from
+    4096); ragged slice counts exercise the zero-node padding of the last block."""
+    h0, hks, c = case(1100 + n + d, d, 2, n)
+    u, m = run_gpu("fp64", h0, hks, c, 0.01, "none", mixed=True)
+    ref = eo.equiprop(h0, hks, c, 0.01, mode="none", cycles=m)
+    assert rel_err(u, ref) < 1e-5
+
+
 @pytest.mark.parametrize("cycles", [3, 5, 9])
 def test_mixed_manual_cycles(cycles):
     h0, hks, c = case(900 + cycles, 16, 1, 40)
