@@ -121,14 +121,21 @@ LIBSPEC double OneNorm_fp64(cuDoubleComplex *mat, unsigned int dim);
 /* Arithmetic of a complex128 context's equiprop (for callers that want the fast
  * mixed-precision path behind the fp64 interface):
  *   PARAMENT_PRECISION_NATIVE: every step in FP64 (default; the reference algorithm).
- *   PARAMENT_PRECISION_MIXED:  FP64 for Hnorm, M (fp32 table), the Bessel
- *       coefficients, the slice Hamiltonians, the first-order Chebyshev terms and the
- *       whole product tree; the higher-order Chebyshev remainder runs as bf16
- *       three-way split products with FP32 accumulation on the matrix cores.
- *       Available for dim <= 64; larger dims run NATIVE. */
+ *   PARAMENT_PRECISION_MIXED:  FP64 for Hnorm, M (fp32 table plus an accumulated-truncation
+ *       bound), the Bessel coefficients, the slice Hamiltonians, the first-order Chebyshev
+ *       terms and the upper levels of the product tree; the higher-order Chebyshev remainder
+ *       and the lowest three tree levels run as bf16 three-way split products with FP32
+ *       accumulation on the matrix cores (FP32-accurate products).
+ *   PARAMENT_PRECISION_MIXED_FP32:   the same with FP32 matrix-core products.
+ *   PARAMENT_PRECISION_MIXED_BF16X2: the same with bf16 hi+lo (two-way split) products
+ *       (about 16 significant bits; for precision studies).
+ * MIXED runs the fused on-chip kernel for dim <= 64 and batched GEMMs above; the other two
+ * always run the batched GEMMs. */
 typedef enum Parament_PrecisionMode {
     PARAMENT_PRECISION_NATIVE = 0,
-    PARAMENT_PRECISION_MIXED = 1
+    PARAMENT_PRECISION_MIXED = 1,
+    PARAMENT_PRECISION_MIXED_FP32 = 2,
+    PARAMENT_PRECISION_MIXED_BF16X2 = 3
 } Parament_PrecisionMode;
 
 LIBSPEC Parament_ErrorCode Parament_setPrecisionMode_fp64(Parament_Context_f64 *handle, Parament_PrecisionMode mode);

@@ -120,14 +120,15 @@ template <class R, int TM, int TN> struct CTile {
     }
 };
 
-// C(tile at row0, col0) = A * B over K = KD, operands in LDS planes with row strides SA, SB.
-template <class R, int TM, int TN, int KD, int SA, int SB>
+// C(tile at row0, col0) = A * B over K = KD (ZERO), or C += A * B (!ZERO), operands in LDS
+// planes with row strides SA, SB.
+template <class R, int TM, int TN, int KD, int SA, int SB, bool ZERO = true>
 __device__ __forceinline__ void wave_cgemm(const R* __restrict__ Ar, const R* __restrict__ Ai,
                                            const R* __restrict__ Br, const R* __restrict__ Bi,
                                            int row0, int col0, CTile<R, TM, TN>& c) {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, q = lane >> 4;
-    c.zero();
+    if (ZERO) c.zero();
 #pragma unroll 4
     for (int k0 = 0; k0 < KD; k0 += 4) {
         R ar[TM], ai[TM], an[TM], br[TN], bi[TN];

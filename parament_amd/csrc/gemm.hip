@@ -1,0 +1,271 @@
+// Batched complex GEMM on the gfx950 matrix cores for the step-wise (any-dim) path:
+//     C_b = alpha A_b B_b + beta Cin_b + diag I  (+ A_b + B_b when add_ab)
+// over a batch of d x d row-major complex matrices, with the arithmetic of the products set by
+// the mode: FP64 (16x16x4 f64 instruction), FP32 (16x16x4 f32), or bf16 split products
+// (16x16x32 bf16; every FP32 operand is split into 2 or 3 bf16 slices, s0 = bf16(x),
+// s1 = bf16(x - s0), s2 = bf16(x - s0 - s1), and the slice products with p + q < slices are
+// accumulated in FP32).  The epilogue is evaluated in FP64 and rounded once to the output type,
+// so a Clenshaw step's a_k I is added exactly.  Used for the Chebyshev steps
+// (parament.cpp:569-652) and the product-tree levels (parament.cpp:657-718) at dims where a
+// slice does not fit on chip.
+//
+// Tiling: a work-group computes a 64x64 output tile with 4 waves of 32x32 (2x2 blocks of
+// 16x16); K advances in chunks of 32 staged in LDS (zero-padded past d).  Work items (batch b,
+// tile t) are assigned so that every XCD receives a contiguous range of them: the tiles of one
+// slice then share that XCD's L2 for their A rows and B columns.
+#include "kernels.hpp"
+
+namespace pa {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4g __attribute__((ext_vector_type(4)));
+
+constexpr int BT = 64;   // output tile
+constexpr int KT = 32;   // K chunk
+
+// Work item of this block: XCD x (= blockIdx.x mod 8) takes items [x * per, (x + 1) * per).
+__device__ __forceinline__ long work_item(long total) {
+    const long nb = gridDim.x, b = blockIdx.x;
+    const long per = nb / 8, rem = nb % 8;     // blocks per XCD (the first `rem` XCDs get one more)
+    const long x = b % 8, i = b / 8;
+    const long start = x * per + (x < rem ? x : rem);
+    const long w = start + i;
+    return w < total ? w : -1;
+}
+
+template <class T> __device__ __forceinline__ double2 ld(const T* p, long i);
+template <> __device__ __forceinline__ double2 ld<double2>(const double2* p, long i) { return p[i]; }
+template <> __device__ __forceinline__ double2 ld<float2>(const float2* p, long i) {
+    const float2 v = p[i];
+    return mk<double>(v.x, v.y);
+}
+__device__ __forceinline__ void st(double2* p, long i, double2 v) { p[i] = v; }
+__device__ __forceinline__ void st(float2* p, long i, double2 v) { p[i] = make_float2((float)v.x, (float)v.y); }
+
+template <class TA, class TB, class TC, class TO>
+struct EpiIO {
+    const TA* A; const TB* B; const TC* Cin; TO* C;
+};
+
+// FP64 / FP32 matrix-core tiles: A chunk [64][32] (stride SA), B chunk [32][64] (stride SB).
+template <class R, class TA, class TB, class TC, class TO>
+__global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
+    constexpr int SA = KT + Pad<R>::A, SB = BT + Pad<R>::B;
+    __shared__ R Ar[BT * SA], Ai[BT * SA], Br[KT * SB], Bi[KT * SB];
+    const int d = g.d, tiles = (d + BT - 1) / BT;
+    const long total = (long)g.batch * tiles * tiles;
+    const long w = work_item(total);
+    if (w < 0) return;
+    const long b = w / (tiles * tiles);
+    const int t = (int)(w % (tiles * tiles)), ti = t / tiles, tj = t % tiles;
+    const long dd = (long)d * d;
+    const TA* A = io.A + b * g.sa * dd;
+    const TB* B = io.B + b * g.sb * dd;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int wr = 32 * (wave >> 1), wc = 32 * (wave & 1);
+    CTile<R, 2, 2> acc;
+    acc.zero();
+    for (int k0 = 0; k0 < d; k0 += KT) {
+        for (int e = tid; e < BT * KT; e += 256) {
+            const int r = e / KT, k = e % KT;
+            const int gi = ti * BT + r, gk = k0 + k;
+            double2 v = mk<double>(0, 0);
+            if (gi < d && gk < d) v = ld(A, (long)gi * d + gk);
+            Ar[r * SA + k] = (R)v.x; Ai[r * SA + k] = (R)v.y;
+        }
+        for (int e = tid; e < KT * BT; e += 256) {
+            const int k = e / BT, c = e % BT;
+            const int gk = k0 + k, gj = tj * BT + c;
+            double2 v = mk<double>(0, 0);
+            if (gk < d && gj < d) v = ld(B, (long)gk * d + gj);
+            Br[k * SB + c] = (R)v.x; Bi[k * SB + c] = (R)v.y;
+        }
+        __syncthreads();
+        wave_cgemm<R, 2, 2, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc);
+        __syncthreads();
+    }
+    // epilogue in FP64
+    const long cb = b * g.so * dd, ib = b * g.sc * dd;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = ti * BT + acc_row<R>(wr, m, v), gj = tj * BT + acc_col(wc, n);
+                if (gi >= d || gj >= d) continue;
+                const long o = (long)gi * d + gj;
+                const double xr = acc.re[m][n][v], xi = acc.im[m][n][v];
+                double2 y = mk<double>(g.alpha_re * xr - g.alpha_im * xi, g.alpha_re * xi + g.alpha_im * xr);
+                if (io.Cin) {
+                    const double2 c = ld(io.Cin, ib + o);
+                    y.x += g.beta_re * c.x - g.beta_im * c.y;
+                    y.y += g.beta_re * c.y + g.beta_im * c.x;
+                }
+                if (g.add_ab) {
+                    const double2 a = ld(A, o), bb = ld(B, o);
+                    y.x += a.x + bb.x; y.y += a.y + bb.y;
+                }
+                if (gi == gj) { y.x += g.diag_re; y.y += g.diag_im; }
+                st(io.C, cb + o, y);
+            }
+}
+
+// bf16 split products: A chunk planes [slice][re/im][64][32 + 8] row-major, B chunk planes
+// transposed [slice][re/im][64 cols][32 + 8].  Operand fragments of the 16x16x32 instruction:
+// lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7]; accumulator element v is
+// C[4(l/16) + v][l%16].
+template <int NS, class TA, class TB, class TC, class TO>
+__global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
+    constexpr int S = KT + 8, PL = BT * S;
+    __shared__ __attribute__((aligned(16))) __bf16 Ap[2 * NS * PL];
+    __shared__ __attribute__((aligned(16))) __bf16 Bp[2 * NS * PL];
+    const int d = g.d, tiles = (d + BT - 1) / BT;
+    const long total = (long)g.batch * tiles * tiles;
+    const long w = work_item(total);
+    if (w < 0) return;
+    const long b = w / (tiles * tiles);
+    const int t = (int)(w % (tiles * tiles)), ti = t / tiles, tj = t % tiles;
+    const long dd = (long)d * d;
+    const TA* A = io.A + b * g.sa * dd;
+    const TB* B = io.B + b * g.sb * dd;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wr = 32 * (wave >> 1), wc = 32 * (wave & 1);
+    const int r16 = lane & 15, kq = lane >> 4;
+    // acc[L][m][n]: L = 0 leading products, 1 the rest
+    f32x4g accr[2][2][2], acci[2][2][2];
+#pragma unroll
+    for (int L = 0; L < 2; ++L)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) { accr[L][m][n][v] = 0.f; acci[L][m][n][v] = 0.f; }
+    auto split_store = [&](__bf16* P, int o, float x, int c) {
+        float r = x;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const __bf16 h = (__bf16)r;
+            P[(2 * s + c) * PL + o] = h;
+            r -= (float)h;
+        }
+    };
+    for (int k0 = 0; k0 < d; k0 += KT) {
+        for (int e = tid; e < BT * KT; e += 256) {
+            const int r = e / KT, k = e % KT;
+            const int gi = ti * BT + r, gk = k0 + k;
+            double2 v = mk<double>(0, 0);
+            if (gi < d && gk < d) v = ld(A, (long)gi * d + gk);
+            split_store(Ap, r * S + k, (float)v.x, 0);
+            split_store(Ap, r * S + k, (float)v.y, 1);
+        }
+        for (int e = tid; e < KT * BT; e += 256) {
+            const int k = e / BT, c = e % BT;
+            const int gk = k0 + k, gj = tj * BT + c;
+            double2 v = mk<double>(0, 0);
+            if (gk < d && gj < d) v = ld(B, (long)gk * d + gj);
+            split_store(Bp, c * S + k, (float)v.x, 0);
+            split_store(Bp, c * S + k, (float)v.y, 1);
+        }
+        __syncthreads();
+        bf16x8 br[2][NS], bi[2][NS];
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                const int o = (wc + 16 * n + r16) * S + 8 * kq;
+                br[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q) * PL + o);
+                bi[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q + 1) * PL + o);
+            }
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int p = 0; p < NS; ++p) {
+                const int o = (wr + 16 * m + r16) * S + 8 * kq;
+                const bf16x8 ar = *reinterpret_cast<const bf16x8*>(Ap + (2 * p) * PL + o);
+                const bf16x8 ai = *reinterpret_cast<const bf16x8*>(Ap + (2 * p + 1) * PL + o);
+                typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 u = __builtin_bit_cast(u32x4, ai);
+                u ^= 0x80008000u;
+                const bf16x8 an = __builtin_bit_cast(bf16x8, u);
+#pragma unroll
+                for (int n = 0; n < 2; ++n)
+#pragma unroll
+                    for (int q = 0; q + p < NS; ++q) {
+                        const int L = (p + q) > 0 ? 1 : 0;
+                        accr[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br[n][q], accr[L][m][n], 0, 0, 0);
+                        acci[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi[n][q], acci[L][m][n], 0, 0, 0);
+                        accr[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(an, bi[n][q], accr[L][m][n], 0, 0, 0);
+                        acci[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br[n][q], acci[L][m][n], 0, 0, 0);
+                    }
+            }
+        __syncthreads();
+    }
+    const long cb = b * g.so * dd, ib = b * g.sc * dd;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = ti * BT + wr + 16 * m + 4 * kq + v, gj = tj * BT + wc + 16 * n + r16;
+                if (gi >= d || gj >= d) continue;
+                const long o = (long)gi * d + gj;
+                const double xr = (double)accr[0][m][n][v] + (double)accr[1][m][n][v];
+                const double xi = (double)acci[0][m][n][v] + (double)acci[1][m][n][v];
+                double2 y = mk<double>(g.alpha_re * xr - g.alpha_im * xi, g.alpha_re * xi + g.alpha_im * xr);
+                if (io.Cin) {
+                    const double2 c = ld(io.Cin, ib + o);
+                    y.x += g.beta_re * c.x - g.beta_im * c.y;
+                    y.y += g.beta_re * c.y + g.beta_im * c.x;
+                }
+                if (g.add_ab) {
+                    const double2 a = ld(A, o), bb = ld(B, o);
+                    y.x += a.x + bb.x; y.y += a.y + bb.y;
+                }
+                if (gi == gj) { y.x += g.diag_re; y.y += g.diag_im; }
+                st(io.C, cb + o, y);
+            }
+}
+
+template <class TA, class TB, class TC, class TO>
+hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
+    EpiIO<TA, TB, TC, TO> io;
+    io.A = static_cast<const TA*>(g.A); io.B = static_cast<const TB*>(g.B);
+    io.Cin = static_cast<const TC*>(g.Cin); io.C = static_cast<TO*>(g.C);
+    const int tiles = (g.d + BT - 1) / BT;
+    const long total = (long)g.batch * tiles * tiles;
+    if (total <= 0) return hipSuccess;
+    if (total > 0x7fffffffL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)total), block(256);
+    switch (mode) {
+        case GEMM_F64: hipLaunchKernelGGL((zgemm_fp_kernel<double, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_F32: hipLaunchKernelGGL((zgemm_fp_kernel<float, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_BF16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_BF16X3: hipLaunchKernelGGL((zgemm_split_kernel<3, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Operand / output element types: complex128 when the flag is set, complex64 otherwise; Cin
+// has the output's type.  FP64 products take complex64 operands (the transition from
+// low-precision tree levels) and complex128 ones; the FP32 and split modes run on complex64.
+hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st) {
+    if (g.d <= 0 || g.batch < 0) return hipErrorInvalidValue;
+    if (mode == GEMM_F64) {
+        if (!g.out_c128) return hipErrorInvalidValue;
+        if (g.a_c128 && g.b_c128) return launch_typed<double2, double2, double2, double2>(g, mode, st);
+        if (!g.a_c128 && !g.b_c128) return launch_typed<float2, float2, double2, double2>(g, mode, st);
+        return hipErrorInvalidValue;
+    }
+    if (g.a_c128 || g.b_c128 || g.out_c128) return hipErrorInvalidValue;
+    return launch_typed<float2, float2, float2, float2>(g, mode, st);
+}
+
+}  // namespace pa

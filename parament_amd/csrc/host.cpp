@@ -166,6 +166,31 @@ bool grow(void** p, size_t* cap, size_t bytes) {
     return true;
 }
 
+// Batched complex GEMM on the matrix cores (gemm.hip): C = alpha A B + beta Cin + diag I
+// (+ A + B with add_ab); pointers with their element type (true = complex128), strides in
+// matrices.
+struct GemmOp { const void* p; bool c128; long stride; };
+hipError_t zgemm(int mode, GemmOp A, GemmOp B, GemmOp Cin, void* C, bool c_c128, long so, double2 alpha, double2 beta,
+                 double2 diag, int d, int batch, hipStream_t st, bool add_ab = false) {
+    pa::GemmArgs g;
+    g.A = A.p; g.sa = A.stride; g.B = B.p; g.sb = B.stride; g.Cin = Cin.p; g.sc = Cin.stride; g.C = C; g.so = so;
+    g.alpha_re = alpha.x; g.alpha_im = alpha.y; g.beta_re = beta.x; g.beta_im = beta.y;
+    g.diag_re = diag.x; g.diag_im = diag.y;
+    g.d = d; g.batch = batch; g.add_ab = add_ab ? 1 : 0;
+    g.a_c128 = A.c128; g.b_c128 = B.c128; g.out_c128 = c_c128;
+    return pa::launch_zgemm(g, mode, st);
+}
+// The same in the context's own precision (FP64 products for complex128, FP32 for complex64).
+template <class R>
+hipError_t zgemm_native(const cplx<R>* A, long sa, const cplx<R>* B, long sb, const cplx<R>* Cin, long sc, cplx<R>* C,
+                        long so, double2 alpha, double2 beta, double2 diag, int d, int batch, hipStream_t st,
+                        bool add_ab = false) {
+    const bool w = sizeof(R) == 8;
+    return zgemm(w ? pa::GEMM_F64 : pa::GEMM_F32, GemmOp{A, w, sa}, GemmOp{B, w, sb}, GemmOp{Cin, w, sc}, C, w, so, alpha,
+                 beta, diag, d, batch, st, add_ab);
+}
+inline double2 c2(double re, double im = 0.0) { return pa::mk<double>(re, im); }
+
 template <class R> Parament_ErrorCode bind_device(Context<R>* h) {
     if (h->bound) {
         if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
@@ -213,11 +238,10 @@ template <class R> hipError_t magnus_commutators(Context<R>* h) {
     const unsigned d = h->dim, A = h->amps;
     const size_t dd = (size_t)d * d;
     C* ops = h->d_ops;
-    const C one = pa::mk<R>(1, 0), mone = pa::mk<R>(-1, 0), zero = pa::mk<R>(0, 0);
     auto comm = [&](const C* X, const C* Y, C* out) -> hipError_t {
-        hipError_t e = pa::launch_gemm<R>(X, 0, Y, 0, nullptr, 0, out, 0, one, zero, zero, d, 1, h->stream);
+        hipError_t e = zgemm_native<R>(X, 0, Y, 0, nullptr, 0, out, 0, c2(1), c2(0), c2(0), d, 1, h->stream);
         if (e != hipSuccess) return e;
-        return pa::launch_gemm<R>(Y, 0, X, 0, out, 0, out, 0, mone, one, zero, d, 1, h->stream);
+        return zgemm_native<R>(Y, 0, X, 0, out, 0, out, 0, c2(-1), c2(1), c2(0), d, 1, h->stream);
     };
     for (unsigned k = 0; k < A; ++k) {
         hipError_t e = comm(ops, ops + (1 + k) * dd, ops + (1 + A + k) * dd);
@@ -299,6 +323,108 @@ template <class R> void mark(Context<R>* h, int i) {
     hipEventRecord(h->ev[i], h->stream);
 }
 
+// Mixed-precision step-wise path (complex128 interface, any dim): the Chebyshev GEMMs and the
+// lowest tree levels run on complex64 data with the products of the selected mode (bf16 three-
+// way split, FP32, or bf16 hi+lo), every GEMM epilogue in FP64; the first-order part
+// (a_0 - 1) I + a_1 s X_j is formed in FP64, and the tree levels above the lowest three in FP64.
+int gemm_mode_of(int precision) {
+    switch (precision) {
+        case PARAMENT_PRECISION_MIXED_FP32: return pa::GEMM_F32;
+        case PARAMENT_PRECISION_MIXED_BF16X2: return pa::GEMM_BF16X2;
+        default: return pa::GEMM_BF16X3;
+    }
+}
+
+constexpr int kLowTreeLevels = 3;   // tree levels run in the low-precision products
+
+Parament_ErrorCode run_mixed_stepwise(Context<double>* h, const double2* carr_dev, int N, double tau, int M,
+                                      unsigned pts, unsigned amps, double2* out_dev) {
+    const int d = (int)h->dim;
+    const size_t dd = (size_t)d * d;
+    const int mode = gemm_mode_of(h->precision);
+    hipStream_t st = h->stream;
+    double are[pa::kMaxCheb], aim[pa::kMaxCheb];
+    bessel(M, tau * h->hnorm, are, aim);
+    pa::ChebArgs<double> ca;
+    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = d; ca.N = N; ca.M = M; ca.s = 2.0 / h->hnorm;
+    ca.ctl.c = carr_dev; ca.ctl.pts = (int)pts; ca.ctl.amps_in = (int)amps; ca.ctl.amps_set = (int)h->amps;
+    ca.ctl.mode = h->quad; ca.ctl.magnus = h->magnus ? 1 : 0; ca.ctl.tau = tau;
+    for (int k = 0; k <= M; ++k) { ca.are[k] = are[k]; ca.aim[k] = aim[k]; }
+    ca.are[0] = are[0] - 1.0;
+    ca.out = nullptr;
+    // complex64 working arrays inside the (complex128-sized) step-wise buffers
+    float2* X = reinterpret_cast<float2*>(h->d_gen[0]);
+    float2* b1 = reinterpret_cast<float2*>(h->d_gen[1]);
+    float2* b2 = reinterpret_cast<float2*>(h->d_gen[2]);
+    const float2 aM = make_float2((float)are[M], (float)aim[M]);
+    const float2 saM = make_float2((float)(ca.s * are[M]), (float)(ca.s * aim[M]));
+    mark(h, 0);
+    if (pa::launch_assemble_c64(ca, X, st) != hipSuccess ||
+        pa::launch_fill<float>(nullptr, make_float2(0, 0), aM, d, N, b2, st) != hipSuccess ||
+        pa::launch_fill<float>(X, saM, make_float2((float)are[M - 1], (float)aim[M - 1]), d, N, b1, st) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    // b_k = s X b_{k+1} - b_{k+2} + a_k I, k = M-2 .. 2, then b_1' (no a_1 I)
+    for (int k = M - 2; k >= 1; --k) {
+        const double2 diag = (k >= 2) ? c2(are[k], aim[k]) : c2(0);
+        if (zgemm(mode, GemmOp{X, false, 1}, GemmOp{b1, false, 1}, GemmOp{b2, false, 1}, b2, false, 1, c2(ca.s), c2(-1),
+                  diag, d, N, st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        std::swap(b1, b2);
+    }
+    // R = s X b_1' - 2 b_2 (into b2), then E = (a_0 - 1) I + a_1 s X + R in FP64 (into b1)
+    if (zgemm(mode, GemmOp{X, false, 1}, GemmOp{b1, false, 1}, GemmOp{b2, false, 1}, b2, false, 1, c2(ca.s), c2(-2), c2(0),
+              d, N, st) != hipSuccess ||
+        pa::launch_first_order(ca, b2, b1, st) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    mark(h, 1);
+    // lowest tree levels on complex64 (E' = A + B + A B), ping-pong b1 <-> X
+    float2* rd = b1;
+    float2* wr = X;
+    int r = N;
+    for (int l = 0; l < kLowTreeLevels && r > 1; ++l) {
+        const int half = r / 2;
+        if (zgemm(mode, GemmOp{rd + dd, false, 2}, GemmOp{rd, false, 2}, GemmOp{nullptr, false, 0}, wr, false, 1, c2(1), c2(0),
+                  c2(0), d, half, st, true) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        if (r % 2 && hipMemcpyAsync(wr + half * dd, rd + (size_t)(r - 1) * dd, dd * sizeof(float2), hipMemcpyDeviceToDevice,
+                                    st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        r = half + (r % 2);
+        std::swap(rd, wr);
+    }
+    // FP64 levels above: the first one reads the complex64 nodes
+    const size_t n64 = (size_t)(r + 1) / 2;
+    if (!grow(&h->d_nodes, &h->cap_nodes, 2 * std::max<size_t>(n64, 1) * dd * sizeof(double2)))
+        return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    double2* cur = reinterpret_cast<double2*>(h->d_nodes);
+    double2* nxt = cur + std::max<size_t>(n64, 1) * dd;
+    if (r == 1) {
+        if (pa::launch_widen(rd, d, 1, cur, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    } else {
+        const int half = r / 2;
+        if (zgemm(pa::GEMM_F64, GemmOp{rd + dd, false, 2}, GemmOp{rd, false, 2}, GemmOp{nullptr, true, 0}, cur, true, 1, c2(1),
+                  c2(0), c2(0), d, half, st, true) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        if (r % 2 && pa::launch_widen(rd + (size_t)(r - 1) * dd, d, 1, cur + half * dd, st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        r = half + (r % 2);
+        while (r > 1) {
+            const int hf = r / 2;
+            if (zgemm(pa::GEMM_F64, GemmOp{cur + dd, true, 2}, GemmOp{cur, true, 2}, GemmOp{nullptr, true, 0}, nxt, true, 1,
+                      c2(1), c2(0), c2(0), d, hf, st, true) != hipSuccess)
+                return PARAMENT_STATUS_CUBLAS_FAILED;
+            if (r % 2 && hipMemcpyAsync(nxt + hf * dd, cur + (size_t)(r - 1) * dd, dd * sizeof(double2),
+                                        hipMemcpyDeviceToDevice, st) != hipSuccess)
+                return PARAMENT_STATUS_CUBLAS_FAILED;
+            r = hf + (r % 2);
+            std::swap(cur, nxt);
+        }
+    }
+    if (pa::launch_fill<double>(cur, c2(1), c2(1), d, 1, out_dev, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    mark(h, 2);
+    return PARAMENT_STATUS_SUCCESS;
+}
+
 // Runs the whole pipeline on the context's stream; carr_dev / out_dev are device pointers.
 template <class R>
 Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps, cplx<R>* out_dev) {
@@ -308,7 +434,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     slices(h, pts, dt, &N, &tau);
     if (N <= 0) return PARAMENT_STATUS_INVALID_VALUE;
 
-    const bool mixed_req = (sizeof(R) == 8) && h->precision == PARAMENT_PRECISION_MIXED;
+    const bool mixed_req = (sizeof(R) == 8) && h->precision != PARAMENT_PRECISION_NATIVE;
     int M = h->mmax;
     if (!h->manual) {
         M = (sizeof(R) == 4 || mixed_req) ? select_fp32(h->hnorm, tau) : select_fp64(h->hnorm, tau);
@@ -325,7 +451,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     const unsigned d = h->dim;
     const size_t dd = (size_t)d * d;
     const bool fused = (d <= 64) && h->path == 0;
-    const bool mixed = mixed_req && fused && pa::mixed_supported((int)d);
+    const bool mixed = mixed_req && h->precision == PARAMENT_PRECISION_MIXED && fused && pa::mixed_supported((int)d);
 
     pa::Controls<R> ctl;
     ctl.c = carr_dev; ctl.pts = (int)pts; ctl.amps_in = (int)amps; ctl.amps_set = (int)h->amps;
@@ -388,7 +514,7 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         return PARAMENT_STATUS_SUCCESS;
     }
 
-    // Step-wise path: X, then M-1 GEMM launches, then ceil(log2 N) tree levels.
+    // Step-wise path: X, then M-1 batched GEMM launches, then ceil(log2 N) tree levels.
     const size_t gen_bytes = (size_t)N * dd * sizeof(C);
     if (h->cap_gen < gen_bytes) {
         for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
@@ -401,22 +527,27 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
             }
         h->cap_gen = gen_bytes;
     }
+    hipStream_t st = h->stream;
+    if constexpr (sizeof(R) == 8) {
+        if (mixed_req)
+            return run_mixed_stepwise(h, reinterpret_cast<const double2*>(carr_dev), N, tau, M, pts, amps,
+                                      reinterpret_cast<double2*>(out_dev));
+    }
     C* X = h->d_gen[0];
     C* b1 = h->d_gen[1];
     C* b2 = h->d_gen[2];
-    const C zero = pa::mk<R>(0, 0), one = pa::mk<R>(1, 0);
     const C aM = pa::mk<R>(a_m_top, ca.aim[M]);
     const C saM = pa::cmul(pa::mk<R>(ca.s, 0), aM);
     mark(h, 0);
-    if (pa::launch_assemble<R>(ca, X, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (pa::launch_fill<R>(nullptr, zero, aM, (int)d, N, b2, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (pa::launch_fill<R>(X, saM, pa::mk<R>(ca.are[M - 1], ca.aim[M - 1]), (int)d, N, b1, h->stream) != hipSuccess)
+    if (pa::launch_assemble<R>(ca, X, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (pa::launch_fill<R>(nullptr, pa::mk<R>(0, 0), aM, (int)d, N, b2, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (pa::launch_fill<R>(X, saM, pa::mk<R>(ca.are[M - 1], ca.aim[M - 1]), (int)d, N, b1, st) != hipSuccess)
         return PARAMENT_STATUS_CUBLAS_FAILED;
     for (int k = M - 2; k >= 0; --k) {
-        const C beta = pa::mk<R>(k == 0 ? R(-2) : R(-1), 0);
-        // b2 <- s X b1 + beta b2 + a_k I, then swap roles
-        if (pa::launch_gemm<R>(X, 1, b1, 1, b2, 1, b2, 1, pa::mk<R>(ca.s, 0), beta, pa::mk<R>(ca.are[k], ca.aim[k]),
-                               (int)d, N, h->stream) != hipSuccess)
+        // b2 <- s X b1 + beta b2 + a_k I (k = 0: a_0 - 1, the slice as E = U - I), then swap roles
+        const double beta = (k == 0) ? -2.0 : -1.0;
+        const double akr = (k == 0) ? a0m1 : (double)ca.are[k], aki = ca.aim[k];
+        if (zgemm_native<R>(X, 1, b1, 1, b2, 1, b2, 1, c2((double)ca.s), c2(beta), c2(akr, aki), (int)d, N, st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         std::swap(b1, b2);
     }
@@ -428,18 +559,18 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     int r = N;
     while (r > 1) {
         const int half = r / 2;
-        if (pa::launch_gemm<R>(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, one, zero, zero, (int)d, half, h->stream,
-                               true) != hipSuccess)
+        if (zgemm_native<R>(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, c2(1), c2(0), c2(0), (int)d, half, st, true) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         if (r % 2) {
-            if (hipMemcpyAsync(wr + half * dd, rd + (size_t)(r - 1) * dd, dd * sizeof(C), hipMemcpyDeviceToDevice,
-                               h->stream) != hipSuccess)
+            if (hipMemcpyAsync(wr + half * dd, rd + (size_t)(r - 1) * dd, dd * sizeof(C), hipMemcpyDeviceToDevice, st) !=
+                hipSuccess)
                 return PARAMENT_STATUS_CUBLAS_FAILED;
         }
         r = half + (r % 2);
         std::swap(rd, wr);
     }
-    if (pa::launch_fill<R>(rd, one, one, (int)d, 1, out_dev, h->stream) != hipSuccess)  // U = I + E
+    const C one = pa::mk<R>(1, 0);
+    if (pa::launch_fill<R>(rd, one, one, (int)d, 1, out_dev, st) != hipSuccess)  // U = I + E
         return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 2);
     return PARAMENT_STATUS_SUCCESS;
@@ -790,7 +921,8 @@ double OneNorm_fp64(cuDoubleComplex* mat, unsigned int dim) { return mat ? one_n
 // ---- additions ----
 Parament_ErrorCode Parament_setPrecisionMode_fp64(Parament_Context_f64* handle, Parament_PrecisionMode mode) {
     if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
-    if (mode != PARAMENT_PRECISION_NATIVE && mode != PARAMENT_PRECISION_MIXED)
+    if (mode != PARAMENT_PRECISION_NATIVE && mode != PARAMENT_PRECISION_MIXED && mode != PARAMENT_PRECISION_MIXED_FP32 &&
+        mode != PARAMENT_PRECISION_MIXED_BF16X2)
         return CTX64(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
     CTX64(handle)->precision = (int)mode;
     return CTX64(handle)->last_error = PARAMENT_STATUS_SUCCESS;

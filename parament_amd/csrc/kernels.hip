@@ -274,8 +274,8 @@ template hipError_t launch_tree<double>(const void*, bool, int, int, double2*, d
 // ---------------------------------------------------------------------------------------
 // Generic step-wise kernels (any dim): the reference's launch structure
 // (parament.cpp:486-718) with the diagonal add and the scaling fused into the GEMM.
-template <class R>
-__global__ __launch_bounds__(256) void assemble_kernel(ChebArgs<R> a, cplx<R>* __restrict__ X, int blocks_per) {
+template <class R, class TO>
+__global__ __launch_bounds__(256) void assemble_kernel(ChebArgs<R> a, TO* __restrict__ X, int blocks_per) {
     __shared__ cplx<R> coef[kMaxCoeffs];
     const int j = blockIdx.x / blocks_per, b = blockIdx.x % blocks_per;
     for (int t = threadIdx.x; t < a.ncoef; t += blockDim.x) coef[t] = slice_coefficient(a.ctl, t, j);
@@ -285,18 +285,68 @@ __global__ __launch_bounds__(256) void assemble_kernel(ChebArgs<R> a, cplx<R>* _
     if (e >= dd) return;
     cplx<R> x = a.ops[e];
     for (int t = 0; t < a.ncoef; ++t) x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + e]));
-    X[(size_t)j * dd + e] = x;
+    TO o;
+    o.x = x.x; o.y = x.y;
+    X[(size_t)j * dd + e] = o;
 }
 
 template <class R>
 hipError_t launch_assemble(const ChebArgs<R>& a, cplx<R>* X, hipStream_t st) {
     if (a.ncoef > kMaxCoeffs) return hipErrorInvalidValue;
     const int blocks_per = (a.d * a.d + 255) / 256;
-    hipLaunchKernelGGL((assemble_kernel<R>), dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, X, blocks_per);
+    hipLaunchKernelGGL((assemble_kernel<R, cplx<R>>), dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, X, blocks_per);
     return hipGetLastError();
 }
 template hipError_t launch_assemble<float>(const ChebArgs<float>&, float2*, hipStream_t);
 template hipError_t launch_assemble<double>(const ChebArgs<double>&, double2*, hipStream_t);
+
+hipError_t launch_assemble_c64(const ChebArgs<double>& a, float2* X, hipStream_t st) {
+    if (a.ncoef > kMaxCoeffs) return hipErrorInvalidValue;
+    const int blocks_per = (a.d * a.d + 255) / 256;
+    hipLaunchKernelGGL((assemble_kernel<double, float2>), dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, X,
+                       blocks_per);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void widen_kernel(const float2* __restrict__ in, long total, double2* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < total) out[i] = mk<double>(in[i].x, in[i].y);
+}
+
+hipError_t launch_widen(const float2* in, int d, int n, double2* out, hipStream_t st) {
+    const long total = (long)n * d * d;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, in, total, out);
+    return hipGetLastError();
+}
+
+// E_j = (a_0 - 1) I + a_1 s X_j + R_j in FP64 (X_j recomputed from the operators), rounded once
+// to complex64: the first-order part of the mixed-precision slice propagator.
+__global__ __launch_bounds__(256) void first_order_kernel(ChebArgs<double> a, const float2* __restrict__ R,
+                                                          float2* __restrict__ E, int blocks_per) {
+    __shared__ double2 coef[kMaxCoeffs];
+    const int j = blockIdx.x / blocks_per, b = blockIdx.x % blocks_per;
+    for (int t = threadIdx.x; t < a.ncoef; t += blockDim.x) coef[t] = slice_coefficient(a.ctl, t, j);
+    __syncthreads();
+    const int d = a.d, dd = d * d;
+    const int e = b * 256 + threadIdx.x;
+    if (e >= dd) return;
+    double2 x = a.ops[e];
+    for (int t = 0; t < a.ncoef; ++t) x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + e]));
+    const double2 a1s = mk<double>(a.are[1] * a.s, a.aim[1] * a.s);
+    const double2 f = cmul(a1s, x);
+    const float2 r = R[(size_t)j * dd + e];
+    const double er = f.x + (double)r.x + ((e / d == e % d) ? a.are[0] : 0.0);
+    const double ei = f.y + (double)r.y;
+    E[(size_t)j * dd + e] = make_float2((float)er, (float)ei);
+}
+
+hipError_t launch_first_order(const ChebArgs<double>& a, const float2* R, float2* E, hipStream_t st) {
+    if (a.ncoef > kMaxCoeffs) return hipErrorInvalidValue;
+    const int blocks_per = (a.d * a.d + 255) / 256;
+    hipLaunchKernelGGL(first_order_kernel, dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, R, E, blocks_per);
+    return hipGetLastError();
+}
 
 template <class R>
 __global__ __launch_bounds__(256) void fill_kernel(const cplx<R>* __restrict__ X, cplx<R> alpha, cplx<R> diag, int d,
@@ -321,83 +371,5 @@ hipError_t launch_fill(const cplx<R>* X, cplx<R> alpha, cplx<R> diag, int d, int
 }
 template hipError_t launch_fill<float>(const float2*, float2, float2, int, int, float2*, hipStream_t);
 template hipError_t launch_fill<double>(const double2*, double2, double2, int, int, double2*, hipStream_t);
-
-// C_b = alpha A_b B_b + beta Cin_b + diag I (+ A_b + B_b) with 32x32 output tiles, 16-deep K tiles.
-template <class R>
-__global__ __launch_bounds__(256) void gemm_kernel(const cplx<R>* __restrict__ A, long sa, const cplx<R>* __restrict__ B,
-                                                   long sb, const cplx<R>* Cin, long sc, cplx<R>* Cout, long so,
-                                                   cplx<R> alpha, cplx<R> beta, cplx<R> diag, int d, int tiles,
-                                                   int add_ab) {
-    __shared__ cplx<R> As[32][17];
-    __shared__ cplx<R> Bs[16][33];
-    const int t2 = tiles * tiles;
-    const long b = blockIdx.x / t2;
-    const int t = blockIdx.x % t2, ti = t / tiles, tj = t % tiles;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const long dd = (long)d * d;
-    const cplx<R>* Ab = A + b * sa * dd;
-    const cplx<R>* Bb = B + b * sb * dd;
-    cplx<R> acc[2][2];
-    for (int p = 0; p < 2; ++p) for (int q = 0; q < 2; ++q) acc[p][q] = mk<R>(0, 0);
-    for (int k0 = 0; k0 < d; k0 += 16) {
-        for (int e = threadIdx.x; e < 512; e += 256) {
-            const int r = e >> 4, kk = e & 15;
-            const int gi = ti * 32 + r, gk = k0 + kk;
-            As[r][kk] = (gi < d && gk < d) ? Ab[(long)gi * d + gk] : mk<R>(0, 0);
-            const int kr = e >> 5, c = e & 31;
-            const int gk2 = k0 + kr, gj = tj * 32 + c;
-            Bs[kr][c] = (gk2 < d && gj < d) ? Bb[(long)gk2 * d + gj] : mk<R>(0, 0);
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 16; ++kk) {
-            const cplx<R> a0 = As[ty][kk], a1 = As[ty + 16][kk];
-            const cplx<R> b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
-            acc[0][0] = cadd(acc[0][0], cmul(a0, b0));
-            acc[0][1] = cadd(acc[0][1], cmul(a0, b1));
-            acc[1][0] = cadd(acc[1][0], cmul(a1, b0));
-            acc[1][1] = cadd(acc[1][1], cmul(a1, b1));
-        }
-        __syncthreads();
-    }
-    for (int p = 0; p < 2; ++p)
-        for (int q = 0; q < 2; ++q) {
-            const int gi = ti * 32 + ty + 16 * p, gj = tj * 32 + tx + 16 * q;
-            if (gi >= d || gj >= d) continue;
-            cplx<R> v = cmul(alpha, acc[p][q]);
-            if (Cin) v = cadd(v, cmul(beta, Cin[b * sc * dd + (long)gi * d + gj]));
-            if (add_ab) v = cadd(v, cadd(Ab[(long)gi * d + gj], Bb[(long)gi * d + gj]));
-            if (gi == gj) v = cadd(v, diag);
-            Cout[b * so * dd + (long)gi * d + gj] = v;
-        }
-}
-
-template <class R>
-hipError_t launch_gemm(const cplx<R>* A, long sa, const cplx<R>* B, long sb, const cplx<R>* Cin, long sc,
-                       cplx<R>* Cout, long so, cplx<R> alpha, cplx<R> beta, cplx<R> diag, int d, int batch,
-                       hipStream_t st, bool add_ab) {
-    const int tiles = (d + 31) / 32;
-    const long grid = (long)batch * tiles * tiles;
-    if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((gemm_kernel<R>), dim3((unsigned)grid), dim3(256), 0, st, A, sa, B, sb, Cin, sc, Cout, so,
-                       alpha, beta, diag, d, tiles, add_ab ? 1 : 0);
-    return hipGetLastError();
-}
-template hipError_t launch_gemm<float>(const float2*, long, const float2*, long, const float2*, long, float2*, long,
-                                       float2, float2, float2, int, int, hipStream_t, bool);
-template hipError_t launch_gemm<double>(const double2*, long, const double2*, long, const double2*, long, double2*,
-                                        long, double2, double2, double2, int, int, hipStream_t, bool);
-
-__global__ void zero_kernel(unsigned* p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-template <class R>
-hipError_t launch_zero(unsigned* p, size_t n, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
-    return hipGetLastError();
-}
-template hipError_t launch_zero<float>(unsigned*, size_t, hipStream_t);
 
 }  // namespace pa

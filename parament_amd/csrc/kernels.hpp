@@ -55,13 +55,29 @@ template <class R> hipError_t launch_assemble(const ChebArgs<R>& a, cplx<R>* X, 
 // out_b = alpha * X_b + diag * I  (X may be null when alpha == 0)
 template <class R> hipError_t launch_fill(const cplx<R>* X, cplx<R> alpha, cplx<R> diag, int d, int batch,
                                           cplx<R>* out, hipStream_t st);
-// C_b = alpha * A_b B_b + beta * Cin_b + diag * I (+ A_b + B_b when add_ab); strides in
-// matrices of d x d.
-template <class R> hipError_t launch_gemm(const cplx<R>* A, long sa, const cplx<R>* B, long sb,
-                                          const cplx<R>* Cin, long sc, cplx<R>* Cout, long so,
-                                          cplx<R> alpha, cplx<R> beta, cplx<R> diag, int d, int batch,
-                                          hipStream_t st, bool add_ab = false);
-template <class R> hipError_t launch_zero(unsigned* p, size_t n, hipStream_t st);
+// X_j assembled in FP64 and rounded to complex64 (operand of the low-precision GEMMs).
+hipError_t launch_assemble_c64(const ChebArgs<double>& a, float2* X, hipStream_t st);
+// n d x d complex64 matrices -> complex128.
+hipError_t launch_widen(const float2* in, int d, int n, double2* out, hipStream_t st);
+// E_j = (a_0 - 1) I + a_1 s X_j + R_j in FP64 (are[0] = a_0 - 1), stored as complex64.
+hipError_t launch_first_order(const ChebArgs<double>& a, const float2* R, float2* E, hipStream_t st);
+
+// Batched complex GEMM on the matrix cores (gemm.hip), any dim:
+//     C_b = alpha A_b B_b + beta Cin_b + diag I (+ A_b + B_b when add_ab),
+// strides in matrices of d x d (0 broadcasts one matrix).  Element types: complex128 where the
+// flag is set, complex64 otherwise (Cin has C's type).  GEMM_F64 needs a complex128 output and
+// takes complex128 or complex64 operands; the other modes run on complex64 throughout.
+enum : int { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X2 = 2, GEMM_BF16X3 = 3 };
+struct GemmArgs {
+    const void* A; long sa;
+    const void* B; long sb;
+    const void* Cin; long sc;
+    void* C; long so;
+    double alpha_re, alpha_im, beta_re, beta_im, diag_re, diag_im;
+    int d, batch, add_ab;
+    int a_c128, b_c128, out_c128;
+};
+hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st);
 
 // Adjoint gate-fidelity gradient (grad.hip), dims <= 32.  Tree levels are stored as
 // consecutive d x d matrices; see grad.hip for the scan.

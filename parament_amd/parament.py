@@ -20,7 +20,8 @@ from functools import wraps
 import numpy as np
 
 from .constants import *  # noqa: F401,F403
-from .constants import (PARAMENT_FAIL, PARAMENT_PRECISION_MIXED, PARAMENT_PRECISION_NATIVE,
+from .constants import (PARAMENT_FAIL, PARAMENT_PRECISION_MIXED, PARAMENT_PRECISION_MIXED_BF16X2,
+                        PARAMENT_PRECISION_MIXED_FP32, PARAMENT_PRECISION_NATIVE,
                         PARAMENT_QUADRATURE_MIDPOINT, PARAMENT_QUADRATURE_NONE, PARAMENT_QUADRATURE_SIMPSON,
                         PARAMENT_STATUS_CUBLAS_FAILED, PARAMENT_STATUS_CUBLAS_INIT_FAILED,
                         PARAMENT_STATUS_DEVICE_ALLOC_FAILED, PARAMENT_STATUS_HOST_ALLOC_FAILED,
@@ -203,12 +204,16 @@ class Parament:
 
     @_must_be_alive
     def set_precision_mode(self, mode):
-        """'native' (default) or 'mixed' (complex128 contexts only)."""
+        """Arithmetic of a complex128 context: 'native' (default, FP64 throughout), 'mixed'
+        (bf16 three-way split products), 'mixed_fp32' (FP32 products) or 'mixed_bf16x2'
+        (bf16 hi+lo products); the mixed modes keep FP64 for the first-order terms and the
+        upper product-tree levels."""
         if not self._use_doubles:
             raise ValueError("precision modes apply to 'fp64' contexts")
-        code = {'native': PARAMENT_PRECISION_NATIVE, 'mixed': PARAMENT_PRECISION_MIXED}.get(mode)
+        code = {'native': PARAMENT_PRECISION_NATIVE, 'mixed': PARAMENT_PRECISION_MIXED,
+                'mixed_fp32': PARAMENT_PRECISION_MIXED_FP32, 'mixed_bf16x2': PARAMENT_PRECISION_MIXED_BF16X2}.get(mode)
         if code is None:
-            raise ValueError("mode must be 'native' or 'mixed'")
+            raise ValueError("mode must be 'native', 'mixed', 'mixed_fp32' or 'mixed_bf16x2'")
         self._check_error(self._lib.Parament_setPrecisionMode_fp64(self._handle, code))
 
     @_must_be_alive

@@ -21,7 +21,7 @@ def run_gpu(prec, h0, hks, c, dt, mode="none", magnus=False, path="auto", cycles
     with parament.Parament(prec) as p:
         p.set_kernel_path(path)
         if mixed:
-            p.set_precision_mode("mixed")
+            p.set_precision_mode("mixed" if mixed is True else mixed)
         if cycles is not None:
             p.set_iteration_cycles(cycles)
         p.set_hamiltonian(h0, *hks, use_magnus=magnus, quadrature_mode=mode)
@@ -224,3 +224,34 @@ def test_full_size_heisenberg6_mixed_within_gate():
     eye = np.eye(64)
     assert np.linalg.norm(u64.conj().T @ u64 - eye) < 1e-9
     assert np.linalg.norm(um.conj().T @ um - eye) < 1e-4
+
+
+# ---- dims above the on-chip kernels: batched matrix-core GEMMs ----
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("d,n", [(65, 9), (96, 17), (130, 6), (256, 3)])
+def test_stepwise_large_dims(prec, d, n):
+    """Dims above 64 run the step-wise path on the batched GEMM kernel (64x64 tiles, ragged
+    edges, odd tree carries)."""
+    h0, hks, c = case(1300 + d, d, 2, n)
+    u, m = run_gpu(prec, h0, hks, c, 0.02, "none")
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 0.02, "none")) < TOL[prec], m
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("mode,magnus", MODES)
+def test_stepwise_forced_small_dim(prec, mode, magnus):
+    h0, hks, c = case(1400, 24, 2, 33)
+    u, m = run_gpu(prec, h0, hks, c, 0.03, mode, magnus, path="stepwise")
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 0.03, mode, magnus)) < TOL[prec], m
+
+
+@pytest.mark.parametrize("variant,tol", [("mixed", 1e-5), ("mixed_fp32", 1e-5), ("mixed_bf16x2", 5e-3)])
+@pytest.mark.parametrize("d,n,mode", [(65, 40, "simpson"), (100, 33, "none"), (128, 20, "midpoint"), (20, 70, "none")])
+def test_mixed_stepwise_modes(variant, tol, d, n, mode):
+    """Mixed precision on batched GEMMs (dims above 64, and every dim for the FP32 and bf16
+    hi+lo products): Frobenius error against the FP64 restatement with the same truncation."""
+    h0, hks, c = case(1500 + d + n, d, 2, n)
+    u, m = run_gpu("fp64", h0, hks, c, 0.01, mode, mixed=variant)
+    ref = eo.equiprop(h0, hks, c, 0.01, mode=mode, cycles=m)
+    assert rel_err(u, ref) < tol
