@@ -214,6 +214,36 @@ LIBSPEC Parament_ErrorCode Parament_setGradientCheckpointing_fp64(Parament_Conte
 LIBSPEC size_t Parament_getGradientWorkspace(Parament_Context_f32 *handle);
 LIBSPEC size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64 *handle);
 
+/* Ensembles of independent complex64 problems (no reference counterpart; for batched studies
+ * and optimisations over many systems of different sizes).  Instance i has dimension dims[i],
+ * amps[i] control Hamiltonians, pts[i] control samples and time step dt[i]; one quadrature
+ * mode (NONE, MIDPOINT or SIMPSON) applies to all.  Packed layouts, instance after instance:
+ *   H:   (1 + amps[i]) row-major dims[i] x dims[i] matrices (H0, then H_1 ..),
+ *   carr: amps[i] x pts[i] samples (carr layout of Parament_equiprop),
+ *   out: dims[i] x dims[i] propagators.
+ * Each instance computes what Parament_equiprop computes for it (FP32 iteration-count table,
+ * same quadrature and product order).  setExecution(0) (default) runs the whole ensemble as
+ * one captured HIP graph: the instances of dim <= 64 in multi-instance kernels per dim class
+ * (16, 32, 64), the larger ones on batched GEMMs, on concurrent branches; the wave split, tree
+ * block size and grid of each class and the number of branches are tuned on first use (or by
+ * ensembleTune).  setExecution(1) runs one instance after the other (the per-instance loop).
+ * getTuning fills config[10]: (T, log2 tree block, grid multiple) for classes 16, 32, 64, then
+ * the number of branches. */
+typedef struct Parament_Ensemble Parament_Ensemble;
+LIBSPEC Parament_ErrorCode Parament_ensembleCreate(Parament_Ensemble **ensemble_p, unsigned int count,
+    const unsigned int *dims, const unsigned int *amps, const unsigned int *pts, const double *dt,
+    Parament_QuadratureSpec quadrature_mode);
+LIBSPEC Parament_ErrorCode Parament_ensembleDestroy(Parament_Ensemble *ensemble);
+LIBSPEC Parament_ErrorCode Parament_ensembleSetHamiltonians(Parament_Ensemble *ensemble, const cuComplex *H);
+LIBSPEC Parament_ErrorCode Parament_ensembleSetExecution(Parament_Ensemble *ensemble, int mode);
+LIBSPEC Parament_ErrorCode Parament_ensembleTune(Parament_Ensemble *ensemble);
+LIBSPEC Parament_ErrorCode Parament_ensembleGetTuning(Parament_Ensemble *ensemble, int *config);
+LIBSPEC Parament_ErrorCode Parament_ensembleEquiprop(Parament_Ensemble *ensemble, const cuComplex *carr, cuComplex *out);
+LIBSPEC Parament_ErrorCode Parament_ensembleEquipropDevice(Parament_Ensemble *ensemble, const cuComplex *carr_dev,
+    cuComplex *out_dev);
+LIBSPEC Parament_ErrorCode Parament_ensembleSynchronize(Parament_Ensemble *ensemble);
+LIBSPEC Parament_ErrorCode Parament_ensemblePeekAtLastError(Parament_Ensemble *ensemble);
+
 #ifdef __cplusplus
 }
 #endif

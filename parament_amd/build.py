@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libparament.so")
-SOURCES = ["host.cpp", "kernels.hip", "mixed.hip", "grad.hip", "gemm.hip"]
+SOURCES = ["host.cpp", "kernels.hip", "mixed.hip", "grad.hip", "gemm.hip", "ensemble.hip"]
 ARCH = os.environ.get("PARAMENT_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 VALU instructions are disabled for all device code.
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -10,7 +10,8 @@
 // slice does not fit on chip.
 //
 // Tiling: a work-group computes a 64x64 output tile with 4 waves of 32x32 (2x2 blocks of
-// 16x16); K advances in chunks of 32 staged in LDS (zero-padded past d).  Work items (batch b,
+// 16x16), or a 32x32 tile with 4 waves of 16x16 (less padding for dims just above a multiple
+// of 64); K advances in chunks of 32 staged in LDS (zero-padded past d).  Work items (batch b,
 // tile t) are assigned so that every XCD receives a contiguous range of them: the tiles of one
 // slice then share that XCD's L2 for their A rows and B columns.
 #include "kernels.hpp"
@@ -22,7 +23,6 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4g __attribute__((ext_vector_type(4)));
 
-constexpr int BT = 64;   // output tile
 constexpr int KT = 32;   // K chunk
 
 // Work item of this block: XCD x (= blockIdx.x mod 8) takes items [x * per, (x + 1) * per).
@@ -33,6 +33,14 @@ __device__ __forceinline__ long work_item(long total) {
     const long start = x * per + (x < rem ? x : rem);
     const long w = start + i;
     return w < total ? w : -1;
+}
+
+// Output tile of a launch: the caller's choice (32 or 64), else the one with the smaller
+// padded product volume, 32-tiles counted 1.5x for their lower matrix-core efficiency.
+int gemm_tile(const GemmArgs& g) {
+    if (g.tile == 32 || g.tile == 64) return g.tile;
+    auto vol = [&](int bt) { const double p = (double)((g.d + bt - 1) / bt) * bt; return p * p * (double)((g.d + KT - 1) / KT) * KT; };
+    return 1.5 * vol(32) < vol(64) ? 32 : 64;
 }
 
 template <class T> __device__ __forceinline__ double2 ld(const T* p, long i);
@@ -50,8 +58,9 @@ struct EpiIO {
 };
 
 // FP64 / FP32 matrix-core tiles: A chunk [64][32] (stride SA), B chunk [32][64] (stride SB).
-template <class R, class TA, class TB, class TC, class TO>
+template <class R, int BT, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
+    constexpr int TW = BT / 32;   // 16x16 blocks per wave side (4 waves, 2 x 2)
     constexpr int SA = KT + Pad<R>::A, SB = BT + Pad<R>::B;
     __shared__ R Ar[BT * SA], Ai[BT * SA], Br[KT * SB], Bi[KT * SB];
     const int d = g.d, tiles = (d + BT - 1) / BT;
@@ -64,8 +73,8 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
     const TA* A = io.A + b * g.sa * dd;
     const TB* B = io.B + b * g.sb * dd;
     const int tid = threadIdx.x, wave = tid >> 6;
-    const int wr = 32 * (wave >> 1), wc = 32 * (wave & 1);
-    CTile<R, 2, 2> acc;
+    const int wr = (BT / 2) * (wave >> 1), wc = (BT / 2) * (wave & 1);
+    CTile<R, TW, TW> acc;
     acc.zero();
     for (int k0 = 0; k0 < d; k0 += KT) {
         for (int e = tid; e < BT * KT; e += 256) {
@@ -83,15 +92,15 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
             Br[k * SB + c] = (R)v.x; Bi[k * SB + c] = (R)v.y;
         }
         __syncthreads();
-        wave_cgemm<R, 2, 2, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc);
+        wave_cgemm<R, TW, TW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc);
         __syncthreads();
     }
     // epilogue in FP64
     const long cb = b * g.so * dd, ib = b * g.sc * dd;
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < TW; ++m)
 #pragma unroll
-        for (int n = 0; n < 2; ++n)
+        for (int n = 0; n < TW; ++n)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = ti * BT + acc_row<R>(wr, m, v), gj = tj * BT + acc_col(wc, n);
@@ -117,8 +126,9 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
 // transposed [slice][re/im][64 cols][32 + 8].  Operand fragments of the 16x16x32 instruction:
 // lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7]; accumulator element v is
 // C[4(l/16) + v][l%16].
-template <int NS, class TA, class TB, class TC, class TO>
+template <int NS, int BT, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
+    constexpr int TW = BT / 32;   // 16x16 blocks per wave side (4 waves, 2 x 2)
     constexpr int S = KT + 8, PL = BT * S;
     __shared__ __attribute__((aligned(16))) __bf16 Ap[2 * NS * PL];
     __shared__ __attribute__((aligned(16))) __bf16 Bp[2 * NS * PL];
@@ -132,16 +142,16 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
     const TA* A = io.A + b * g.sa * dd;
     const TB* B = io.B + b * g.sb * dd;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wr = 32 * (wave >> 1), wc = 32 * (wave & 1);
+    const int wr = (BT / 2) * (wave >> 1), wc = (BT / 2) * (wave & 1);
     const int r16 = lane & 15, kq = lane >> 4;
     // acc[L][m][n]: L = 0 leading products, 1 the rest
-    f32x4g accr[2][2][2], acci[2][2][2];
+    f32x4g accr[2][TW][TW], acci[2][TW][TW];
 #pragma unroll
     for (int L = 0; L < 2; ++L)
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < TW; ++m)
 #pragma unroll
-            for (int n = 0; n < 2; ++n)
+            for (int n = 0; n < TW; ++n)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) { accr[L][m][n][v] = 0.f; acci[L][m][n][v] = 0.f; }
     auto split_store = [&](__bf16* P, int o, float x, int c) {
@@ -171,9 +181,9 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             split_store(Bp, c * S + k, (float)v.y, 1);
         }
         __syncthreads();
-        bf16x8 br[2][NS], bi[2][NS];
+        bf16x8 br[TW][NS], bi[TW][NS];
 #pragma unroll
-        for (int n = 0; n < 2; ++n)
+        for (int n = 0; n < TW; ++n)
 #pragma unroll
             for (int q = 0; q < NS; ++q) {
                 const int o = (wc + 16 * n + r16) * S + 8 * kq;
@@ -181,7 +191,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
                 bi[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q + 1) * PL + o);
             }
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < TW; ++m)
 #pragma unroll
             for (int p = 0; p < NS; ++p) {
                 const int o = (wr + 16 * m + r16) * S + 8 * kq;
@@ -192,7 +202,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
                 u ^= 0x80008000u;
                 const bf16x8 an = __builtin_bit_cast(bf16x8, u);
 #pragma unroll
-                for (int n = 0; n < 2; ++n)
+                for (int n = 0; n < TW; ++n)
 #pragma unroll
                     for (int q = 0; q + p < NS; ++q) {
                         const int L = (p + q) > 0 ? 1 : 0;
@@ -206,9 +216,9 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
     }
     const long cb = b * g.so * dd, ib = b * g.sc * dd;
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < TW; ++m)
 #pragma unroll
-        for (int n = 0; n < 2; ++n)
+        for (int n = 0; n < TW; ++n)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = ti * BT + wr + 16 * m + 4 * kq + v, gj = tj * BT + wc + 16 * n + r16;
@@ -231,24 +241,29 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             }
 }
 
-template <class TA, class TB, class TC, class TO>
-hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
-    EpiIO<TA, TB, TC, TO> io;
-    io.A = static_cast<const TA*>(g.A); io.B = static_cast<const TB*>(g.B);
-    io.Cin = static_cast<const TC*>(g.Cin); io.C = static_cast<TO*>(g.C);
+template <int BT, class TA, class TB, class TC, class TO>
+hipError_t launch_tiled(const GemmArgs& g, int mode, const EpiIO<TA, TB, TC, TO>& io, hipStream_t st) {
     const int tiles = (g.d + BT - 1) / BT;
     const long total = (long)g.batch * tiles * tiles;
     if (total <= 0) return hipSuccess;
     if (total > 0x7fffffffL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)total), block(256);
     switch (mode) {
-        case GEMM_F64: hipLaunchKernelGGL((zgemm_fp_kernel<double, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_F32: hipLaunchKernelGGL((zgemm_fp_kernel<float, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_BF16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_BF16X3: hipLaunchKernelGGL((zgemm_split_kernel<3, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_F64: hipLaunchKernelGGL((zgemm_fp_kernel<double, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_F32: hipLaunchKernelGGL((zgemm_fp_kernel<float, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_BF16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_BF16X3: hipLaunchKernelGGL((zgemm_split_kernel<3, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <class TA, class TB, class TC, class TO>
+hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
+    EpiIO<TA, TB, TC, TO> io;
+    io.A = static_cast<const TA*>(g.A); io.B = static_cast<const TB*>(g.B);
+    io.Cin = static_cast<const TC*>(g.Cin); io.C = static_cast<TO*>(g.C);
+    return gemm_tile(g) == 32 ? launch_tiled<32>(g, mode, io, st) : launch_tiled<64>(g, mode, io, st);
 }
 
 }  // namespace

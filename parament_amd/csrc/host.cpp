@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <new>
 #include <vector>
 
@@ -171,8 +172,9 @@ bool grow(void** p, size_t* cap, size_t bytes) {
 // matrices.
 struct GemmOp { const void* p; bool c128; long stride; };
 hipError_t zgemm(int mode, GemmOp A, GemmOp B, GemmOp Cin, void* C, bool c_c128, long so, double2 alpha, double2 beta,
-                 double2 diag, int d, int batch, hipStream_t st, bool add_ab = false) {
+                 double2 diag, int d, int batch, hipStream_t st, bool add_ab = false, int tile = 0) {
     pa::GemmArgs g;
+    g.tile = tile;
     g.A = A.p; g.sa = A.stride; g.B = B.p; g.sb = B.stride; g.Cin = Cin.p; g.sc = Cin.stride; g.C = C; g.so = so;
     g.alpha_re = alpha.x; g.alpha_im = alpha.y; g.beta_re = beta.x; g.beta_im = beta.y;
     g.diag_re = diag.x; g.diag_im = diag.y;
@@ -184,10 +186,10 @@ hipError_t zgemm(int mode, GemmOp A, GemmOp B, GemmOp Cin, void* C, bool c_c128,
 template <class R>
 hipError_t zgemm_native(const cplx<R>* A, long sa, const cplx<R>* B, long sb, const cplx<R>* Cin, long sc, cplx<R>* C,
                         long so, double2 alpha, double2 beta, double2 diag, int d, int batch, hipStream_t st,
-                        bool add_ab = false) {
+                        bool add_ab = false, int tile = 0) {
     const bool w = sizeof(R) == 8;
     return zgemm(w ? pa::GEMM_F64 : pa::GEMM_F32, GemmOp{A, w, sa}, GemmOp{B, w, sb}, GemmOp{Cin, w, sc}, C, w, so, alpha,
-                 beta, diag, d, batch, st, add_ab);
+                 beta, diag, d, batch, st, add_ab, tile);
 }
 inline double2 c2(double re, double im = 0.0) { return pa::mk<double>(re, im); }
 
@@ -828,6 +830,405 @@ template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, dou
     if (b) *b = t1;
     return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
+// ----- ensemble of independent complex64 problems (ensemble.hip + gemm.hip) -----
+// Instances of dim <= 64 are padded to a dim class (16, 32, 64) and run by the multi-instance
+// kernels, one slice kernel and one tree-level kernel per class and level; larger instances run
+// the step-wise path on batched GEMMs.  The classes and the large instances go on separate
+// branches (streams forked from and joined to the ensemble's stream), the whole solve is
+// captured once as a hipGraph and replayed.  The tuner picks, per class, the wave split, the
+// tree block size and the persistent grid, and the number of branches for large instances, by
+// timing each candidate on the ensemble itself.
+constexpr int kEnsSide = 8;        // side streams: 3 classes + up to 5 large-instance branches
+constexpr int kEnsClasses = 3;     // dim classes 16, 32, 64
+
+struct EnsInst {
+    unsigned d = 0, amps = 0, pts = 0;
+    double dt = 0, tau = 0, hnorm = 0;
+    int N = 0, M = 0, cls = -1;                // cls: class index, -1 for the batched-GEMM path
+    int tile = 0;                               // batched-GEMM output tile (tuned; 0 = by volume)
+    size_t ops_off = 0, c_off = 0, out_off = 0; // element offsets in the packed arrays
+    size_t work_off = 0;                        // element offset of its working memory
+    double are[pa::kMaxCheb], aim[pa::kMaxCheb];
+};
+
+struct EnsClassPlan {
+    int D = 0;
+    int2* d_slice = nullptr; int n_slice = 0;
+    // per tree block size (index logb - 2): levels, each a descriptor table and a task table
+    std::vector<pa::EnsLevel*> d_lv[2];
+    std::vector<int2*> d_lt[2];
+    std::vector<int> n_lt[2];
+};
+
+struct EnsCfg { int T = 1, logb = 3, grid_mult = 0; };
+
+struct Ensemble {
+    Parament_ErrorCode last_error = PARAMENT_STATUS_SUCCESS;
+    std::vector<EnsInst> in;
+    int quad = 0;
+    bool has_h = false;
+    int device = 0, cus = 256;
+    hipStream_t stream = nullptr, side[kEnsSide] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kEnsSide] = {}, t0 = nullptr, t1 = nullptr;
+    float2 *d_ops = nullptr, *d_c = nullptr, *d_out = nullptr, *d_work = nullptr;
+    size_t n_ops = 0, n_c = 0, n_out = 0, n_work = 0;
+    pa::EnsInstance* d_inst = nullptr;
+    EnsClassPlan cls[kEnsClasses];
+    std::vector<int> large;
+    EnsCfg cfg[kEnsClasses];
+    int branches = 2;
+    bool tuned = false;
+    int exec_mode = 0;              // 0: captured graph, 1: per-instance loop
+    hipGraphExec_t gexec = nullptr;
+    std::vector<void*> allocs;      // small device tables
+};
+
+void ens_free_plan(Ensemble* e) {
+    for (void* p : e->allocs) hipFree(p);
+    e->allocs.clear();
+    for (auto& c : e->cls) {
+        c.d_slice = nullptr; c.n_slice = 0;
+        for (int b = 0; b < 2; ++b) { c.d_lv[b].clear(); c.d_lt[b].clear(); c.n_lt[b].clear(); }
+    }
+    if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+    hipFree(e->d_work); e->d_work = nullptr; e->n_work = 0;
+    e->d_inst = nullptr;   // one of allocs
+    e->large.clear();
+    e->tuned = false;
+}
+
+template <class T> T* ens_upload(Ensemble* e, const std::vector<T>& v) {
+    if (v.empty()) return nullptr;
+    void* p = nullptr;
+    if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) { hipGetLastError(); return nullptr; }
+    e->allocs.push_back(p);
+    if (hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return static_cast<T*>(p);
+}
+
+Parament_ErrorCode ens_create(Ensemble** ep, unsigned count, const unsigned* dims, const unsigned* amps,
+                              const unsigned* pts, const double* dts, int quad) {
+    if (!ep || count == 0 || !dims || !amps || !pts || !dts) return PARAMENT_STATUS_INVALID_VALUE;
+    if (quad != PARAMENT_QUADRATURE_NONE && quad != PARAMENT_QUADRATURE_MIDPOINT && quad != PARAMENT_QUADRATURE_SIMPSON)
+        return PARAMENT_STATUS_INVALID_VALUE;
+    Ensemble* e = new (std::nothrow) Ensemble();
+    if (!e) return PARAMENT_STATUS_HOST_ALLOC_FAILED;
+    e->quad = quad;
+    e->in.resize(count);
+    const bool doubled = quad == PARAMENT_QUADRATURE_SIMPSON;
+    for (unsigned i = 0; i < count; ++i) {
+        EnsInst& x = e->in[i];
+        x.d = dims[i]; x.amps = amps[i]; x.pts = pts[i]; x.dt = dts[i];
+        if (x.d == 0 || x.amps == 0 || x.amps > (unsigned)pa::kMaxCoeffs || !(x.dt > 0)) { delete e; return PARAMENT_STATUS_INVALID_VALUE; }
+        x.tau = doubled ? 2 * x.dt : x.dt;
+        x.N = doubled ? (int)((x.pts - 1) / 2) : (quad == PARAMENT_QUADRATURE_MIDPOINT ? (int)x.pts - 1 : (int)x.pts);
+        if (x.pts == 0 || x.N <= 0) { delete e; return PARAMENT_STATUS_INVALID_VALUE; }
+        x.ops_off = e->n_ops; e->n_ops += (size_t)(1 + x.amps) * x.d * x.d;
+        x.c_off = e->n_c; e->n_c += (size_t)x.amps * x.pts;
+        x.out_off = e->n_out; e->n_out += (size_t)x.d * x.d;
+    }
+    if (hipGetDevice(&e->device) != hipSuccess) { hipGetLastError(); delete e; return PARAMENT_FAIL; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) == hipSuccess) e->cus = prop.multiProcessorCount;
+    bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreate(&e->t0) == hipSuccess && hipEventCreate(&e->t1) == hipSuccess;
+    for (int k = 0; ok && k < kEnsSide; ++k)
+        ok = hipStreamCreateWithFlags(&e->side[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&e->join_ev[k], hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipMalloc(&e->d_ops, e->n_ops * sizeof(float2)) == hipSuccess &&
+         hipMalloc(&e->d_c, std::max<size_t>(e->n_c, 1) * sizeof(float2)) == hipSuccess &&
+         hipMalloc(&e->d_out, e->n_out * sizeof(float2)) == hipSuccess;
+    *ep = e;
+    if (!ok) { hipGetLastError(); return PARAMENT_STATUS_DEVICE_ALLOC_FAILED; }
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+Parament_ErrorCode ens_destroy(Ensemble* e) {
+    if (!e) return PARAMENT_STATUS_SUCCESS;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    ens_free_plan(e);
+    hipFree(e->d_ops); hipFree(e->d_c); hipFree(e->d_out);
+    for (int k = 0; k < kEnsSide; ++k) {
+        if (e->side[k]) hipStreamDestroy(e->side[k]);
+        if (e->join_ev[k]) hipEventDestroy(e->join_ev[k]);
+    }
+    if (e->fork_ev) hipEventDestroy(e->fork_ev);
+    if (e->t0) hipEventDestroy(e->t0);
+    if (e->t1) hipEventDestroy(e->t1);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+// H: packed instance by instance, (1 + amps_i) row-major d_i x d_i matrices (H0 then H_k).
+Parament_ErrorCode ens_set_hamiltonians(Ensemble* e, const cuComplex* H) {
+    if (!H) return PARAMENT_STATUS_INVALID_VALUE;
+    if (hipSetDevice(e->device) != hipSuccess) return PARAMENT_FAIL;
+    ens_free_plan(e);
+    e->has_h = false;
+    if (hipMemcpy(e->d_ops, H, e->n_ops * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    std::vector<pa::EnsInstance> tab(e->in.size());
+    std::vector<std::vector<int2>> slices(kEnsClasses);
+    for (size_t i = 0; i < e->in.size(); ++i) {
+        EnsInst& x = e->in[i];
+        const size_t dd = (size_t)x.d * x.d;
+        x.hnorm = 0;
+        for (unsigned k = 0; k <= x.amps; ++k) x.hnorm += one_norm(H + x.ops_off + k * dd, x.d);
+        x.M = select_fp32(x.hnorm, x.tau);
+        if (x.M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
+        bessel(x.M, x.tau * x.hnorm, x.are, x.aim);
+        x.cls = x.d <= 16 ? 0 : (x.d <= 32 ? 1 : (x.d <= 64 ? 2 : -1));
+        // working memory: classes: leaves N + tree scratch 2 ceil(N / 4); large: X, b1, b2
+        x.work_off = e->n_work;
+        e->n_work += x.cls >= 0 ? ((size_t)x.N + 2 * (size_t)((x.N + 3) / 4)) * dd : 3 * (size_t)x.N * dd;
+        if (x.cls < 0) e->large.push_back((int)i);
+    }
+    if (hipMalloc(&e->d_work, e->n_work * sizeof(float2)) != hipSuccess) { hipGetLastError(); e->d_work = nullptr; return PARAMENT_STATUS_DEVICE_ALLOC_FAILED; }
+    for (size_t i = 0; i < e->in.size(); ++i) {
+        const EnsInst& x = e->in[i];
+        pa::EnsInstance& t = tab[i];
+        t.ops = e->d_ops + x.ops_off; t.c = e->d_c + x.c_off; t.leaves = e->d_work + x.work_off;
+        t.d = (int)x.d; t.ncoef = (int)x.amps; t.amps = (int)x.amps; t.pts = (int)x.pts; t.mode = e->quad; t.M = x.M;
+        t.s = (float)(2.0 / x.hnorm); t.tau = (float)x.tau;
+        for (int k = 0; k < pa::kMaxCheb; ++k) { t.are[k] = k <= x.M ? (float)x.are[k] : 0.f; t.aim[k] = k <= x.M ? (float)x.aim[k] : 0.f; }
+        t.are[0] = (float)(x.are[0] - 1.0);
+        if (x.cls >= 0)
+            for (int j = 0; j < x.N; ++j) slices[x.cls].push_back(make_int2((int)i, j));
+    }
+    e->d_inst = ens_upload(e, tab);
+    if (!e->d_inst) return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    for (int c = 0; c < kEnsClasses; ++c) {
+        EnsClassPlan& P = e->cls[c];
+        P.D = 16 << c;
+        P.n_slice = (int)slices[c].size();
+        if (P.n_slice) {
+            P.d_slice = ens_upload(e, slices[c]);
+            if (!P.d_slice) return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+        }
+        for (int b = 0; b < 2; ++b) {
+            const int logb = b + 2, B = 1 << logb;
+            // level l of every member: src -> dst, until one node remains (written as U = I + E)
+            std::vector<std::vector<pa::EnsLevel>> lv;
+            std::vector<std::vector<int2>> lt;
+            for (size_t i = 0; i < e->in.size(); ++i) {
+                const EnsInst& x = e->in[i];
+                if (x.cls != c) continue;
+                const size_t dd = (size_t)x.d * x.d;
+                float2* leaves = e->d_work + x.work_off;
+                float2* scr[2] = {leaves + (size_t)x.N * dd, leaves + ((size_t)x.N + (size_t)((x.N + 3) / 4)) * dd};
+                int n = x.N, l = 0;
+                const float2* src = leaves;
+                do {
+                    const int n_out = (n + B - 1) / B;
+                    if ((int)lv.size() <= l) { lv.emplace_back(); lt.emplace_back(); }
+                    pa::EnsLevel L;
+                    L.src = src; L.n_in = n; L.d = (int)x.d; L.final_level = n_out == 1;
+                    L.dst = n_out == 1 ? e->d_out + x.out_off : scr[l & 1];
+                    const int li = (int)lv[l].size();
+                    lv[l].push_back(L);
+                    for (int blk = 0; blk < n_out; ++blk) lt[l].push_back(make_int2(li, blk));
+                    src = L.dst; n = n_out; ++l;
+                } while (n > 1);
+            }
+            for (size_t l = 0; l < lv.size(); ++l) {
+                P.d_lv[b].push_back(ens_upload(e, lv[l]));
+                P.d_lt[b].push_back(ens_upload(e, lt[l]));
+                P.n_lt[b].push_back((int)lt[l].size());
+                if (!P.d_lv[b].back() || !P.d_lt[b].back()) return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+            }
+        }
+    }
+    e->has_h = true;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+int ens_grid(const Ensemble* e, int ntasks, int mult) {
+    return mult > 0 ? std::max(1, std::min(ntasks, e->cus * mult)) : ntasks;
+}
+
+hipError_t ens_enqueue_class(Ensemble* e, int c, const EnsCfg& cf, hipStream_t st) {
+    const EnsClassPlan& P = e->cls[c];
+    if (!P.n_slice) return hipSuccess;
+    hipError_t r = pa::launch_ens_slices(P.D, cf.T, e->d_inst, P.d_slice, P.n_slice, ens_grid(e, P.n_slice, cf.grid_mult), st);
+    const int b = cf.logb - 2;
+    for (size_t l = 0; r == hipSuccess && l < P.d_lv[b].size(); ++l)
+        r = pa::launch_ens_tree(P.D, cf.T, cf.logb, P.d_lv[b][l], P.d_lt[b][l], P.n_lt[b][l],
+                                ens_grid(e, P.n_lt[b][l], cf.grid_mult), st);
+    return r;
+}
+
+// One large instance: the step-wise path on batched FP32 matrix-core GEMMs (as Context<float>).
+hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st) {
+    const EnsInst& x = e->in[i];
+    const int d = (int)x.d, N = x.N, M = x.M;
+    const size_t dd = (size_t)d * d;
+    float2* X = e->d_work + x.work_off;
+    float2* b1 = X + (size_t)N * dd;
+    float2* b2 = b1 + (size_t)N * dd;
+    pa::ChebArgs<float> ca;
+    ca.ops = e->d_ops + x.ops_off; ca.ncoef = (int)x.amps; ca.d = d; ca.N = N; ca.M = M; ca.s = (float)(2.0 / x.hnorm);
+    ca.ctl.c = e->d_c + x.c_off; ca.ctl.pts = (int)x.pts; ca.ctl.amps_in = (int)x.amps; ca.ctl.amps_set = (int)x.amps;
+    ca.ctl.mode = e->quad; ca.ctl.magnus = 0; ca.ctl.tau = (float)x.tau;
+    for (int k = 0; k <= M; ++k) { ca.are[k] = (float)x.are[k]; ca.aim[k] = (float)x.aim[k]; }
+    ca.out = nullptr;
+    const float2 aM = make_float2((float)x.are[M], (float)x.aim[M]);
+    const float2 saM = make_float2(ca.s * aM.x, ca.s * aM.y);
+    hipError_t r = pa::launch_assemble<float>(ca, X, st);
+    if (r == hipSuccess) r = pa::launch_fill<float>(nullptr, make_float2(0, 0), aM, d, N, b2, st);
+    if (r == hipSuccess) r = pa::launch_fill<float>(X, saM, make_float2(ca.are[M - 1], ca.aim[M - 1]), d, N, b1, st);
+    for (int k = M - 2; r == hipSuccess && k >= 0; --k) {
+        const double akr = (k == 0) ? x.are[0] - 1.0 : (double)ca.are[k];
+        r = zgemm_native<float>(X, 1, b1, 1, b2, 1, b2, 1, c2((double)ca.s), c2(k == 0 ? -2.0 : -1.0), c2(akr, ca.aim[k]),
+                                d, N, st, false, x.tile);
+        std::swap(b1, b2);
+    }
+    // b1 holds the leaves E_j; b2 is free for the other half of the ping-pong
+    float2* rd = b1;
+    float2* wr = b2;
+    int n = N;
+    while (r == hipSuccess && n > 1) {
+        const int half = n / 2;
+        r = zgemm_native<float>(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, c2(1), c2(0), c2(0), d, half, st, true, x.tile);
+        if (r == hipSuccess && n % 2)
+            r = pa::launch_fill<float>(rd + (size_t)(n - 1) * dd, make_float2(1, 0), make_float2(0, 0), d, 1, wr + half * dd, st);
+        n = half + (n % 2);
+        std::swap(rd, wr);
+    }
+    if (r == hipSuccess) r = pa::launch_fill<float>(rd, make_float2(1, 0), make_float2(1, 0), d, 1, e->d_out + x.out_off, st);
+    return r;
+}
+
+// The whole solve on e->stream: branches for the classes and for the large instances.
+hipError_t ens_enqueue(Ensemble* e) {
+    hipError_t r = hipEventRecord(e->fork_ev, e->stream);
+    const int nb = std::max(1, std::min(e->branches, kEnsSide - kEnsClasses));
+    int used[kEnsSide] = {};
+    for (int c = 0; r == hipSuccess && c < kEnsClasses; ++c) {
+        if (!e->cls[c].n_slice) continue;
+        r = hipStreamWaitEvent(e->side[c], e->fork_ev, 0);
+        if (r == hipSuccess) r = ens_enqueue_class(e, c, e->cfg[c], e->side[c]);
+        used[c] = 1;
+    }
+    for (size_t q = 0; r == hipSuccess && q < e->large.size(); ++q) {
+        const int k = kEnsClasses + (int)(q % nb);
+        if (!used[k]) { r = hipStreamWaitEvent(e->side[k], e->fork_ev, 0); used[k] = 1; }
+        if (r == hipSuccess) r = ens_enqueue_large(e, e->large[q], e->side[k]);
+    }
+    for (int k = 0; r == hipSuccess && k < kEnsSide; ++k) {
+        if (!used[k]) continue;
+        r = hipEventRecord(e->join_ev[k], e->side[k]);
+        if (r == hipSuccess) r = hipStreamWaitEvent(e->stream, e->join_ev[k], 0);
+    }
+    return r;
+}
+
+// Per-instance loop (the baseline the ensemble replaces): one problem after the other on one
+// stream with the single-problem kernels.
+hipError_t ens_enqueue_loop(Ensemble* e) {
+    hipError_t r = hipSuccess;
+    for (size_t i = 0; r == hipSuccess && i < e->in.size(); ++i) {
+        const EnsInst& x = e->in[i];
+        if (x.cls < 0) { r = ens_enqueue_large(e, (int)i, e->stream); continue; }
+        const size_t dd = (size_t)x.d * x.d;
+        pa::ChebArgs<float> ca;
+        ca.ops = e->d_ops + x.ops_off; ca.ncoef = (int)x.amps; ca.d = (int)x.d; ca.N = x.N; ca.M = x.M;
+        ca.s = (float)(2.0 / x.hnorm);
+        ca.ctl.c = e->d_c + x.c_off; ca.ctl.pts = (int)x.pts; ca.ctl.amps_in = (int)x.amps; ca.ctl.amps_set = (int)x.amps;
+        ca.ctl.mode = e->quad; ca.ctl.magnus = 0; ca.ctl.tau = (float)x.tau;
+        for (int k = 0; k <= x.M; ++k) { ca.are[k] = (float)x.are[k]; ca.aim[k] = (float)x.aim[k]; }
+        ca.are[0] = (float)(x.are[0] - 1.0);
+        float2* leaves = e->d_work + x.work_off;
+        ca.out = leaves;
+        r = pa::launch_cheb_fused<float>(ca, e->stream);
+        float2* s0 = leaves + (size_t)x.N * dd;
+        float2* s1 = s0 + (size_t)((x.N + 3) / 4) * dd;
+        if (r == hipSuccess) r = pa::launch_tree<float>(leaves, false, x.N, (int)x.d, s0, s1, e->d_out + x.out_off, e->stream);
+    }
+    return r;
+}
+
+float ens_time(Ensemble* e, const std::function<hipError_t()>& f, int reps) {
+    if (f() != hipSuccess) return 1e30f;   // warm-up
+    hipEventRecord(e->t0, e->stream);
+    for (int k = 0; k < reps; ++k)
+        if (f() != hipSuccess) return 1e30f;
+    hipEventRecord(e->t1, e->stream);
+    if (hipEventSynchronize(e->t1) != hipSuccess) return 1e30f;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e->t0, e->t1);
+    return ms / reps;
+}
+
+Parament_ErrorCode ens_tune(Ensemble* e) {
+    if (!e->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (hipSetDevice(e->device) != hipSuccess) return PARAMENT_FAIL;
+    if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+    for (int c = 0; c < kEnsClasses; ++c) {
+        if (!e->cls[c].n_slice) continue;
+        const int D = 16 << c;
+        float best = 1e30f;
+        for (int T = 1; T <= 2; ++T) {
+            if (!pa::ens_shape_valid(D, T)) continue;
+            for (int logb = 2; logb <= 3; ++logb)
+                for (int gm : {0, 2, 8}) {
+                    EnsCfg cf; cf.T = T; cf.logb = logb; cf.grid_mult = gm;
+                    const float t = ens_time(e, [&] { return ens_enqueue_class(e, c, cf, e->stream); }, 2);
+                    if (t < best) { best = t; e->cfg[c] = cf; }
+                }
+        }
+    }
+    for (int i : e->large) {   // output tile of each large instance's batched GEMMs
+        float best = 1e30f;
+        int best_tile = 64;
+        for (int tile : {64, 32}) {
+            e->in[i].tile = tile;
+            const float t = ens_time(e, [&] { return ens_enqueue_large(e, i, e->stream); }, 1);
+            if (t < best) { best = t; best_tile = tile; }
+        }
+        e->in[i].tile = best_tile;
+    }
+    if (!e->large.empty()) {
+        float best = 1e30f;
+        int best_nb = 1;
+        int saved[kEnsClasses];   // time the large-instance branches alone
+        for (int c = 0; c < kEnsClasses; ++c) { saved[c] = e->cls[c].n_slice; e->cls[c].n_slice = 0; }
+        for (int nb : {1, 2, 4}) {
+            if (nb > 1 && (size_t)nb > e->large.size()) continue;
+            e->branches = nb;
+            const float t = ens_time(e, [&] { return ens_enqueue(e); }, 1);
+            if (t < best) { best = t; best_nb = nb; }
+        }
+        for (int c = 0; c < kEnsClasses; ++c) e->cls[c].n_slice = saved[c];
+        e->branches = best_nb;
+    }
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    e->tuned = true;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+Parament_ErrorCode ens_run(Ensemble* e) {
+    if (!e->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (e->exec_mode == 1) return ens_enqueue_loop(e) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+    if (!e->tuned) {
+        const Parament_ErrorCode err = ens_tune(e);
+        if (err != PARAMENT_STATUS_SUCCESS) return err;
+    }
+    if (!e->gexec) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        const hipError_t r = ens_enqueue(e);
+        const hipError_t r2 = hipStreamEndCapture(e->stream, &g);
+        if (r != hipSuccess || r2 != hipSuccess || !g) { hipGetLastError(); if (g) hipGraphDestroy(g); return PARAMENT_STATUS_CUBLAS_FAILED; }
+        const hipError_t r3 = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (r3 != hipSuccess) { hipGetLastError(); e->gexec = nullptr; return PARAMENT_STATUS_CUBLAS_FAILED; }
+    }
+    return hipGraphLaunch(e->gexec, e->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1026,6 +1427,75 @@ Parament_ErrorCode Parament_setGradientCheckpointing_fp64(Parament_Context_f64* 
 }
 size_t Parament_getGradientWorkspace(Parament_Context_f32* handle) { return handle ? CTX32(handle)->grad_workspace : 0; }
 size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64* handle) { return handle ? CTX64(handle)->grad_workspace : 0; }
+
+Parament_ErrorCode Parament_ensembleCreate(Parament_Ensemble** ensemble_p, unsigned int count, const unsigned int* dims,
+                                           const unsigned int* amps, const unsigned int* pts, const double* dt,
+                                           Parament_QuadratureSpec quadrature_mode) {
+    return ens_create(reinterpret_cast<Ensemble**>(ensemble_p), count, dims, amps, pts, dt, (int)quadrature_mode);
+}
+Parament_ErrorCode Parament_ensembleDestroy(Parament_Ensemble* ensemble) { return ens_destroy(reinterpret_cast<Ensemble*>(ensemble)); }
+Parament_ErrorCode Parament_ensembleSetHamiltonians(Parament_Ensemble* ensemble, const cuComplex* H) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    return e->last_error = ens_set_hamiltonians(e, H);
+}
+Parament_ErrorCode Parament_ensembleSetExecution(Parament_Ensemble* ensemble, int mode) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    if (mode != 0 && mode != 1) return e->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    e->exec_mode = mode;
+    return e->last_error = PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_ensembleTune(Parament_Ensemble* ensemble) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    return e->last_error = ens_tune(e);
+}
+Parament_ErrorCode Parament_ensembleGetTuning(Parament_Ensemble* ensemble, int* config) {
+    if (!ensemble || !config) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    for (int c = 0; c < kEnsClasses; ++c) {
+        config[3 * c] = e->cfg[c].T; config[3 * c + 1] = e->cfg[c].logb; config[3 * c + 2] = e->cfg[c].grid_mult;
+    }
+    config[9] = e->branches;
+    return PARAMENT_STATUS_SUCCESS;
+}
+Parament_ErrorCode Parament_ensembleEquiprop(Parament_Ensemble* ensemble, const cuComplex* carr, cuComplex* out) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    if (!carr || !out) return e->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    if (hipSetDevice(e->device) != hipSuccess) return e->last_error = PARAMENT_FAIL;
+    if (e->n_c && hipMemcpyAsync(e->d_c, carr, e->n_c * sizeof(float2), hipMemcpyHostToDevice, e->stream) != hipSuccess)
+        return e->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    Parament_ErrorCode err = ens_run(e);
+    if (err == PARAMENT_STATUS_SUCCESS &&
+        (hipMemcpyAsync(out, e->d_out, e->n_out * sizeof(float2), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+         hipStreamSynchronize(e->stream) != hipSuccess))
+        err = PARAMENT_STATUS_CUBLAS_FAILED;
+    if (err != PARAMENT_STATUS_SUCCESS) hipStreamSynchronize(e->stream);
+    return e->last_error = err;
+}
+Parament_ErrorCode Parament_ensembleEquipropDevice(Parament_Ensemble* ensemble, const cuComplex* carr_dev, cuComplex* out_dev) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    if (!carr_dev || !out_dev) return e->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    if (hipSetDevice(e->device) != hipSuccess) return e->last_error = PARAMENT_FAIL;
+    if (e->n_c && hipMemcpyAsync(e->d_c, carr_dev, e->n_c * sizeof(float2), hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+        return e->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    Parament_ErrorCode err = ens_run(e);
+    if (err == PARAMENT_STATUS_SUCCESS &&
+        hipMemcpyAsync(out_dev, e->d_out, e->n_out * sizeof(float2), hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+        err = PARAMENT_STATUS_CUBLAS_FAILED;
+    return e->last_error = err;
+}
+Parament_ErrorCode Parament_ensembleSynchronize(Parament_Ensemble* ensemble) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    return hipStreamSynchronize(reinterpret_cast<Ensemble*>(ensemble)->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS
+                                                                                            : PARAMENT_STATUS_CUBLAS_FAILED;
+}
+Parament_ErrorCode Parament_ensemblePeekAtLastError(Parament_Ensemble* ensemble) {
+    return ensemble ? reinterpret_cast<Ensemble*>(ensemble)->last_error : PARAMENT_STATUS_INVALID_VALUE;
+}
 
 void device_info(void) {
     int n = 0;

@@ -1,6 +1,7 @@
 // Equiprop device kernels for gfx950 (MI355X): fused Chebyshev slice kernel,
 // single-launch pairwise product tree, and the generic step-wise kernels.
 #include "kernels.hpp"
+#include "fused_device.hpp"
 
 namespace pa {
 
@@ -17,101 +18,12 @@ namespace pa {
 // b_{k+2} are kept in registers in the accumulator layout for the epilogue.
 template <class R, int D>
 __global__ __launch_bounds__(64 * Shape<D>::G) void cheb_fused_kernel(ChebArgs<R> a) {
-    constexpr int G = Shape<D>::G, T = Shape<D>::T;
     constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
     __shared__ R Xr[D * SA], Xi[D * SA], Br[D * SB], Bi[D * SB];
     __shared__ cplx<R> coef[kMaxCoeffs];
-    const int tid = threadIdx.x, wave = tid >> 6;
-    const int row0 = Shape<D>::row0(wave), col0 = Shape<D>::col0(wave);
-    const int d = a.d, dd = d * d, M = a.M;
-    const R s = a.s;
-
-    for (int j = blockIdx.x; j < a.N; j += gridDim.x) {
-        for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, a.j0 + j);
-        __syncthreads();
-        for (int e = tid; e < D * D; e += 64 * G) {
-            const int i = e / D, k = e % D;
-            R xr = R(0), xi = R(0);
-            if (i < d && k < d) {
-                cplx<R> x = a.ops[i * d + k];
-                for (int t = 0; t < a.ncoef; ++t)
-                    x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + i * d + k]));
-                xr = x.x; xi = x.y;
-            }
-            Xr[i * SA + k] = xr; Xi[i * SA + k] = xi;
-        }
-        __syncthreads();
-
-        CTile<R, T, T> b1, b2, acc;
-        {
-            const cplx<R> aM = mk<R>(a.are[M], a.aim[M]);
-            const cplx<R> aM1 = mk<R>(a.are[M - 1], a.aim[M - 1]);
-            const cplx<R> saM = mk<R>(s * aM.x, s * aM.y);
-#pragma unroll
-            for (int m = 0; m < T; ++m)
-#pragma unroll
-                for (int n = 0; n < T; ++n)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                        const bool dg = (row == col);
-                        b2.re[m][n][v] = dg ? aM.x : R(0);
-                        b2.im[m][n][v] = dg ? aM.y : R(0);
-                        const cplx<R> t = cmul(saM, mk<R>(Xr[row * SA + col], Xi[row * SA + col]));
-                        b1.re[m][n][v] = t.x + (dg ? aM1.x : R(0));
-                        b1.im[m][n][v] = t.y + (dg ? aM1.y : R(0));
-                        Br[row * SB + col] = b1.re[m][n][v];
-                        Bi[row * SB + col] = b1.im[m][n][v];
-                    }
-        }
-        __syncthreads();
-
-        for (int k = M - 2; k >= 0; --k) {
-            wave_cgemm<R, T, T, D, SA, SB>(Xr, Xi, Br, Bi, row0, col0, acc);
-            const R beta = (k == 0) ? R(-2) : R(-1);
-            // k == 0: are[0] holds a_0 - 1 (rounded from FP64), so the result is E_j = U_j - I
-            const R akr = a.are[k], aki = a.aim[k];
-#pragma unroll
-            for (int m = 0; m < T; ++m)
-#pragma unroll
-                for (int n = 0; n < T; ++n)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const bool dg = acc_row<R>(row0, m, v) == acc_col(col0, n);
-                        const R nr = s * acc.re[m][n][v] + beta * b2.re[m][n][v] + (dg ? akr : R(0));
-                        const R ni = s * acc.im[m][n][v] + beta * b2.im[m][n][v] + (dg ? aki : R(0));
-                        b2.re[m][n][v] = b1.re[m][n][v];
-                        b2.im[m][n][v] = b1.im[m][n][v];
-                        b1.re[m][n][v] = nr;
-                        b1.im[m][n][v] = ni;
-                    }
-            if (k > 0) {
-                __syncthreads();
-#pragma unroll
-                for (int m = 0; m < T; ++m)
-#pragma unroll
-                    for (int n = 0; n < T; ++n)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                            Br[row * SB + col] = b1.re[m][n][v];
-                            Bi[row * SB + col] = b1.im[m][n][v];
-                        }
-                __syncthreads();
-            }
-        }
-        cplx<R>* out = a.out + (size_t)j * dd;
-#pragma unroll
-        for (int m = 0; m < T; ++m)
-#pragma unroll
-            for (int n = 0; n < T; ++n)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                    if (row < d && col < d) out[row * d + col] = mk<R>(b1.re[m][n][v], b1.im[m][n][v]);
-                }
-        __syncthreads();
-    }
+    for (int j = blockIdx.x; j < a.N; j += gridDim.x)
+        cheb_slice<R, D, Shape<D>>(a.ops, a.ncoef, a.d, a.ctl, a.j0 + j, a.M, a.s, a.are, a.aim, Xr, Xi, Br, Bi, coef,
+                                   a.out + (size_t)j * a.d * a.d);
 }
 
 template <class R>
@@ -143,82 +55,12 @@ template hipError_t launch_cheb_fused<double>(const ChebArgs<double>&, hipStream
 template <class R, int D, int LOGB, class SrcT>
 __global__ __launch_bounds__(64 * Shape<D>::G) void tree_block_kernel(const SrcT* __restrict__ src, int n_in, int d,
                                                                          cplx<R>* __restrict__ dst, int final_level) {
-    constexpr int G = Shape<D>::G, T = Shape<D>::T;
     constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
-    constexpr int B = 1 << LOGB;
     __shared__ R Ar[D * SA], Ai[D * SA], Br[D * SB], Bi[D * SB];
-    const int tid = threadIdx.x, wave = tid >> 6;
-    const int row0 = Shape<D>::row0(wave), col0 = Shape<D>::col0(wave);
-    const int dd = d * d;
-    const long base = (long)blockIdx.x * B;
-
-    auto stage = [&](long j, R* Pr, R* Pi, int S) {
-        const bool live = j < n_in;
-        const SrcT* p = src + (size_t)(live ? j : 0) * dd;
-        for (int e = tid; e < D * D; e += 64 * G) {
-            const int i = e / D, k = e % D;
-            R xr = R(0), xi = R(0);
-            if (live && i < d && k < d) { xr = R(p[i * d + k].x); xi = R(p[i * d + k].y); }
-            Pr[i * S + k] = xr; Pi[i * S + k] = xi;
-        }
-    };
-    auto tile_to_lds = [&](const CTile<R, T, T>& t, R* Pr, R* Pi, int S) {
-#pragma unroll
-        for (int m = 0; m < T; ++m)
-#pragma unroll
-            for (int n = 0; n < T; ++n)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                    Pr[row * S + col] = t.re[m][n][v];
-                    Pi[row * S + col] = t.im[m][n][v];
-                }
-    };
-    // out = A + B + A B from the staged planes (A later, B earlier)
-    auto combine_staged = [&](CTile<R, T, T>& out) {
-        __syncthreads();
-        wave_cgemm<R, T, T, D, SA, SB>(Ar, Ai, Br, Bi, row0, col0, out);
-#pragma unroll
-        for (int m = 0; m < T; ++m)
-#pragma unroll
-            for (int n = 0; n < T; ++n)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                    out.re[m][n][v] += Ar[row * SA + col] + Br[row * SB + col];
-                    out.im[m][n][v] += Ai[row * SA + col] + Bi[row * SB + col];
-                }
-        __syncthreads();
-    };
-
-    CTile<R, T, T> st[LOGB > 1 ? LOGB : 1];  // st[l]: completed level-l node waiting for its sibling
-    CTile<R, T, T> cur;
-#pragma unroll
-    for (int p = 0; p < B / 2; ++p) {
-        stage(base + 2 * p + 1, Ar, Ai, SA);
-        stage(base + 2 * p, Br, Bi, SB);
-        combine_staged(cur);  // level-1 node p
-#pragma unroll
-        for (int l = 1; l < LOGB; ++l) {
-            if (((p >> (l - 1)) & 1) == 0) { st[l] = cur; break; }
-            tile_to_lds(cur, Ar, Ai, SA);
-            tile_to_lds(st[l], Br, Bi, SB);
-            combine_staged(cur);
-        }
-    }
-    cplx<R>* out = dst + (size_t)blockIdx.x * dd;
-    const R diag = final_level ? R(1) : R(0);
-#pragma unroll
-    for (int m = 0; m < T; ++m)
-#pragma unroll
-        for (int n = 0; n < T; ++n)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                if (row < d && col < d)
-                    out[row * d + col] = mk<R>(cur.re[m][n][v] + (row == col ? diag : R(0)), cur.im[m][n][v]);
-            }
+    tree_block<R, D, LOGB, SrcT, Shape<D>>(src, n_in, d, dst + (size_t)blockIdx.x * d * d, final_level, blockIdx.x, Ar,
+                                           Ai, Br, Bi);
 }
+
 
 template <class R> constexpr int tree_logb() { return sizeof(R) == 4 ? 3 : 2; }
 

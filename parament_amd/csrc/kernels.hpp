@@ -76,8 +76,31 @@ struct GemmArgs {
     double alpha_re, alpha_im, beta_re, beta_im, diag_re, diag_im;
     int d, batch, add_ab;
     int a_c128, b_c128, out_c128;
+    int tile = 0;          // output tile 32 or 64; 0 chooses by padded volume
 };
 hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st);
+
+// Ensemble kernels (ensemble.hip): per-instance parameters in a device table, task tables
+// of (instance, slice) and (level descriptor, block); complex64.  Dim classes D = 16, 32, 64
+// (instances padded up), wave split T (TileShape<D, T>), tree block size 2^logb (2 or 3).
+struct EnsInstance {
+    const float2* ops;     // (1 + ncoef) d x d: H0, H_1..
+    const float2* c;       // amps x pts controls
+    float2* leaves;        // N x d x d, E_j = U_j - I
+    int d, ncoef, amps, pts, mode, M;
+    float s, tau;
+    float are[kMaxCheb], aim[kMaxCheb];   // are[0] = a_0 - 1
+};
+struct EnsLevel {
+    const float2* src;     // n_in nodes of one instance's tree level (E form)
+    float2* dst;           // ceil(n_in / 2^logb) nodes; the instance's U (= I + E) when final_level
+    int n_in, d, final_level;
+};
+bool ens_shape_valid(int D, int T);
+hipError_t launch_ens_slices(int D, int T, const EnsInstance* inst, const int2* tasks, int ntasks, int grid,
+                             hipStream_t st);
+hipError_t launch_ens_tree(int D, int T, int logb, const EnsLevel* lv, const int2* tasks, int ntasks, int grid,
+                           hipStream_t st);
 
 // Adjoint gate-fidelity gradient (grad.hip), dims <= 32.  Tree levels are stored as
 // consecutive d x d matrices; see grad.hip for the scan.

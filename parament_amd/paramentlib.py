@@ -94,6 +94,18 @@ for sfx, cptr, real in (("", c_cuComplex_p, np.float32), ("_fp64", c_cuDoubleCom
     getattr(lib, "Parament_getGradientWorkspace" + sfx).argtypes = [c_ParamentContext_p]
     getattr(lib, "Parament_getGradientWorkspace" + sfx).restype = ctypes.c_size_t
 lib.Parament_setPrecisionMode_fp64.argtypes = [c_ParamentContext_p, ctypes.c_int]
+_uint_p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+lib.Parament_ensembleCreate.argtypes = [ctypes.POINTER(c_ParamentContext_p), ctypes.c_uint, _uint_p, _uint_p, _uint_p,
+                                        np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"), ctypes.c_int]
+lib.Parament_ensembleDestroy.argtypes = [c_ParamentContext_p]
+lib.Parament_ensembleSetHamiltonians.argtypes = [c_ParamentContext_p, c_cuComplex_p]
+lib.Parament_ensembleSetExecution.argtypes = [c_ParamentContext_p, ctypes.c_int]
+lib.Parament_ensembleTune.argtypes = [c_ParamentContext_p]
+lib.Parament_ensembleGetTuning.argtypes = [c_ParamentContext_p, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
+lib.Parament_ensembleEquiprop.argtypes = [c_ParamentContext_p, c_cuComplex_p, c_cuComplex_p]
+lib.Parament_ensembleEquipropDevice.argtypes = [c_ParamentContext_p, ctypes.c_void_p, ctypes.c_void_p]
+lib.Parament_ensembleSynchronize.argtypes = [c_ParamentContext_p]
+lib.Parament_ensemblePeekAtLastError.argtypes = [c_ParamentContext_p]
 lib.Parament_errorMessage.argtypes = [ctypes.c_int]
 lib.Parament_errorMessage.restype = ctypes.c_char_p
 lib.Parament_selectIterationCycles_fp32.argtypes = [ctypes.c_double, ctypes.c_double]

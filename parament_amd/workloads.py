@@ -88,3 +88,15 @@ def gue32_grape(seed=20260004, pts=4096):
     h = [gue(rng, 32) for _ in range(5)]
     c = rng.uniform(-1, 1, (4, pts))
     return h[0], h[1:], c, 2e-3, "none", haar_unitary(rng, 32)
+
+
+def ensemble512(seed=20260005, count=512, dt=1e-2):
+    """512 independent systems: dim = round(exp(U[ln 4, ln 128])), 1-4 controls, 256-8192 steps,
+    GUE H0 and controls (spectral norm 1), controls uniform in [-1, 1], no quadrature."""
+    rng = np.random.default_rng(seed)
+    dims = np.rint(np.exp(rng.uniform(np.log(4), np.log(128), count))).astype(int)
+    amps = rng.integers(1, 5, count)
+    pts = rng.integers(256, 8193, count)
+    hs = [[gue(rng, int(d)) for _ in range(int(a) + 1)] for d, a in zip(dims, amps)]
+    cs = [rng.uniform(-1, 1, (int(a), int(p))) for a, p in zip(amps, pts)]
+    return dims, amps, pts, dt, hs, cs
