@@ -13,7 +13,11 @@ with HIP events on the stream it runs on; ``cpu_baseline`` times the numpy FP64
 restatement of the reference algorithm (oracle/) on a bounded sample of the same
 workload on the host cores.  ``grape_grads_per_s`` is the rate of complete FP64 gradients
 (F and every dF/dc) on the GRAPE workload (dim 32, 4 controls, 4096 steps); ``grape`` holds
-both precisions, both memory layouts and the gradient / forward cost ratio.
+both precisions, both memory layouts and the gradient / forward cost ratio;
+``precision_variant_dim256`` times the dim-256 Ising workload in FP64, the three
+mixed-precision product modes and complex64, with each one's error against FP64;
+``ensemble`` times 512 independent systems of dims 4-128 as one tuned graph and as the
+per-instance loop.
 """
 from __future__ import annotations
 
@@ -31,10 +35,19 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "mixed": 2500.0 / 6.0}  # mixed: bf16 dense / 6 split products
 
 
-def slice_flops(d, m, ncoef, n):
-    """Algorithmic flops of the slice kernel: M-1 complex d^3 GEMMs (the reference's first
-    Clenshaw GEMM multiplies by a_M I, a scaling) and the assembly, 8 flops per complex MAC."""
-    return n * (8.0 * d ** 3 * (m - 1) + 8.0 * d * d * ncoef)
+def fused_levels(path, n):
+    """Tree levels the mixed slice kernel folds in (the library's rule for N slices)."""
+    if path != "mixed":
+        return 0
+    return 3 if n >= 4096 else 2 if n >= 1024 else 1 if n >= 256 else 0
+
+
+def slice_flops(d, m, ncoef, n, levels=0):
+    """Algorithmic flops of the slice kernel: M-1 complex d^3 GEMMs per slice (the reference's
+    first Clenshaw GEMM multiplies by a_M I, a scaling), the assembly, and one d^3 product per
+    node of the tree levels folded into the kernel; 8 flops per complex MAC."""
+    nodes = sum((n + (1 << l) - 1) >> l for l in range(1, levels + 1))
+    return n * (8.0 * d ** 3 * (m - 1) + 8.0 * d * d * ncoef) + nodes * 8.0 * d ** 3
 
 
 def run_path(name, prec, mixed, h0, hk, c, dt, quad, steps, warmup):
@@ -126,6 +139,81 @@ def grape_bench(steps, warmup):
     return res
 
 
+def precision_variant(calls):
+    """Dim-256 precision study: 8-qubit transverse-field Ising chain (J = 1, h ~ U[0, 2]) with 8
+    local sigma_x controls, Gaussian-smoothed N(0, 1) controls, 16,384 steps, dt 2e-3, no
+    quadrature.  Time per equipropDevice call and Frobenius error against the FP64 path, for
+    FP64, the three mixed-precision product modes (FP64 first-order terms and upper tree
+    levels) and plain complex64."""
+    import torch
+    import parament_amd as parament
+    from parament_amd import workloads
+    h0, hk, c, dt, quad = workloads.ising8()
+    amps, pts = c.shape
+    d = h0.shape[0]
+    dev = torch.device("cuda:0")
+    out, ref = {}, None
+    for name, prec, mode in (("fp64", "fp64", "native"), ("mixed_bf16x3", "fp64", "mixed"),
+                             ("mixed_fp32", "fp64", "mixed_fp32"), ("mixed_bf16x2", "fp64", "mixed_bf16x2"),
+                             ("fp32", "fp32", None)):
+        ct = torch.complex128 if prec == "fp64" else torch.complex64
+        carr = torch.tensor(c, dtype=ct, device=dev).contiguous()
+        u = torch.empty((d, d), dtype=ct, device=dev)
+        ctx = parament.Parament(prec)
+        if mode:
+            ctx.set_precision_mode(mode)
+        ctx.set_hamiltonian(h0, *hk, quadrature_mode=quad)
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        call = lambda: ctx.equiprop_device(dt, carr.data_ptr(), pts, amps, u.data_ptr())
+        call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            call()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / calls
+        un = u.cpu().numpy().astype(np.complex128)
+        ref = un if ref is None else ref
+        out[name] = {"ms_per_call": t * 1e3, "steps_per_s": pts / t, "M": ctx.iteration_cycles,
+                     "rel_err_vs_fp64": float(np.linalg.norm(un - ref) / np.linalg.norm(ref))}
+        ctx.destroy()
+    out["workload"] = "ising8: dim 256, 8 controls, 16384 steps, dt 2e-3, no quadrature (16384 slices)"
+    return out
+
+
+def ensemble_bench(calls):
+    """512 independent complex64 systems (dims round(exp(U[ln 4, ln 128])), 1-4 controls, 256-8192
+    steps, dt 1e-2): time per solve of the whole ensemble as one tuned hipGraph and as the
+    per-instance loop, with the total steps per second of each."""
+    import torch
+    from parament_amd.ensemble import Ensemble
+    from parament_amd import workloads
+    dims, amps, pts, dt, hs, cs = workloads.ensemble512()
+    e = Ensemble(dims, amps, pts, dt)
+    e.set_hamiltonians(hs)
+    carr = torch.from_numpy(e._pack_controls(cs)).to("cuda:0")
+    out = torch.empty(int(np.sum(dims.astype(np.int64) ** 2)), dtype=torch.complex64, device="cuda:0")
+    res = {}
+    for mode in ("loop", "graph"):
+        e.set_execution(mode)
+        t0 = time.perf_counter()
+        e.equiprop_device(carr.data_ptr(), out.data_ptr())   # graph: tunes and captures
+        e.synchronize()
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            e.equiprop_device(carr.data_ptr(), out.data_ptr())
+        e.synchronize()
+        t = (time.perf_counter() - t0) / calls
+        res[mode] = {"ms_per_solve": t * 1e3, "steps_per_s": float(pts.sum()) / t, "first_call_s": first}
+    res["graph_speedup_vs_loop"] = res["loop"]["ms_per_solve"] / res["graph"]["ms_per_solve"]
+    res["tuning"] = {str(k): [int(x) for x in v] if isinstance(v, tuple) else v for k, v in e.tuning.items()}
+    res["workload"] = (f"ensemble512: {len(dims)} systems, dims 4-128 ({int((dims > 64).sum())} above 64), "
+                       f"{int(pts.sum())} steps in total, complex64")
+    e.destroy()
+    return res
+
+
 def cpu_baseline(h0, hk, c, dt, quad, sample_pts):
     from oracle import equiprop_oracle as eo
     try:
@@ -148,18 +236,46 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample-pts", type=int, default=8193)
+    ap.add_argument("--cpu-sample-pts", type=int, default=49153)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["all", "fp64", "mixed", "fp32", "grape", "precision", "ensemble"], default="all",
+                    help="run one part only (for profiling); prints that part's measurements")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (a known-size kernel for calibrating "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build's equiprop path runs on a single device "
               f"(--gpus {args.gpus} is not supported; the multi-GPU sharding is design-only).", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe_mb:
+        import torch
+        x = torch.empty(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+
     from parament_amd import workloads
     h0, hk, c, dt, quad = workloads.heisenberg6()
     d, amps, pts = h0.shape[0], len(hk), c.shape[1]
     n_slices = (pts - 1) // 2
+    if args.only != "all":
+        part = {}
+        if args.only in ("fp64", "mixed", "fp32"):
+            r = run_path(args.only, "fp32" if args.only == "fp32" else "fp64", args.only == "mixed", h0, hk, c, dt, quad,
+                         args.steps, args.warmup)
+            part = {k: v for k, v in r.items() if k != "u"}
+        elif args.only == "grape":
+            part = grape_bench(args.steps, args.warmup)
+        elif args.only == "precision":
+            part = precision_variant(args.steps)
+        else:
+            part = ensemble_bench(args.steps)
+        print(json.dumps({"only": args.only, **part}), flush=True)
+        return
 
     runs = [run_path(*p, h0, hk, c, dt, quad, args.steps, args.warmup)
             for p in (("fp64", "fp64", False), ("mixed", "fp64", True), ("fp32", "fp32", False))]
@@ -167,7 +283,7 @@ def main():
     paths = {}
     for r in runs:
         err = float(np.linalg.norm(r["u"] - u64) / np.linalg.norm(u64))
-        fl = slice_flops(d, r["M"], amps, n_slices)
+        fl = slice_flops(d, r["M"], amps, n_slices, fused_levels(r["name"], n_slices))
         ach = fl / (r["slice_ms"] * 1e-3) / 1e12
         paths[r["name"]] = {"steps_per_s": pts / r["s_per_call"], "ms_per_call": r["s_per_call"] * 1e3,
                             "rel_err_vs_fp64": err, "M": r["M"], "slice_kernel_ms": r["slice_ms"],
@@ -176,7 +292,7 @@ def main():
     ok = [k for k in ("mixed", "fp32", "fp64") if paths[k]["rel_err_vs_fp64"] <= 1e-5]
     best = max(ok, key=lambda k: paths[k]["steps_per_s"])
     p = paths[best]
-    fl = slice_flops(d, p["M"], amps, n_slices)
+    fl = slice_flops(d, p["M"], amps, n_slices, fused_levels(best, n_slices))
     line = {
         "metric": "steps/s at dim 64 @<=1e-5 vs FP64 (mixed vs plain FP64); GRAPE grads/s",
         "value": p["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -194,6 +310,8 @@ def main():
     grape = grape_bench(args.steps, args.warmup)
     line["grape_grads_per_s"] = grape["fp64"]["full_tree"]["grads_per_s"]
     line["grape"] = grape
+    line["precision_variant_dim256"] = precision_variant(3)
+    line["ensemble"] = ensemble_bench(3)
     if not args.no_cpu:
         ucpu, cb = cpu_baseline(h0, hk, c, dt, quad, args.cpu_sample_pts)
         line["cpu_baseline"] = cb

@@ -214,7 +214,7 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
 
     // E_j = U_j - I of slice j into e (accumulator layout, FP32); jn is the slice this
     // work-group runs next (or -1).
-    auto slice = [&](int j, int jn, Tile& e) {
+    auto slice = [&](int j, int jn, Tile& e) __attribute__((always_inline)) {
         if (!coef_ready)
             for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
         __syncthreads();
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
     };
 
     // out = A + B + A B for the later node A and the earlier node B (split product).
-    auto combine = [&](const Tile& A, const Tile& B, Tile& out) {
+    auto combine = [&](const Tile& A, const Tile& B, Tile& out) __attribute__((always_inline)) {
         write_split<D, false>(A, Xp, row0, col0);
         write_split<D, true>(B, Bp, row0, col0);
         __syncthreads();
@@ -379,36 +379,48 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
     };
 
     for (int w = blockIdx.x; w < n_nodes; w += gridDim.x) {
+        // next slice this work-group runs after slice j: j + 1 inside the block, else the first
+        // slice of its next block (or none)
+        auto next_of = [&](int j) -> int {
+            if (j + 1 < a.N && j + 1 < (w + 1) * NB) return j + 1;
+            const int nb = (w + (int)gridDim.x) * NB;
+            return nb < a.N ? nb : -1;
+        };
+        // Binary counter over the block's slices p = 0 .. 2^F - 1 (unrolled, so every stack index is
+        // a constant): slice p completes the level-l nodes for the trailing one bits of p, each
+        // the product of the pending earlier sibling stk[l] and the newer node (later on the left).
         Tile stk[F > 0 ? F : 1];
         Tile node;
 #pragma unroll
         for (int p = 0; p < NB; ++p) {
             const int j = w * NB + p;
-            if (j >= a.N) break;   // the rest of the block are zero nodes
-            Tile e;
-            // next slice of this work-group: j + 1 inside the block, else the first of its next block
-            const int jn = (p + 1 < NB && j + 1 < a.N) ? j + 1 : ((w + (int)gridDim.x) * NB < a.N ? (w + (int)gridDim.x) * NB : -1);
-            slice(j, jn, e);
-            // binary counter over p: merge completed siblings (later node on the left)
-            int l = 0;
+            if (j < a.N) {   // uniform
+                Tile e;
+                slice(j, next_of(j), e);
+                int merged = 0;   // levels completed by slice p: the trailing one bits of p
 #pragma unroll
-            for (; l < F; ++l) {
-                if (((p >> l) & 1) == 0) { stk[l] = e; break; }
-                Tile m;
-                combine(e, stk[l], m);
-                e = m;
+                for (int l = 0; l < F; ++l) {
+                    const int mask = (2 << l) - 1;
+                    if ((p & mask) == mask) { Tile m; combine(e, stk[l], m); e = m; merged = l + 1; }
+                }
+                if (merged == F) {
+                    node = e;
+                } else {
+#pragma unroll
+                    for (int l = 0; l < F; ++l)
+                        if (l == merged) stk[l] = e;
+                }
             }
-            if (l == F) node = e;
         }
-        // a ragged block: the pending nodes stk[l], l a set bit of the live count, fold from the
-        // lowest level up (each is earlier than everything folded before it)
+        // a ragged block: the pending nodes stk[l] for the set bits l of the live count fold from
+        // the lowest level up (each earlier than everything folded before it)
         const int live = min(a.N - w * NB, NB);
         if (live < NB) {
-            int have = 0;
+            bool have = false;
 #pragma unroll
             for (int l = 0; l < F; ++l) {
                 if ((live >> l) & 1) {
-                    if (!have) { node = stk[l]; have = 1; }
+                    if (!have) { node = stk[l]; have = true; }
                     else { Tile m; combine(node, stk[l], m); node = m; }
                 }
             }
