@@ -34,6 +34,11 @@ sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "mixed": 2500.0 / 6.0}  # mixed: bf16 dense / 6 split products
 
+# HBM bytes per launch of the slice kernel on this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes over `bench.py --only mixed`, each scaled by the counter's reading for a
+# 1 GiB device copy in the same run (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 249 MB + 242 MB.
+MEASURED_TRAFFIC_BYTES = {"mixed": 4.91e8}
+
 
 def fused_levels(path, n):
     """Tree levels the mixed slice kernel folds in (the library's rule for N slices)."""
@@ -302,7 +307,8 @@ def main():
         "config": {"workload": "heisenberg6: dim 64, 3 controls, 65536 steps, dt 5e-3, simpson (32767 slices)",
                    "path": best, "dim": d, "controls": amps, "pts": pts, "slices": n_slices},
         "roofline": {"bound": "mfma", "achieved": p["slice_kernel_tflops"], "peak": PEAK_TFLOPS[best],
-                     "unit": "TFLOP/s", "frac": p["slice_kernel_frac_of_peak"], "traffic": None,
+                     "unit": "TFLOP/s", "frac": p["slice_kernel_frac_of_peak"],
+                     "traffic": MEASURED_TRAFFIC_BYTES.get(best),
                      "kernel": "Chebyshev slice kernel", "flops_per_launch": fl},
         "speedup_mixed_vs_fp64": paths["mixed"]["steps_per_s"] / paths["fp64"]["steps_per_s"],
         "paths": paths,
