@@ -82,6 +82,45 @@ void bessel(int m, double x, double* re, double* im) {
     }
 }
 
+// Power-form coefficients of the Chebyshev approximant: with y = X / Hnorm and the series
+// p(y) = a_0 + 2 sum_{k>=1} a_k T_k(y) that the Clenshaw recurrence evaluates, returns m_k with
+// p = sum_k m_k Y^k for Y = tau X = x y (x = tau Hnorm), in FP64; mr[0] holds m_0 - 1.
+void power_coefficients(int m, double x, const double* are, const double* aim, double* mr, double* mi) {
+    std::vector<double> tprev(m + 1, 0.0), tcur(m + 1, 0.0), tnext(m + 1, 0.0);
+    std::vector<double> cr(m + 1, 0.0), ci(m + 1, 0.0);
+    tprev[0] = 1.0;   // T_0
+    tcur[1] = 1.0;    // T_1
+    cr[0] += are[0]; ci[0] += aim[0];
+    for (int k = 1; k <= m; ++k) {
+        for (int i = 0; i <= k; ++i) { cr[i] += 2.0 * are[k] * tcur[i]; ci[i] += 2.0 * aim[k] * tcur[i]; }
+        if (k == m) break;
+        std::fill(tnext.begin(), tnext.end(), 0.0);   // T_{k+1} = 2 y T_k - T_{k-1}
+        for (int i = 0; i <= k; ++i) tnext[i + 1] += 2.0 * tcur[i];
+        for (int i = 0; i <= k - 1; ++i) tnext[i] -= tprev[i];
+        tprev.swap(tcur);
+        tcur.swap(tnext);
+    }
+    double xp = 1.0;
+    for (int k = 0; k <= m; ++k, xp *= x) { mr[k] = cr[k] / xp; mi[k] = ci[k] / xp; }
+    mr[0] -= 1.0;
+}
+
+// Split level of each inner product Y^2 Q_{j+1} of the power form (j = 1 .. (M-1)/2 - 1), two
+// bits per j.  The product enters U with weight <= x^(2j+2) / (2j+2)! (|Y| <= x = tau Hnorm
+// on the series' domain, Q_{j+1} ~ m_{2j+2}); a level dropping the slice pairs p + q > L has
+// relative error ~2^-8(L+1), and the level is the lowest whose error stays below 1e-8 -- an
+// unbiased rounding-size error, far under the FP32 rounding of the stored slice itself.
+int power_split_levels(int m, double x) {
+    int bits = 0;
+    double w = x * x / 2.0;   // x^(2j+2) / (2j+2)! for j = 0
+    for (int j = 1; j < (m - 1) / 2; ++j) {
+        w *= x * x / ((2.0 * j + 1.0) * (2.0 * j + 2.0));
+        const int lv = (w * 0x1p-8 <= 1e-8) ? 0 : (w * 0x1p-16 <= 1e-8 ? 1 : 2);
+        bits |= lv << (2 * j);
+    }
+    return bits;
+}
+
 // Chebyshev order of the mixed-precision path.  The FP32 table bounds the per-slice tail
 // 2 sum_{k>M} |J_k(x)| at ~0.05 eps32; that truncation error is the same for every slice and
 // adds up over the product, so the mixed path (whose target is 1e-5 against FP64) also
@@ -474,6 +513,15 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         ma.N = N; ma.M = M; ma.s = (double)sf;
         for (int k = 0; k <= M; ++k) { ma.are[k] = bre[k]; ma.aim[k] = bim[k]; }
         ma.are[0] = bre[0] - 1.0;
+        // Small tau Hnorm: evaluate the same degree-M polynomial in powers of Y = tau X, which
+        // needs (M + 1) / 2 products instead of M - 1 (the power coefficients are then close to
+        // (-i)^k / k! and the form is well conditioned).
+        const double xarg = tau * hn2;
+        ma.ps = (xarg <= 1.0) ? 1 : 0;
+        if (ma.ps) {
+            power_coefficients(M, xarg, bre, bim, ma.are, ma.aim);
+            ma.ps_levels = power_split_levels(M, xarg);
+        }
         // the slice kernel folds the lowest F tree levels; the FP64 tree takes its level-F nodes
         const int F = pa::mixed_fused_levels(N);
         const int n_nodes = (N + (1 << F) - 1) >> F;

@@ -32,8 +32,10 @@ struct MixedArgs {
     int N;
     int M;
     double s;
-    double are[kMaxCheb], aim[kMaxCheb];   // are[0] holds a_0 - 1
+    double are[kMaxCheb], aim[kMaxCheb];   // are[0] holds a_0 - 1 (ps: m_0 - 1)
     float2* out;           // ceil(N / 2^F) x d x d
+    int ps = 0;            // 1: are/aim hold the power-form coefficients m_k of Y = tau X
+    int ps_levels = -1;    // power form: bits 2j, 2j+1 = split level (0..2) of the product Y^2 Q_{j+1}
 };
 
 // Fused Chebyshev slice kernel, dims <= 64 (padded to 16 / 32 / 64).

@@ -46,8 +46,19 @@ template <int D> struct MShape {
     static constexpr int WR = D / 16;        // wave rows
     static constexpr int WC = D / (16 * TN); // wave columns
     static constexpr int G = WR * WC;        // waves per slice
-    static constexpr int S = D + 8;          // plane row stride (bf16 elements), 16-byte rows
-    static constexpr int PL = D * S;         // plane size
+    // Plane layout.  Dim 64: unpadded 128-byte rows whose 16-byte chunks are permuted by
+    // chunk ^ (row & 7), so that the 16-byte fragment reads of 16 rows and the 8-byte
+    // transposed tile writes both spread over the banks (measured: 1.6x the fragment-read rate
+    // of 144-byte padded rows).  Dim 32: rows padded by 16 bytes.
+    static constexpr bool SWZ = (D == 64);
+    static constexpr int S = SWZ ? D : D + 8;  // row stride (bf16 elements)
+    static constexpr int PL = D * S;           // plane size
+    // element offset of (row, col); col and col + 1 .. col + 7 within one aligned group of 8
+    // stay contiguous
+    static __device__ __forceinline__ int off(int row, int col) {
+        if constexpr (SWZ) return row * D + ((((col >> 3) ^ row) & 7) << 3) + (col & 7);
+        else return row * S + col;
+    }
 };
 
 // Three-way bf16 split of two FP32 values (one packed conversion per slice pair).
@@ -83,10 +94,12 @@ __device__ __forceinline__ bf16x8 negate(bf16x8 v) {
 // C = A * B on the wave's 16 x 16TN tile.  A slice planes are row-major, B slice planes
 // transposed (Bt[col][k]); plane 2 * slice + {0: re, 1: im}.  Operand fragments of the
 // 16x16x32 instruction: lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7].
-template <int D>
+// LV is the largest p + q of the slice pairs summed: 2 (six products, FP32 accurate), 1 (three
+// products, relative error ~2^-16) or 0 (the leading product only, ~2^-8).
+template <int D, int LV = 2>
 __device__ __forceinline__ void wave_cgemm_split(const __bf16* __restrict__ Ap, const __bf16* __restrict__ Bp,
                                                  int row0, int col0, LTile<MShape<D>::TN>& c) {
-    constexpr int S = MShape<D>::S, PL = MShape<D>::PL, TN = MShape<D>::TN;
+    constexpr int PL = MShape<D>::PL, TN = MShape<D>::TN;
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, kq = lane >> 4;
 #pragma unroll
@@ -97,27 +110,27 @@ __device__ __forceinline__ void wave_cgemm_split(const __bf16* __restrict__ Ap, 
             for (int v = 0; v < 4; ++v) { c.re[L][n][v] = 0.f; c.im[L][n][v] = 0.f; }
 #pragma unroll 1
     for (int ks = 0; ks < D / 32; ++ks) {
-        const int ka = (row0 + r) * S + 32 * ks + 8 * kq;
-        bf16x8 Br[TN][3], Bi[TN][3];
+        const int ka = MShape<D>::off(row0 + r, 32 * ks + 8 * kq);
+        bf16x8 Br[TN][LV + 1], Bi[TN][LV + 1];
 #pragma unroll
         for (int n = 0; n < TN; ++n) {
-            const int kb = (col0 + 16 * n + r) * S + 32 * ks + 8 * kq;
+            const int kb = MShape<D>::off(col0 + 16 * n + r, 32 * ks + 8 * kq);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
+            for (int q = 0; q <= LV; ++q) {
                 Br[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q) * PL + kb);
                 Bi[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q + 1) * PL + kb);
             }
         }
-        // A slice p against the B slices q <= 2 - p (slice pairs with p + q <= 2)
+        // A slice p against the B slices q <= LV - p (slice pairs with p + q <= LV)
 #pragma unroll
-        for (int p = 0; p < 3; ++p) {
+        for (int p = 0; p <= LV; ++p) {
             const bf16x8 Ar = *reinterpret_cast<const bf16x8*>(Ap + (2 * p) * PL + ka);
             const bf16x8 Ai = *reinterpret_cast<const bf16x8*>(Ap + (2 * p + 1) * PL + ka);
             const bf16x8 An = negate(Ai);
 #pragma unroll
             for (int n = 0; n < TN; ++n)
 #pragma unroll
-                for (int q = 0; q + p <= 2; ++q) {
+                for (int q = 0; q + p <= LV; ++q) {
                     const int L = (p + q) > 0 ? 1 : 0;
                     c.re[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Br[n][q], c.re[L][n], 0, 0, 0);
                     c.im[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Bi[n][q], c.im[L][n], 0, 0, 0);
@@ -133,7 +146,7 @@ __device__ __forceinline__ void wave_cgemm_split(const __bf16* __restrict__ Ap, 
 // block.
 template <int D, bool TRANSPOSED>
 __device__ __forceinline__ void write_split(const MTile<MShape<D>::TN>& b, __bf16* __restrict__ P, int row0, int col0) {
-    constexpr int S = MShape<D>::S, PL = MShape<D>::PL, TN = MShape<D>::TN;
+    constexpr int PL = MShape<D>::PL, TN = MShape<D>::TN;
     const int rbase = mrow(row0, 0);
 #pragma unroll
     for (int n = 0; n < TN; ++n) {
@@ -147,13 +160,13 @@ __device__ __forceinline__ void write_split(const MTile<MShape<D>::TN>& b, __bf1
             if (TRANSPOSED) {
                 const bf16x4 h = {h0[0], h0[1], h1[0], h1[1]}, m = {m0[0], m0[1], m1[0], m1[1]},
                              l = {l0[0], l0[1], l1[0], l1[1]};
-                *reinterpret_cast<bf16x4*>(P + (0 + c) * PL + col * S + rbase) = h;
-                *reinterpret_cast<bf16x4*>(P + (2 + c) * PL + col * S + rbase) = m;
-                *reinterpret_cast<bf16x4*>(P + (4 + c) * PL + col * S + rbase) = l;
+                *reinterpret_cast<bf16x4*>(P + (0 + c) * PL + MShape<D>::off(col, rbase)) = h;
+                *reinterpret_cast<bf16x4*>(P + (2 + c) * PL + MShape<D>::off(col, rbase)) = m;
+                *reinterpret_cast<bf16x4*>(P + (4 + c) * PL + MShape<D>::off(col, rbase)) = l;
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int o = (rbase + q) * S + col;
+                    const int o = MShape<D>::off(rbase + q, col);
                     const bf16x2& hh = (q < 2) ? h0 : h1;
                     const bf16x2& mm = (q < 2) ? m0 : m1;
                     const bf16x2& ll = (q < 2) ? l0 : l1;
@@ -190,9 +203,9 @@ __device__ __forceinline__ double2 coef_finish(const Controls<double>& q, const 
     return mk<double>(s.x * sixth, s.y * sixth);
 }
 
-template <int D, int F>
+template <int D, int F, bool PS>
 __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs a) {
-    constexpr int G = MShape<D>::G, S = MShape<D>::S, PL = MShape<D>::PL, WC = MShape<D>::WC;
+    constexpr int G = MShape<D>::G, PL = MShape<D>::PL, WC = MShape<D>::WC;
     constexpr int TN = MShape<D>::TN;
     constexpr int NB = 1 << F;
     typedef MTile<TN> Tile;
@@ -243,17 +256,36 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
                         x1 = cadd(x1, cmul(coef[t0 + q], o1[q]));
                     }
             }
+            if constexpr (PS) {   // the power form works on Y = tau X
+                const double tau = a.ctl.tau;
+                x0.x *= tau; x0.y *= tau; x1.x *= tau; x1.y *= tau;
+            }
             bf16x2 p[6];
             split3((float)x0.x, (float)x1.x, p[0], p[2], p[4]);
             split3((float)x0.y, (float)x1.y, p[1], p[3], p[5]);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) *reinterpret_cast<bf16x2*>(Xp + q * PL + i * S + k) = p[q];
-            // residuals for the FP64 first-order term (the sum of the three slices is exact in FP64)
-            auto res = [&](double x, int h, int c) -> float {
-                return (float)(x - (((double)(float)p[4 + c][h] + (double)(float)p[2 + c][h]) + (double)(float)p[c][h]));
-            };
-            Rx[i * SR + k] = make_float2(res(x0.x, 0, 0), res(x0.y, 0, 1));
-            Rx[i * SR + k + 1] = make_float2(res(x1.x, 1, 0), res(x1.y, 1, 1));
+            for (int q = 0; q < 6; ++q) *reinterpret_cast<bf16x2*>(Xp + q * PL + MShape<D>::off(i, k)) = p[q];
+            if constexpr (PS) {
+                // Y also as the transposed B operand (for Y Y), and the residual of the FP32 sum
+                // ((s2 + s1) + s0) that the power form keeps in registers
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    Bp[q * PL + MShape<D>::off(k, i)] = p[q][0];
+                    Bp[q * PL + MShape<D>::off(k + 1, i)] = p[q][1];
+                }
+                auto res = [&](double x, int h, int c) -> float {
+                    return (float)(x - (double)(((float)p[4 + c][h] + (float)p[2 + c][h]) + (float)p[c][h]));
+                };
+                Rx[i * SR + k] = make_float2(res(x0.x, 0, 0), res(x0.y, 0, 1));
+                Rx[i * SR + k + 1] = make_float2(res(x1.x, 1, 0), res(x1.y, 1, 1));
+            } else {
+                // residuals for the FP64 first-order term (the sum of the three slices is exact in FP64)
+                auto res = [&](double x, int h, int c) -> float {
+                    return (float)(x - (((double)(float)p[4 + c][h] + (double)(float)p[2 + c][h]) + (double)(float)p[c][h]));
+                };
+                Rx[i * SR + k] = make_float2(res(x0.x, 0, 0), res(x0.y, 0, 1));
+                Rx[i * SR + k + 1] = make_float2(res(x1.x, 1, 0), res(x1.y, 1, 1));
+            }
         }
         __syncthreads();
         // coef[] is free from here on: start loading the next slice's control samples
@@ -261,6 +293,98 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
         RawCoef raw;
         if (pre && tid < a.ncoef) raw = coef_load(a.ctl, tid, jn);
 
+        if constexpr (PS) {
+            // Power form of the same polynomial, Y = tau X, coefficients m_k = are/aim[k]
+            // (m_0 - 1 in are[0]):  E = (m_0 - 1) I + m_1 Y + R,  R = Y^2 Q_1,
+            // Q_J = m_{2J} I + m_{2J+1} Y,  Q_j = m_{2j} I + m_{2j+1} Y + Y^2 Q_{j+1}  (J = (M-1)/2):
+            // (M + 1) / 2 products.  Q_1 (the largest terms) is formed in FP64.  The product
+            // Y^2 Q_{j+1} enters E with weight ~ |Y|^(2j+2) / (2j+2)!, so the inner ones may drop
+            // slice pairs (levels chosen by the host); Y Y and Y^2 Q_1 are always FP32 accurate.
+            const int J = (M - 1) / 2;
+            Tile ye, q;
+            LTile<TN> acc;
+#pragma unroll
+            for (int n = 0; n < TN; ++n) {
+                const int col = mcol(col0, n);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int o = MShape<D>::off(mrow(row0, v), col);
+                    ye.re[n][v] = ((float)Xp[4 * PL + o] + (float)Xp[2 * PL + o]) + (float)Xp[o];
+                    ye.im[n][v] = ((float)Xp[5 * PL + o] + (float)Xp[3 * PL + o]) + (float)Xp[PL + o];
+                }
+            }
+            wave_cgemm_split<D>(Xp, Bp, row0, col0, acc);   // Y Y
+#pragma unroll
+            for (int n = 0; n < TN; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    q.re[n][v] = acc.re[0][n][v] + acc.re[1][n][v];
+                    q.im[n][v] = acc.im[0][n][v] + acc.im[1][n][v];
+                }
+            __syncthreads();
+            write_split<D, false>(q, Xp, row0, col0);      // A := Y^2 for the rest of the slice
+            // Q_j = m_{2j} I + m_{2j+1} Y (+ acc): FP32, or FP64 for j = 1
+            auto bracket = [&](int jj, bool with_acc) {
+                const double mr = a.are[2 * jj], mi = a.aim[2 * jj];
+                const double nr = a.are[2 * jj + 1], ni = a.aim[2 * jj + 1];
+#pragma unroll
+                for (int n = 0; n < TN; ++n)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const bool dg = mrow(row0, v) == mcol(col0, n);
+                        const float cr = with_acc ? acc.re[0][n][v] + acc.re[1][n][v] : 0.f;
+                        const float ci = with_acc ? acc.im[0][n][v] + acc.im[1][n][v] : 0.f;
+                        if (jj == 1) {
+                            const double yr = ye.re[n][v], yi = ye.im[n][v];
+                            q.re[n][v] = (float)((nr * yr - ni * yi) + (double)cr + (dg ? mr : 0.0));
+                            q.im[n][v] = (float)((nr * yi + ni * yr) + (double)ci + (dg ? mi : 0.0));
+                        } else {
+                            const float yr = ye.re[n][v], yi = ye.im[n][v];
+                            q.re[n][v] = ((float)nr * yr - (float)ni * yi) + cr + (dg ? (float)mr : 0.f);
+                            q.im[n][v] = ((float)nr * yi + (float)ni * yr) + ci + (dg ? (float)mi : 0.f);
+                        }
+                    }
+            };
+            bracket(J, false);
+            write_split<D, true>(q, Bp, row0, col0);
+            __syncthreads();
+            for (int jj = J - 1; jj >= 1; --jj) {
+                // Y^2 Q_{jj+1}, at the precision level the host chose for its weight in E
+                const int lv = (a.ps_levels >> (2 * jj)) & 3;
+                if (lv == 0) wave_cgemm_split<D, 0>(Xp, Bp, row0, col0, acc);
+                else if (lv == 1) wave_cgemm_split<D, 1>(Xp, Bp, row0, col0, acc);
+                else wave_cgemm_split<D, 2>(Xp, Bp, row0, col0, acc);
+                bracket(jj, true);
+                __syncthreads();
+                write_split<D, true>(q, Bp, row0, col0);
+                __syncthreads();
+            }
+            if (pre && tid < a.ncoef) coef[tid] = coef_finish(a.ctl, raw);
+            coef_ready = pre;
+            wave_cgemm_split<D>(Xp, Bp, row0, col0, acc);       // R = Y^2 Q_1
+            const double m0m1 = a.are[0];
+            const double2 m1 = mk<double>(a.are[1], a.aim[1]);
+#pragma unroll
+            for (int n = 0; n < TN; ++n) {
+                const int col = mcol(col0, n);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = mrow(row0, v);
+                    double er = 0.0, ei = 0.0;
+                    if (row < d && col < d) {
+                        const float2 rx = Rx[row * SR + col];
+                        const double2 y = mk<double>((double)ye.re[n][v] + (double)rx.x, (double)ye.im[n][v] + (double)rx.y);
+                        const double2 t = cmul(m1, y);
+                        er = t.x + ((double)acc.re[0][n][v] + (double)acc.re[1][n][v]) + (row == col ? m0m1 : 0.0);
+                        ei = t.y + ((double)acc.im[0][n][v] + (double)acc.im[1][n][v]);
+                    }
+                    e.re[n][v] = (float)er;
+                    e.im[n][v] = (float)ei;
+                }
+            }
+            __syncthreads();
+            return;
+        }
         Tile b1, b2;
         LTile<TN> acc;
         {
@@ -273,7 +397,7 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
                 for (int v = 0; v < 4; ++v) {
                     const int row = mrow(row0, v);
                     const bool dg = row == col;
-                    const int o = row * S + col;
+                    const int o = MShape<D>::off(row, col);
                     const float xr = ((float)Xp[4 * PL + o] + (float)Xp[2 * PL + o]) + (float)Xp[o];
                     const float xi = ((float)Xp[5 * PL + o] + (float)Xp[3 * PL + o]) + (float)Xp[PL + o];
                     b2.re[n][v] = dg ? aMr : 0.f;
@@ -344,7 +468,7 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
                     const double cr = (double)acc.re[0][n][v] + (double)acc.re[1][n][v];
                     const double ci = (double)acc.im[0][n][v] + (double)acc.im[1][n][v];
                     // X_j in FP64 from its split and residual
-                    const int o = row * S + col;
+                    const int o = MShape<D>::off(row, col);
                     const float2 rx = Rx[row * SR + col];
                     const double xr = (((double)(float)Xp[4 * PL + o] + (double)(float)Xp[2 * PL + o]) +
                                        (double)(float)Xp[o]) + (double)rx.x;
@@ -446,17 +570,19 @@ hipError_t launch_cheb_mixed(const MixedArgs& a, hipStream_t st) {
     const int F = mixed_fused_levels(a.N);
     const int grid = (a.N + (1 << F) - 1) >> F;
     auto go = [&](auto kern, int g) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * g), 0, st, a); };
-    if (a.d <= 32) {
-        if (F == 3) go(cheb_mixed_kernel<32, 3>, MShape<32>::G);
-        else if (F == 2) go(cheb_mixed_kernel<32, 2>, MShape<32>::G);
-        else if (F == 1) go(cheb_mixed_kernel<32, 1>, MShape<32>::G);
-        else go(cheb_mixed_kernel<32, 0>, MShape<32>::G);
-    } else {
-        if (F == 3) go(cheb_mixed_kernel<64, 3>, MShape<64>::G);
-        else if (F == 2) go(cheb_mixed_kernel<64, 2>, MShape<64>::G);
-        else if (F == 1) go(cheb_mixed_kernel<64, 1>, MShape<64>::G);
-        else go(cheb_mixed_kernel<64, 0>, MShape<64>::G);
+#define PA_MIXED_GO(DD, PSV)                                                        \
+    switch (F) {                                                                    \
+        case 3: go(cheb_mixed_kernel<DD, 3, PSV>, MShape<DD>::G); break;            \
+        case 2: go(cheb_mixed_kernel<DD, 2, PSV>, MShape<DD>::G); break;            \
+        case 1: go(cheb_mixed_kernel<DD, 1, PSV>, MShape<DD>::G); break;            \
+        default: go(cheb_mixed_kernel<DD, 0, PSV>, MShape<DD>::G); break;           \
     }
+    if (a.d <= 32) {
+        if (a.ps) { PA_MIXED_GO(32, true) } else { PA_MIXED_GO(32, false) }
+    } else {
+        if (a.ps) { PA_MIXED_GO(64, true) } else { PA_MIXED_GO(64, false) }
+    }
+#undef PA_MIXED_GO
     return hipGetLastError();
 }
 

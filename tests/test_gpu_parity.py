@@ -194,6 +194,19 @@ def test_mixed_fused_tree_levels(d, n):
     assert rel_err(u, ref) < 1e-5
 
 
+@pytest.mark.parametrize("d,n", [(16, 40), (64, 300), (64, 1100)])
+def test_mixed_large_step_keeps_recurrence(d, n):
+    """tau Hnorm > 1 selects the Clenshaw recurrence of the mixed kernel (small steps use the
+    power form of the same polynomial); both are checked against the FP64 restatement."""
+    h0, hks, c = case(1200 + d + n, d, 2, n)
+    hn = eo.hamiltonian_norm(h0, hks)
+    dt = 2.5 / hn
+    u, m = run_gpu("fp64", h0, hks, c, dt, "none", mixed=True)
+    assert m >= 9
+    ref = eo.equiprop(h0, hks, c, dt, mode="none", cycles=m)
+    assert rel_err(u, ref) < 1e-5
+
+
 @pytest.mark.parametrize("cycles", [3, 5, 9])
 def test_mixed_manual_cycles(cycles):
     h0, hks, c = case(900 + cycles, 16, 1, 40)
@@ -214,7 +227,8 @@ def test_mixed_long_product_dim32():
 
 def test_full_size_heisenberg6_mixed_within_gate():
     """The benchmark workload at full size (dim 64, 65,536 steps, Simpson): the mixed path is
-    within 1e-5 of the FP64 path, and both propagators are unitary to their precision."""
+    within 1e-5 of the FP64 path, and both propagators are unitary to their precision (for the
+    mixed path: what a deviation of 1e-5 of |U|_F from a unitary allows, 2 * 1e-5 * sqrt(64))."""
     from parament_amd import workloads
     h0, hk, c, dt, quad = workloads.heisenberg6()
     u64, m64 = run_gpu("fp64", h0, hk, c, dt, quad)
@@ -223,7 +237,7 @@ def test_full_size_heisenberg6_mixed_within_gate():
     assert rel_err(um, u64) < 1e-5
     eye = np.eye(64)
     assert np.linalg.norm(u64.conj().T @ u64 - eye) < 1e-9
-    assert np.linalg.norm(um.conj().T @ um - eye) < 1e-4
+    assert np.linalg.norm(um.conj().T @ um - eye) < 2 * 1e-5 * np.sqrt(64)
 
 
 # ---- dims above the on-chip kernels: batched matrix-core GEMMs ----
