@@ -889,6 +889,11 @@ template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, dou
 constexpr int kEnsSide = 8;        // side streams: 3 classes + up to 5 large-instance branches
 constexpr int kEnsClasses = 3;     // dim classes 16, 32, 64
 
+// nodes of one tree scratch buffer of an ensemble instance with N slices
+size_t ens_tree_scratch(int N) {
+    return std::max((size_t)((N + 3) / 4), (size_t)pa::tree_scratch_nodes<float>(N));
+}
+
 struct EnsInst {
     unsigned d = 0, amps = 0, pts = 0;
     double dt = 0, tau = 0, hnorm = 0;
@@ -1029,9 +1034,10 @@ Parament_ErrorCode ens_set_hamiltonians(Ensemble* e, const cuComplex* H) {
         if (x.M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
         bessel(x.M, x.tau * x.hnorm, x.are, x.aim);
         x.cls = x.d <= 16 ? 0 : (x.d <= 32 ? 1 : (x.d <= 64 ? 2 : -1));
-        // working memory: classes: leaves N + tree scratch 2 ceil(N / 4); large: X, b1, b2
+        // working memory: classes: leaves N + two tree scratch buffers (the multi-instance tree
+        // writes ceil(N / 4) nodes, launch_tree tree_scratch_nodes(N)); large: X, b1, b2
         x.work_off = e->n_work;
-        e->n_work += x.cls >= 0 ? ((size_t)x.N + 2 * (size_t)((x.N + 3) / 4)) * dd : 3 * (size_t)x.N * dd;
+        e->n_work += x.cls >= 0 ? ((size_t)x.N + 2 * ens_tree_scratch(x.N)) * dd : 3 * (size_t)x.N * dd;
         if (x.cls < 0) e->large.push_back((int)i);
     }
     if (hipMalloc(&e->d_work, e->n_work * sizeof(float2)) != hipSuccess) { hipGetLastError(); e->d_work = nullptr; return PARAMENT_STATUS_DEVICE_ALLOC_FAILED; }
@@ -1066,7 +1072,7 @@ Parament_ErrorCode ens_set_hamiltonians(Ensemble* e, const cuComplex* H) {
                 if (x.cls != c) continue;
                 const size_t dd = (size_t)x.d * x.d;
                 float2* leaves = e->d_work + x.work_off;
-                float2* scr[2] = {leaves + (size_t)x.N * dd, leaves + ((size_t)x.N + (size_t)((x.N + 3) / 4)) * dd};
+                float2* scr[2] = {leaves + (size_t)x.N * dd, leaves + ((size_t)x.N + ens_tree_scratch(x.N)) * dd};
                 int n = x.N, l = 0;
                 const float2* src = leaves;
                 do {
@@ -1192,7 +1198,7 @@ hipError_t ens_enqueue_loop(Ensemble* e) {
         ca.out = leaves;
         r = pa::launch_cheb_fused<float>(ca, e->stream);
         float2* s0 = leaves + (size_t)x.N * dd;
-        float2* s1 = s0 + (size_t)((x.N + 3) / 4) * dd;
+        float2* s1 = s0 + ens_tree_scratch(x.N) * dd;
         if (r == hipSuccess) r = pa::launch_tree<float>(leaves, false, x.N, (int)x.d, s0, s1, e->d_out + x.out_off, e->stream);
     }
     return r;

@@ -62,7 +62,56 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void tree_block_kernel(const SrcT
 }
 
 
+// Top levels of the tree, where few products remain and a work-group per block of nodes would
+// leave most CUs idle: one pairwise level per launch, each product spread over (D/16)^2
+// work-groups of four waves.  Work-group (p, t) forms the 16 x 16 output tile t of node p of the
+// next level, E = A + B + A B (A = node 2p + 1, the later, on the left; B = node 2p), its waves
+// each taking a quarter of the inner dimension straight from memory and adding their partial
+// tiles through LDS.  A last unpaired node is carried up unchanged (the odd-carry rule).
+template <class R, int D, class SrcT>
+__global__ __launch_bounds__(256) void tree_split_kernel(const SrcT* __restrict__ src, int n_in, int d,
+                                                         cplx<R>* __restrict__ dst, int final_level) {
+    constexpr int TT = D / 16, KW = D / 4;
+    __shared__ cplx<R> part[4][256];
+    const int prod = blockIdx.x / (TT * TT), t = blockIdx.x % (TT * TT);
+    const int r0 = 16 * (t / TT), c0 = 16 * (t % TT);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
+    const int dd = d * d;
+    const bool pair = 2L * prod + 1 < n_in;
+    const SrcT* A = src + (size_t)(2L * prod + (pair ? 1 : 0)) * dd;
+    const SrcT* B = src + (size_t)(2L * prod) * dd;
+    typename Mma<R>::acc cr, ci;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { cr[v] = R(0); ci[v] = R(0); }
+    if (pair) {
+#pragma unroll
+        for (int k0 = wave * KW; k0 < wave * KW + KW; k0 += 4) {
+            const int k = k0 + q, ra = r0 + r, cb = c0 + r;
+            R ar = R(0), ai = R(0), br = R(0), bi = R(0);
+            if (ra < d && k < d) { ar = R(A[ra * d + k].x); ai = R(A[ra * d + k].y); }
+            if (k < d && cb < d) { br = R(B[k * d + cb].x); bi = R(B[k * d + cb].y); }
+            cr = Mma<R>::op(ar, br, cr);
+            ci = Mma<R>::op(ar, bi, ci);
+            cr = Mma<R>::op(-ai, bi, cr);
+            ci = Mma<R>::op(ai, br, ci);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part[wave][(acc_row<R>(0, 0, v)) * 16 + r] = mk<R>(cr[v], ci[v]);
+    __syncthreads();
+    const int i = r0 + (tid >> 4), j = c0 + (tid & 15);
+    if (i < d && j < d) {
+        cplx<R> e = cadd(cadd(part[0][tid], part[1][tid]), cadd(part[2][tid], part[3][tid]));
+        if (pair) e = cadd(e, mk<R>(R(A[i * d + j].x), R(A[i * d + j].y)));
+        e = cadd(e, mk<R>(R(B[i * d + j].x), R(B[i * d + j].y)));
+        if (final_level && i == j) e.x += R(1);
+        dst[(size_t)prod * dd + i * d + j] = e;
+    }
+}
+
 template <class R> constexpr int tree_logb() { return sizeof(R) == 4 ? 3 : 2; }
+// levels with at most this many nodes go through tree_split_kernel
+constexpr int kTreeSplitNodes = 1024;
 
 template <class R, int D, class SrcT>
 static void launch_tree_level(const SrcT* src, int n, int d, cplx<R>* dst, bool final_level, hipStream_t st) {
@@ -72,10 +121,18 @@ static void launch_tree_level(const SrcT* src, int n, int d, cplx<R>* dst, bool 
                        dst, final_level ? 1 : 0);
 }
 
+template <class R, int D, class SrcT>
+static void launch_tree_split(const SrcT* src, int n, int d, cplx<R>* dst, bool final_level, hipStream_t st) {
+    const int grid = ((n + 1) / 2) * (D / 16) * (D / 16);
+    hipLaunchKernelGGL((tree_split_kernel<R, D, SrcT>), dim3(grid), dim3(256), 0, st, src, n, d, dst,
+                       final_level ? 1 : 0);
+}
+
+// nodes of the widest level a scratch buffer receives
 template <class R>
 int tree_scratch_nodes(int N) {
     constexpr int B = 1 << tree_logb<R>();
-    return (N + B - 1) / B;
+    return N <= kTreeSplitNodes ? (N + 1) / 2 : (N + B - 1) / B;
 }
 template int tree_scratch_nodes<float>(int);
 template int tree_scratch_nodes<double>(int);
@@ -85,28 +142,32 @@ hipError_t launch_tree(const void* leaves, bool leaves_are_c64, int N, int d, cp
                        cplx<R>* root, hipStream_t st) {
     if (d <= 0 || d > 64 || N <= 0) return hipErrorInvalidValue;
     constexpr int B = 1 << tree_logb<R>();
-    auto level = [&](const auto* src, int n, cplx<R>* dst, bool fin) {
-        if (d <= 16) launch_tree_level<R, 16>(src, n, d, dst, fin, st);
-        else if (d <= 32) launch_tree_level<R, 32>(src, n, d, dst, fin, st);
-        else launch_tree_level<R, 64>(src, n, d, dst, fin, st);
+    // a level of n nodes: B-fold blocks on one work-group each, or one pairwise level split over
+    // work-groups once few nodes remain; returns the node count of the level it writes
+    auto level = [&](const auto* src, int n, cplx<R>* root_or_dst, cplx<R>* dst) -> int {
+        const bool split = n <= kTreeSplitNodes;
+        const int n_out = split ? (n + 1) / 2 : (n + B - 1) / B;
+        const bool fin = n_out == 1;
+        cplx<R>* out = fin ? root_or_dst : dst;
+        if (split) {
+            if (d <= 16) launch_tree_split<R, 16>(src, n, d, out, fin, st);
+            else if (d <= 32) launch_tree_split<R, 32>(src, n, d, out, fin, st);
+            else launch_tree_split<R, 64>(src, n, d, out, fin, st);
+        } else {
+            if (d <= 16) launch_tree_level<R, 16>(src, n, d, out, fin, st);
+            else if (d <= 32) launch_tree_level<R, 32>(src, n, d, out, fin, st);
+            else launch_tree_level<R, 64>(src, n, d, out, fin, st);
+        }
+        return n_out;
     };
-    int n = N;
     cplx<R>* bufs[2] = {scratch0, scratch1};
     int which = 0;
     // first level reads the leaves
-    {
-        const int n_out = (n + B - 1) / B;
-        cplx<R>* dst = (n_out == 1) ? root : bufs[which];
-        if (leaves_are_c64) level(reinterpret_cast<const float2*>(leaves), n, dst, n_out == 1);
-        else level(reinterpret_cast<const cplx<R>*>(leaves), n, dst, n_out == 1);
-        n = n_out;
-    }
+    int n = leaves_are_c64 ? level(reinterpret_cast<const float2*>(leaves), N, root, bufs[which])
+                           : level(reinterpret_cast<const cplx<R>*>(leaves), N, root, bufs[which]);
     while (n > 1) {
-        const int n_out = (n + B - 1) / B;
-        cplx<R>* dst = (n_out == 1) ? root : bufs[which ^ 1];
-        level(static_cast<const cplx<R>*>(bufs[which]), n, dst, n_out == 1);
+        n = level(static_cast<const cplx<R>*>(bufs[which]), n, root, bufs[which ^ 1]);
         which ^= 1;
-        n = n_out;
     }
     return hipGetLastError();
 }

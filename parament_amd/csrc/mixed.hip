@@ -111,40 +111,54 @@ __device__ __forceinline__ void wave_cgemm_split(const __bf16* __restrict__ Ap, 
 #pragma unroll 1
     for (int ks = 0; ks < D / 32; ++ks) {
         const int ka = MShape<D>::off(row0 + r, 32 * ks + 8 * kq);
-        bf16x8 Br[TN][LV + 1], Bi[TN][LV + 1];
+        // B slice q (both column blocks) against the A slices p <= LV - q
 #pragma unroll
-        for (int n = 0; n < TN; ++n) {
-            const int kb = MShape<D>::off(col0 + 16 * n + r, 32 * ks + 8 * kq);
+        for (int q = 0; q <= LV; ++q) {
+            bf16x8 Br[TN], Bi[TN];
 #pragma unroll
-            for (int q = 0; q <= LV; ++q) {
-                Br[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q) * PL + kb);
-                Bi[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q + 1) * PL + kb);
+            for (int n = 0; n < TN; ++n) {
+                const int kb = MShape<D>::off(col0 + 16 * n + r, 32 * ks + 8 * kq);
+                Br[n] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q) * PL + kb);
+                Bi[n] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q + 1) * PL + kb);
             }
-        }
-        // A slice p against the B slices q <= LV - p (slice pairs with p + q <= LV)
 #pragma unroll
-        for (int p = 0; p <= LV; ++p) {
-            const bf16x8 Ar = *reinterpret_cast<const bf16x8*>(Ap + (2 * p) * PL + ka);
-            const bf16x8 Ai = *reinterpret_cast<const bf16x8*>(Ap + (2 * p + 1) * PL + ka);
-            const bf16x8 An = negate(Ai);
+            for (int p = 0; p + q <= LV; ++p) {
+                const bf16x8 Ar = *reinterpret_cast<const bf16x8*>(Ap + (2 * p) * PL + ka);
+                const bf16x8 Ai = *reinterpret_cast<const bf16x8*>(Ap + (2 * p + 1) * PL + ka);
+                const bf16x8 An = negate(Ai);
+                const int L = (p + q) > 0 ? 1 : 0;
 #pragma unroll
-            for (int n = 0; n < TN; ++n)
-#pragma unroll
-                for (int q = 0; q + p <= LV; ++q) {
-                    const int L = (p + q) > 0 ? 1 : 0;
-                    c.re[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Br[n][q], c.re[L][n], 0, 0, 0);
-                    c.im[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Bi[n][q], c.im[L][n], 0, 0, 0);
-                    c.re[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(An, Bi[n][q], c.re[L][n], 0, 0, 0);
-                    c.im[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, Br[n][q], c.im[L][n], 0, 0, 0);
+                for (int n = 0; n < TN; ++n) {
+                    c.re[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Br[n], c.re[L][n], 0, 0, 0);
+                    c.im[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Bi[n], c.im[L][n], 0, 0, 0);
+                    c.re[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(An, Bi[n], c.re[L][n], 0, 0, 0);
+                    c.im[L][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, Br[n], c.im[L][n], 0, 0, 0);
                 }
+            }
         }
     }
 }
 
-// Tile (accumulator layout) -> split planes: transposed (B operand, Bt[col][row]) or
-// row-major (A operand, A[row][col]).  Each lane holds rows 4(l/16) .. +3 of one column per
-// block.
-template <int D, bool TRANSPOSED>
+// Leading NL bf16 slices of two FP32 values (split3 for NL = 3).
+template <int NL>
+__device__ __forceinline__ void splitn(float x0, float x1, bf16x2 (&s)[3]) {
+    if constexpr (NL == 3) {
+        split3(x0, x1, s[0], s[1], s[2]);
+    } else {
+        s[0][0] = (__bf16)x0; s[0][1] = (__bf16)x1;
+        if constexpr (NL == 2) {
+            s[1][0] = (__bf16)(x0 - (float)s[0][0]);
+            s[1][1] = (__bf16)(x1 - (float)s[0][1]);
+        }
+    }
+}
+
+// Tile (accumulator layout) -> split planes.  TRANSPOSED: the natural write of the layout,
+// P[col][row] = b(row, col), four consecutive rows per lane in one 8-byte store (this plane is
+// the B operand for the product (.) b, or the A operand for b^T (.)).  Otherwise row-major
+// P[row][col] = b(row, col), 2-byte stores.  Only the leading NL slices are written: a plane
+// read only by products of split level NL - 1 needs no more.
+template <int D, bool TRANSPOSED, int NL = 3>
 __device__ __forceinline__ void write_split(const MTile<MShape<D>::TN>& b, __bf16* __restrict__ P, int row0, int col0) {
     constexpr int PL = MShape<D>::PL, TN = MShape<D>::TN;
     const int rbase = mrow(row0, 0);
@@ -154,25 +168,18 @@ __device__ __forceinline__ void write_split(const MTile<MShape<D>::TN>& b, __bf1
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const f32x4v& x = c ? b.im[n] : b.re[n];
-            bf16x2 h0, m0, l0, h1, m1, l1;
-            split3(x[0], x[1], h0, m0, l0);
-            split3(x[2], x[3], h1, m1, l1);
-            if (TRANSPOSED) {
-                const bf16x4 h = {h0[0], h0[1], h1[0], h1[1]}, m = {m0[0], m0[1], m1[0], m1[1]},
-                             l = {l0[0], l0[1], l1[0], l1[1]};
-                *reinterpret_cast<bf16x4*>(P + (0 + c) * PL + MShape<D>::off(col, rbase)) = h;
-                *reinterpret_cast<bf16x4*>(P + (2 + c) * PL + MShape<D>::off(col, rbase)) = m;
-                *reinterpret_cast<bf16x4*>(P + (4 + c) * PL + MShape<D>::off(col, rbase)) = l;
-            } else {
+            bf16x2 lo[3], hi[3];
+            splitn<NL>(x[0], x[1], lo);
+            splitn<NL>(x[2], x[3], hi);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int o = MShape<D>::off(rbase + q, col);
-                    const bf16x2& hh = (q < 2) ? h0 : h1;
-                    const bf16x2& mm = (q < 2) ? m0 : m1;
-                    const bf16x2& ll = (q < 2) ? l0 : l1;
-                    P[(0 + c) * PL + o] = hh[q & 1];
-                    P[(2 + c) * PL + o] = mm[q & 1];
-                    P[(4 + c) * PL + o] = ll[q & 1];
+            for (int t = 0; t < NL; ++t) {
+                if (TRANSPOSED) {
+                    const bf16x4 v = {lo[t][0], lo[t][1], hi[t][0], hi[t][1]};
+                    *reinterpret_cast<bf16x4*>(P + (2 * t + c) * PL + MShape<D>::off(col, rbase)) = v;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        P[(2 * t + c) * PL + MShape<D>::off(rbase + q, col)] = (q < 2 ? lo[t] : hi[t])[q & 1];
                 }
             }
         }
@@ -211,7 +218,8 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
     typedef MTile<TN> Tile;
     __shared__ __attribute__((aligned(16))) __bf16 Xp[6 * PL];
     __shared__ __attribute__((aligned(16))) __bf16 Bp[6 * PL];
-    __shared__ float2 Rx[D * (D + 2)];   // X_j - (s0 + s1 + s2), the FP64 residual of the split
+    // X_j - (s0 + s1 + s2), the FP64 residual of the split; also the FP32 stage of the tree products
+    __shared__ __attribute__((aligned(16))) float2 Rx[D * (D + 2)];
     __shared__ double2 coef[kMaxCoeffs];
     constexpr int SR = D + 2;
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -300,6 +308,11 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
             // (M + 1) / 2 products.  Q_1 (the largest terms) is formed in FP64.  The product
             // Y^2 Q_{j+1} enters E with weight ~ |Y|^(2j+2) / (2j+2)!, so the inner ones may drop
             // slice pairs (levels chosen by the host); Y Y and Y^2 Q_1 are always FP32 accurate.
+            //
+            // The matrix cores form P_A P_B^T of two row-major planes, and the natural (8-byte)
+            // write of a result tile C is the plane C^T.  With Xp = Y and Bp = Z = Y^T, the
+            // product Z Z (Bp, Xp) therefore writes naturally as the plane Y^2, and each
+            // Y^2 Q(Y) takes the natural plane of Q(Y): every plane write is a natural one.
             const int J = (M - 1) / 2;
             Tile ye, q;
             LTile<TN> acc;
@@ -313,7 +326,7 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
                     ye.im[n][v] = ((float)Xp[5 * PL + o] + (float)Xp[3 * PL + o]) + (float)Xp[PL + o];
                 }
             }
-            wave_cgemm_split<D>(Xp, Bp, row0, col0, acc);   // Y Y
+            wave_cgemm_split<D>(Bp, Xp, row0, col0, acc);   // Z Z = (Y^2)^T
 #pragma unroll
             for (int n = 0; n < TN; ++n)
 #pragma unroll
@@ -322,7 +335,7 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
                     q.im[n][v] = acc.im[0][n][v] + acc.im[1][n][v];
                 }
             __syncthreads();
-            write_split<D, false>(q, Xp, row0, col0);      // A := Y^2 for the rest of the slice
+            write_split<D, true>(q, Xp, row0, col0);   // A := Y^2 for the rest of the slice
             // Q_j = m_{2j} I + m_{2j+1} Y (+ acc): FP32, or FP64 for j = 1
             auto bracket = [&](int jj, bool with_acc) {
                 const double mr = a.are[2 * jj], mi = a.aim[2 * jj];
@@ -345,18 +358,27 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
                         }
                     }
             };
+            // level of the product Y^2 Q_{jj+1} (jj >= 1); Y^2 Q_1 (jj = 0) is full precision
+            auto level = [&](int jj) { return jj >= 1 ? (a.ps_levels >> (2 * jj)) & 3 : 2; };
+            // Q_jj as the B plane, with the slices that the product reading it (level(jj - 1)) uses
+            auto write_q = [&](int jj) {
+                const int nl = level(jj - 1) + 1;
+                if (nl == 1) write_split<D, true, 1>(q, Bp, row0, col0);
+                else if (nl == 2) write_split<D, true, 2>(q, Bp, row0, col0);
+                else write_split<D, true, 3>(q, Bp, row0, col0);
+            };
             bracket(J, false);
-            write_split<D, true>(q, Bp, row0, col0);
+            write_q(J);
             __syncthreads();
             for (int jj = J - 1; jj >= 1; --jj) {
                 // Y^2 Q_{jj+1}, at the precision level the host chose for its weight in E
-                const int lv = (a.ps_levels >> (2 * jj)) & 3;
+                const int lv = level(jj);
                 if (lv == 0) wave_cgemm_split<D, 0>(Xp, Bp, row0, col0, acc);
                 else if (lv == 1) wave_cgemm_split<D, 1>(Xp, Bp, row0, col0, acc);
                 else wave_cgemm_split<D, 2>(Xp, Bp, row0, col0, acc);
                 bracket(jj, true);
                 __syncthreads();
-                write_split<D, true>(q, Bp, row0, col0);
+                write_q(jj);
                 __syncthreads();
             }
             if (pre && tid < a.ncoef) coef[tid] = coef_finish(a.ctl, raw);
