@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void tree_split_kernel(const SrcT* __restrict_
     }
 }
 
-template <class R> constexpr int tree_logb() { return sizeof(R) == 4 ? 3 : 2; }
+template <class R> constexpr int tree_logb() { return 3; }
 // levels with at most this many nodes go through tree_split_kernel
 constexpr int kTreeSplitNodes = 1024;
 

@@ -239,54 +239,104 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
         if (!coef_ready)
             for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
         __syncthreads();
-        // X_j = H0 + sum_t c'_t O_t in FP64: each thread issues the operator loads of an element
-        // pair together before the arithmetic, so their latencies overlap.
+        if constexpr (PS) {
+            // Y_j = tau (H0 + sum_t c'_t O_t) in FP64, as 2 x 2 blocks: wave region rg (16 x 16
+            // elements), lane (bi, bk) = (l / 8, l % 8) forms rows i, i + 1 and columns k, k + 1, so
+            // that both Y (Xp) and Z = Y^T (Bp) take two 4-byte stores per row pair, and the residual
+            // of the FP32 sum ((s2 + s1) + s0) that the power form keeps in registers goes to Rx.
+            constexpr int RT = D / 16;
 #pragma unroll 1
-        for (int it = 0; it < D * D / 2 / (64 * G); ++it) {
-            const int e2 = tid + it * 64 * G;
-            const int i = (2 * e2) / D, k = (2 * e2) % D;   // elements (i, k), (i, k + 1)
-            const bool v0 = i < d && k < d, v1 = i < d && k + 1 < d;
-            double2 x0 = mk<double>(0, 0), x1 = mk<double>(0, 0);
-            if (v0) x0 = a.ops[i * d + k];
-            if (v1) x1 = a.ops[i * d + k + 1];
-            for (int t0 = 0; t0 < a.ncoef; t0 += 4) {
-                double2 o0[4], o1[4];
+            for (int it = 0; it < RT * RT / G; ++it) {
+                const int rg = wave + G * it, lane = tid & 63;
+                const int i = 16 * (rg / RT) + 2 * (lane >> 3), k = 16 * (rg % RT) + 2 * (lane & 7);
+                bool v[4];
+                double2 x[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const size_t base = (size_t)(t0 + q + 1) * dd + i * d + k;
-                    o0[q] = (v0 && t0 + q < a.ncoef) ? a.ops[base] : mk<double>(0, 0);
-                    o1[q] = (v1 && t0 + q < a.ncoef) ? a.ops[base + 1] : mk<double>(0, 0);
+                for (int e = 0; e < 4; ++e) {
+                    const int ii = i + (e >> 1), kk = k + (e & 1);
+                    v[e] = ii < d && kk < d;
+                    x[e] = v[e] ? a.ops[ii * d + kk] : mk<double>(0, 0);
                 }
+                for (int t0 = 0; t0 < a.ncoef; t0 += 2) {
+                    double2 o[2][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (t0 + q < a.ncoef) {
-                        x0 = cadd(x0, cmul(coef[t0 + q], o0[q]));
-                        x1 = cadd(x1, cmul(coef[t0 + q], o1[q]));
-                    }
-            }
-            if constexpr (PS) {   // the power form works on Y = tau X
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const size_t at = (size_t)(t0 + q + 1) * dd + (i + (e >> 1)) * d + k + (e & 1);
+                            o[q][e] = (v[e] && t0 + q < a.ncoef) ? a.ops[at] : mk<double>(0, 0);
+                        }
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        if (t0 + q < a.ncoef)
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) x[e] = cadd(x[e], cmul(coef[t0 + q], o[q][e]));
+                }
                 const double tau = a.ctl.tau;
-                x0.x *= tau; x0.y *= tau; x1.x *= tau; x1.y *= tau;
-            }
-            bf16x2 p[6];
-            split3((float)x0.x, (float)x1.x, p[0], p[2], p[4]);
-            split3((float)x0.y, (float)x1.y, p[1], p[3], p[5]);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) *reinterpret_cast<bf16x2*>(Xp + q * PL + MShape<D>::off(i, k)) = p[q];
-            if constexpr (PS) {
-                // Y also as the transposed B operand (for Y Y), and the residual of the FP32 sum
-                // ((s2 + s1) + s0) that the power form keeps in registers
+                for (int e = 0; e < 4; ++e) { x[e].x *= tau; x[e].y *= tau; }
+                // slices of the row pairs: r[c][row][limb] (c: re / im), each bf16x2 = columns k, k + 1
+                bf16x2 r[2][2][3];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    Bp[q * PL + MShape<D>::off(k, i)] = p[q][0];
-                    Bp[q * PL + MShape<D>::off(k + 1, i)] = p[q][1];
-                }
-                auto res = [&](double x, int h, int c) -> float {
-                    return (float)(x - (double)(((float)p[4 + c][h] + (float)p[2 + c][h]) + (float)p[c][h]));
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const double2 &xa = x[2 * h], &xb = x[2 * h + 1];
+                        split3((float)(c ? xa.y : xa.x), (float)(c ? xb.y : xb.x), r[c][h][0], r[c][h][1], r[c][h][2]);
+                    }
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        __bf16* P = Xp + (2 * t + c) * PL;
+                        *reinterpret_cast<bf16x2*>(P + MShape<D>::off(i, k)) = r[c][0][t];
+                        *reinterpret_cast<bf16x2*>(P + MShape<D>::off(i + 1, k)) = r[c][1][t];
+                        __bf16* Q = Bp + (2 * t + c) * PL;
+                        const bf16x2 z0 = {r[c][0][t][0], r[c][1][t][0]}, z1 = {r[c][0][t][1], r[c][1][t][1]};
+                        *reinterpret_cast<bf16x2*>(Q + MShape<D>::off(k, i)) = z0;
+                        *reinterpret_cast<bf16x2*>(Q + MShape<D>::off(k + 1, i)) = z1;
+                    }
+                auto res = [&](double xv, int c, int h, int col) -> float {
+                    return (float)(xv - (double)(((float)r[c][h][2][col] + (float)r[c][h][1][col]) + (float)r[c][h][0][col]));
                 };
-                Rx[i * SR + k] = make_float2(res(x0.x, 0, 0), res(x0.y, 0, 1));
-                Rx[i * SR + k + 1] = make_float2(res(x1.x, 1, 0), res(x1.y, 1, 1));
-            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const double2 &xa = x[2 * h], &xb = x[2 * h + 1];
+                    *reinterpret_cast<float4*>(Rx + (i + h) * SR + k) =
+                        make_float4(res(xa.x, 0, h, 0), res(xa.y, 1, h, 0), res(xb.x, 0, h, 1), res(xb.y, 1, h, 1));
+                }
+            }
+        } else {
+            // X_j = H0 + sum_t c'_t O_t in FP64: each thread issues the operator loads of an element
+            // pair together before the arithmetic, so their latencies overlap.
+#pragma unroll 1
+            for (int it = 0; it < D * D / 2 / (64 * G); ++it) {
+                const int e2 = tid + it * 64 * G;
+                const int i = (2 * e2) / D, k = (2 * e2) % D;   // elements (i, k), (i, k + 1)
+                const bool v0 = i < d && k < d, v1 = i < d && k + 1 < d;
+                double2 x0 = mk<double>(0, 0), x1 = mk<double>(0, 0);
+                if (v0) x0 = a.ops[i * d + k];
+                if (v1) x1 = a.ops[i * d + k + 1];
+                for (int t0 = 0; t0 < a.ncoef; t0 += 4) {
+                    double2 o0[4], o1[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const size_t base = (size_t)(t0 + q + 1) * dd + i * d + k;
+                        o0[q] = (v0 && t0 + q < a.ncoef) ? a.ops[base] : mk<double>(0, 0);
+                        o1[q] = (v1 && t0 + q < a.ncoef) ? a.ops[base + 1] : mk<double>(0, 0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (t0 + q < a.ncoef) {
+                            x0 = cadd(x0, cmul(coef[t0 + q], o0[q]));
+                            x1 = cadd(x1, cmul(coef[t0 + q], o1[q]));
+                        }
+                }
+                bf16x2 p[6];
+                split3((float)x0.x, (float)x1.x, p[0], p[2], p[4]);
+                split3((float)x0.y, (float)x1.y, p[1], p[3], p[5]);
+#pragma unroll
+                for (int q = 0; q < 6; ++q) *reinterpret_cast<bf16x2*>(Xp + q * PL + MShape<D>::off(i, k)) = p[q];
                 // residuals for the FP64 first-order term (the sum of the three slices is exact in FP64)
                 auto res = [&](double x, int h, int c) -> float {
                     return (float)(x - (((double)(float)p[4 + c][h] + (double)(float)p[2 + c][h]) + (double)(float)p[c][h]));
