@@ -36,8 +36,8 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "mixed": 2500.0 / 6.0}  # mixed: bf1
 
 # HBM bytes per launch of the slice kernel on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes over `bench.py --only mixed`, each scaled by the counter's reading for a
-# 1 GiB device copy in the same run (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 249 MB + 242 MB.
-MEASURED_TRAFFIC_BYTES = {"mixed": 4.91e8}
+# 1 GiB device copy in the same run (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 18.5 MB + 306.7 MB.
+MEASURED_TRAFFIC_BYTES = {"mixed": 3.25e8}
 
 
 def fused_levels(path, n):
@@ -47,12 +47,27 @@ def fused_levels(path, n):
     return 3 if n >= 4096 else 2 if n >= 1024 else 1 if n >= 256 else 0
 
 
-def slice_flops(d, m, ncoef, n, levels=0):
-    """Algorithmic flops of the slice kernel: M-1 complex d^3 GEMMs per slice (the reference's
-    first Clenshaw GEMM multiplies by a_M I, a scaling), the assembly, and one d^3 product per
-    node of the tree levels folded into the kernel; 8 flops per complex MAC."""
+def hamiltonian_norm(h0, hk):
+    """The library's Hnorm: the sum of the operators' one-norms (largest absolute row sum)."""
+    return sum(float(np.abs(np.asarray(m)).sum(axis=1).max()) for m in (h0, *hk))
+
+
+def products_per_slice(path, m, tau_hnorm):
+    """Complex d^3 products per slice of the path's slice evaluation: the Clenshaw recurrence
+    takes M - 1 (the reference's first step multiplies by a_M I, a scaling); the mixed path
+    evaluates the same degree-M polynomial in powers of tau X when tau Hnorm <= 1, with
+    (M + 1) / 2 products."""
+    if path == "mixed" and tau_hnorm <= 1.0:
+        return (m + 1) // 2
+    return m - 1
+
+
+def slice_flops(d, products, ncoef, n, levels=0):
+    """Algorithmic flops of the slice kernel: ``products`` complex d^3 GEMMs per slice, the
+    assembly, and one d^3 product per node of the tree levels folded into the kernel; 8 flops
+    per complex MAC."""
     nodes = sum((n + (1 << l) - 1) >> l for l in range(1, levels + 1))
-    return n * (8.0 * d ** 3 * (m - 1) + 8.0 * d * d * ncoef) + nodes * 8.0 * d ** 3
+    return n * (8.0 * d ** 3 * products + 8.0 * d * d * ncoef) + nodes * 8.0 * d ** 3
 
 
 def run_path(name, prec, mixed, h0, hk, c, dt, quad, steps, warmup):
@@ -267,6 +282,7 @@ def main():
     h0, hk, c, dt, quad = workloads.heisenberg6()
     d, amps, pts = h0.shape[0], len(hk), c.shape[1]
     n_slices = (pts - 1) // 2
+    tau_hnorm = 2 * dt * hamiltonian_norm(h0, hk)   # Simpson: one slice spans two steps
     if args.only != "all":
         part = {}
         if args.only in ("fp64", "mixed", "fp32"):
@@ -288,7 +304,8 @@ def main():
     paths = {}
     for r in runs:
         err = float(np.linalg.norm(r["u"] - u64) / np.linalg.norm(u64))
-        fl = slice_flops(d, r["M"], amps, n_slices, fused_levels(r["name"], n_slices))
+        fl = slice_flops(d, products_per_slice(r["name"], r["M"], tau_hnorm), amps, n_slices,
+                         fused_levels(r["name"], n_slices))
         ach = fl / (r["slice_ms"] * 1e-3) / 1e12
         paths[r["name"]] = {"steps_per_s": pts / r["s_per_call"], "ms_per_call": r["s_per_call"] * 1e3,
                             "rel_err_vs_fp64": err, "M": r["M"], "slice_kernel_ms": r["slice_ms"],
@@ -297,7 +314,7 @@ def main():
     ok = [k for k in ("mixed", "fp32", "fp64") if paths[k]["rel_err_vs_fp64"] <= 1e-5]
     best = max(ok, key=lambda k: paths[k]["steps_per_s"])
     p = paths[best]
-    fl = slice_flops(d, p["M"], amps, n_slices, fused_levels(best, n_slices))
+    fl = slice_flops(d, products_per_slice(best, p["M"], tau_hnorm), amps, n_slices, fused_levels(best, n_slices))
     line = {
         "metric": "steps/s at dim 64 @<=1e-5 vs FP64 (mixed vs plain FP64); GRAPE grads/s",
         "value": p["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
