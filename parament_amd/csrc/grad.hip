@@ -78,7 +78,7 @@ __device__ __forceinline__ void tile_to_lds(const CTile<R, GL<R, D>::T, GL<R, D>
             }
 }
 
-// One Clenshaw-derivative step's products in a single pass over K, sharing the operand loads:
+// One derivative step's products in a single pass over K, sharing the operand loads:
 //     cb = X b        (skipped when with_b is false),
 //     cd = W b + X db
 // X, W are A operands (stride SA), b, db are B operands (stride SB); 4M complex products.
@@ -197,27 +197,17 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void down_level_kernel(const cplx
     tile_store<R, D>(w, d, dst);
 }
 
-// Root costate: out = t^H.
-template <class R>
-__global__ void conj_transpose_kernel(const cplx<R>* __restrict__ t, int d, cplx<R>* __restrict__ out) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= d * d) return;
-    const int i = e / d, k = e % d;
-    const cplx<R> v = t[k * d + i];
-    out[e] = mk<R>(v.x, -v.y);
-}
-
 // Per slice j = j0 + jl (jl < a.N; j0 even): X_j, the costate W_j from the level-1 costate of
 // its parent and the E = U - I of its sibling (E holds this launch's leaves, W1 their
 // parents), then Q_j = Dp(X_j)[W_j] and dg[t * dg_stride + j] = Tr(Q_j O_t).
-template <class R, int D>
+template <class R, int D, bool PS>
 __global__ __launch_bounds__(64 * Shape<D>::G) void grad_slice_kernel(ChebArgs<R> a, const cplx<R>* __restrict__ E,
                                                                          const cplx<R>* __restrict__ W1,
                                                                          cplx<R>* __restrict__ dg, int dg_stride) {
     constexpr int G = GL<R, D>::G, T = GL<R, D>::T, SA = GL<R, D>::SA, SB = GL<R, D>::SB;
     __shared__ R Xr[D * SA], Xi[D * SA], Wr[D * SA], Wi[D * SA], Br[D * SB], Bi[D * SB], Dr[D * SB], Di[D * SB];
     __shared__ cplx<R> coef[kGradCoeffs];
-    __shared__ cplx<R> red[G];
+    __shared__ cplx<R> red[kGradCoeffs * G];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int row0 = Shape<D>::row0(wave), col0 = Shape<D>::col0(wave);
     const int d = a.d, dd = d * d, M = a.M;
@@ -265,8 +255,95 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void grad_slice_kernel(ChebArgs<R
         }
         __syncthreads();
 
-        // b_M = a_M I, db_M = 0; b_{M-1} = s a_M X + a_{M-1} I, db_{M-1} = s a_M W
         Tile b1, b2, db1, db2;
+        if constexpr (PS) {
+            // Forward-mode derivative of the power form P(Y) = sum_k m_k Y^k, Y = tau X, by Horner's
+            // rule in S = Y^2 with direction V = tau W:  dS = Y V + V Y,
+            //   Q_J = m_{2J} I + m_{2J+1} Y,  dQ_J = m_{2J+1} V,
+            //   Q_j = m_{2j} I + m_{2j+1} Y + S Q_{j+1},  dQ_j = m_{2j+1} V + dS Q_{j+1} + S dQ_{j+1},
+            //   Dp(X)[W] = m_1 V + dS Q_1 + S dQ_1.
+            // Each step's three products are one pass of wave_step_products (A planes S, dS; B planes
+            // Q, dQ); X X and W X + X W come from the same pass on (X, W; X, W).
+            const R tau = a.ctl.tau, tau2 = tau * tau;
+            const int J = (M - 1) / 2;
+            Tile xt, wt;   // Y / tau and V / tau at the accumulator positions
+#pragma unroll
+            for (int m = 0; m < T; ++m)
+#pragma unroll
+                for (int n = 0; n < T; ++n)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                        xt.re[m][n][v] = Xr[row * SA + col]; xt.im[m][n][v] = Xi[row * SA + col];
+                        wt.re[m][n][v] = Wr[row * SA + col]; wt.im[m][n][v] = Wi[row * SA + col];
+                    }
+            for (int e = tid; e < D * D; e += 64 * G) {
+                const int i = e / D, k = e % D;
+                Br[i * SB + k] = Xr[i * SA + k]; Bi[i * SB + k] = Xi[i * SA + k];
+                Dr[i * SB + k] = Wr[i * SA + k]; Di[i * SB + k] = Wi[i * SA + k];
+            }
+            __syncthreads();
+            Tile cb, cd;
+            wave_step_products<R, T, D, SA, SB>(Xr, Xi, Wr, Wi, Br, Bi, Dr, Di, row0, col0, true, cb, cd);
+            // Q (b1) and dQ (db1) of the current Horner level from m_{2j}, m_{2j+1} and the products
+            auto horner = [&](int jj, bool with_products) {
+                const cplx<R> mA = mk<R>(a.pre[2 * jj], a.pim[2 * jj]);
+                const cplx<R> mB = mk<R>(tau * a.pre[2 * jj + 1], tau * a.pim[2 * jj + 1]);
+#pragma unroll
+                for (int m = 0; m < T; ++m)
+#pragma unroll
+                    for (int n = 0; n < T; ++n)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const bool dg_ = acc_row<R>(row0, m, v) == acc_col(col0, n);
+                            const cplx<R> y = cmul(mB, mk<R>(xt.re[m][n][v], xt.im[m][n][v]));
+                            const cplx<R> w = cmul(mB, mk<R>(wt.re[m][n][v], wt.im[m][n][v]));
+                            b1.re[m][n][v] = y.x + (dg_ ? mA.x : R(0)) + (with_products ? cb.re[m][n][v] : R(0));
+                            b1.im[m][n][v] = y.y + (dg_ ? mA.y : R(0)) + (with_products ? cb.im[m][n][v] : R(0));
+                            db1.re[m][n][v] = w.x + (with_products ? cd.re[m][n][v] : R(0));
+                            db1.im[m][n][v] = w.y + (with_products ? cd.im[m][n][v] : R(0));
+                        }
+            };
+#pragma unroll
+            for (int m = 0; m < T; ++m)
+#pragma unroll
+                for (int n = 0; n < T; ++n)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        b2.re[m][n][v] = tau2 * cb.re[m][n][v]; b2.im[m][n][v] = tau2 * cb.im[m][n][v];     // S
+                        db2.re[m][n][v] = tau2 * cd.re[m][n][v]; db2.im[m][n][v] = tau2 * cd.im[m][n][v];   // dS
+                    }
+            horner(J, false);
+            __syncthreads();
+            tile_to_lds<R, D>(b2, Xr, Xi, SA);
+            tile_to_lds<R, D>(db2, Wr, Wi, SA);
+            tile_to_lds<R, D>(b1, Br, Bi, SB);
+            tile_to_lds<R, D>(db1, Dr, Di, SB);
+            __syncthreads();
+            for (int jj = J - 1; jj >= 1; --jj) {
+                wave_step_products<R, T, D, SA, SB>(Xr, Xi, Wr, Wi, Br, Bi, Dr, Di, row0, col0, true, cb, cd);
+                horner(jj, true);
+                __syncthreads();
+                tile_to_lds<R, D>(b1, Br, Bi, SB);
+                tile_to_lds<R, D>(db1, Dr, Di, SB);
+                __syncthreads();
+            }
+            wave_step_products<R, T, D, SA, SB>(Xr, Xi, Wr, Wi, Br, Bi, Dr, Di, row0, col0, false, cb, cd);
+            {
+                const cplx<R> m1 = mk<R>(tau * a.pre[1], tau * a.pim[1]);
+#pragma unroll
+                for (int m = 0; m < T; ++m)
+#pragma unroll
+                    for (int n = 0; n < T; ++n)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const cplx<R> w = cmul(m1, mk<R>(wt.re[m][n][v], wt.im[m][n][v]));
+                            db1.re[m][n][v] = w.x + cd.re[m][n][v];
+                            db1.im[m][n][v] = w.y + cd.im[m][n][v];
+                        }
+            }
+        } else {
+        // b_M = a_M I, db_M = 0; b_{M-1} = s a_M X + a_{M-1} I, db_{M-1} = s a_M W
         {
             const cplx<R> saM = mk<R>(s * a.are[M], s * a.aim[M]);
 #pragma unroll
@@ -320,7 +397,9 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void grad_slice_kernel(ChebArgs<R
             step(b2, db2, k);       // b2 <- b_k, db2 <- db_k
             step(b1, db1, k - 1);   // b1 <- b_{k-1}, db1 <- db_{k-1}
         }
-        // db1 = Q_j; dg[t][j] = sum_{r,c} Q[r][c] O_t[c][r]
+        }
+        // db1 = Q_j; dg[t][j] = sum_{r,c} Q[r][c] O_t[c][r]: each wave's partial sums for every
+        // control, reduced across lanes, then across waves through LDS with one barrier
         for (int t = 0; t < a.ncoef; ++t) {
             const cplx<R>* O = a.ops + (size_t)(t + 1) * dd;
             R pr = R(0), pi = R(0);
@@ -339,29 +418,39 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void grad_slice_kernel(ChebArgs<R
                     }
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) { pr += __shfl_xor(pr, off, 64); pi += __shfl_xor(pi, off, 64); }
-            cplx<R>* dst = dg + (size_t)t * dg_stride + j;
             if (G == 1) {
-                if (lane == 0) *dst = mk<R>(pr, pi);
-            } else {
-                if (lane == 0) red[wave] = mk<R>(pr, pi);
-                __syncthreads();
-                if (tid == 0) {
-                    cplx<R> sum = red[0];
-                    for (int w = 1; w < G; ++w) sum = cadd(sum, red[w]);
-                    *dst = sum;
-                }
-                __syncthreads();
+                if (lane == 0) dg[(size_t)t * dg_stride + j] = mk<R>(pr, pi);
+            } else if (lane == 0) {
+                red[t * G + wave] = mk<R>(pr, pi);
+            }
+        }
+        if (G > 1) {
+            __syncthreads();
+            for (int t = tid; t < a.ncoef; t += 64 * G) {
+                cplx<R> sum = red[t * G];
+                for (int w = 1; w < G; ++w) sum = cadd(sum, red[t * G + w]);
+                dg[(size_t)t * dg_stride + j] = sum;
             }
         }
         __syncthreads();
     }
 }
 
-// g = Tr(U_t^H U) -> out[0]
+// Root of the gradient sweep, one launch: blocks 0 .. nb-1 write the root costate W = t^H, block nb
+// forms g = Tr(t^H u) (FP64 partial sums, fixed reduction order) into gout[0].
 template <class R>
-__global__ void target_trace_kernel(const cplx<R>* __restrict__ t, const cplx<R>* __restrict__ u, int d,
-                                    cplx<R>* __restrict__ out) {
+__global__ __launch_bounds__(256) void grad_root_kernel(const cplx<R>* __restrict__ t, const cplx<R>* __restrict__ u,
+                                                        int d, int nb, cplx<R>* __restrict__ wroot,
+                                                        cplx<R>* __restrict__ gout) {
     __shared__ double sr[256], si[256];
+    if ((int)blockIdx.x < nb) {
+        const int e = blockIdx.x * 256 + threadIdx.x;
+        if (e >= d * d) return;
+        const int i = e / d, k = e % d;
+        const cplx<R> v = t[k * d + i];
+        wroot[e] = mk<R>(v.x, -v.y);
+        return;
+    }
     double pr = 0.0, pi = 0.0;
     for (int e = threadIdx.x; e < d * d; e += 256) {
         const cplx<R> a = t[e], b = u[e];
@@ -374,7 +463,7 @@ __global__ void target_trace_kernel(const cplx<R>* __restrict__ t, const cplx<R>
         if (threadIdx.x < o) { sr[threadIdx.x] += sr[threadIdx.x + o]; si[threadIdx.x] += si[threadIdx.x + o]; }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = mk<R>((R)sr[0], (R)si[0]);
+    if (threadIdx.x == 0) gout[0] = mk<R>((R)sr[0], (R)si[0]);
 }
 
 // grad[k * pts + p] = sum_j w_jp 2 Re(conj(g) dg[k][j]) / d^2 (quadrature chain rule), F = |g|^2 / d^2.
@@ -422,24 +511,25 @@ hipError_t launch_down_level(const cplx<R>* V, int n_child, int d, const cplx<R>
 }
 
 template <class R>
-hipError_t launch_conj_transpose(const cplx<R>* t, int d, cplx<R>* out, hipStream_t st) {
-    hipLaunchKernelGGL((conj_transpose_kernel<R>), dim3((d * d + 255) / 256), dim3(256), 0, st, t, d, out);
-    return hipGetLastError();
-}
-
-template <class R>
 hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx<R>* W1, cplx<R>* dg, int dg_stride,
                               hipStream_t st) {
     if (a.d <= 0 || a.d > 32 || a.N <= 0 || (a.j0 & 1) || a.M < 3 || a.M >= kMaxCheb || a.ncoef > kGradCoeffs)
         return hipErrorInvalidValue;
-    if (a.d <= 16) hipLaunchKernelGGL((grad_slice_kernel<R, 16>), dim3(a.N), dim3(64 * Shape<16>::G), 0, st, a, E, W1, dg, dg_stride);
-    else hipLaunchKernelGGL((grad_slice_kernel<R, 32>), dim3(a.N), dim3(64 * Shape<32>::G), 0, st, a, E, W1, dg, dg_stride);
+    const dim3 grid(a.N);
+    if (a.d <= 16) {
+        if (a.ps) hipLaunchKernelGGL((grad_slice_kernel<R, 16, true>), grid, dim3(64 * Shape<16>::G), 0, st, a, E, W1, dg, dg_stride);
+        else hipLaunchKernelGGL((grad_slice_kernel<R, 16, false>), grid, dim3(64 * Shape<16>::G), 0, st, a, E, W1, dg, dg_stride);
+    } else {
+        if (a.ps) hipLaunchKernelGGL((grad_slice_kernel<R, 32, true>), grid, dim3(64 * Shape<32>::G), 0, st, a, E, W1, dg, dg_stride);
+        else hipLaunchKernelGGL((grad_slice_kernel<R, 32, false>), grid, dim3(64 * Shape<32>::G), 0, st, a, E, W1, dg, dg_stride);
+    }
     return hipGetLastError();
 }
 
 template <class R>
-hipError_t launch_target_trace(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* out, hipStream_t st) {
-    hipLaunchKernelGGL((target_trace_kernel<R>), dim3(1), dim3(256), 0, st, t, u, d, out);
+hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g, hipStream_t st) {
+    const int nb = (d * d + 255) / 256;
+    hipLaunchKernelGGL((grad_root_kernel<R>), dim3(nb + 1), dim3(256), 0, st, t, u, d, nb, wroot, g);
     return hipGetLastError();
 }
 
@@ -455,10 +545,9 @@ hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pt
 #define PA_GRAD_INSTANTIATE(R)                                                                                       \
     template hipError_t launch_up_level<R>(const cplx<R>*, int, int, cplx<R>*, bool, hipStream_t);                  \
     template hipError_t launch_down_level<R>(const cplx<R>*, int, int, const cplx<R>*, cplx<R>*, hipStream_t);      \
-    template hipError_t launch_conj_transpose<R>(const cplx<R>*, int, cplx<R>*, hipStream_t);                        \
     template hipError_t launch_grad_slices<R>(const ChebArgs<R>&, const cplx<R>*, const cplx<R>*, cplx<R>*, int,     \
                                               hipStream_t);                                                          \
-    template hipError_t launch_target_trace<R>(const cplx<R>*, const cplx<R>*, int, cplx<R>*, hipStream_t);          \
+    template hipError_t launch_grad_root<R>(const cplx<R>*, const cplx<R>*, int, cplx<R>*, cplx<R>*, hipStream_t);   \
     template hipError_t launch_grad_finish<R>(const cplx<R>*, const cplx<R>*, int, int, int, int, int, R*, R*,       \
                                               hipStream_t);
 PA_GRAD_INSTANTIATE(float)

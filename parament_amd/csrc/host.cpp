@@ -747,6 +747,14 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     ca.ctl.mode = h->quad; ca.ctl.magnus = 0; ca.ctl.tau = (R)tau;
     for (int k = 0; k <= M; ++k) { ca.are[k] = (R)are[k]; ca.aim[k] = (R)aim[k]; }
     ca.are[0] = (R)(are[0] - 1.0);  // slices are written as E_j = U_j - I
+    // The derivative kernel evaluates the same polynomial's derivative in powers of tau X when
+    // tau Hnorm <= 1 ((M-1)/2 + 1 product passes instead of M - 1).
+    if (tau * h->hnorm <= 1.0) {
+        double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
+        power_coefficients(M, tau * h->hnorm, are, aim, mr, mi);
+        ca.ps = 1;
+        for (int k = 0; k <= M; ++k) { ca.pre[k] = (R)mr[k]; ca.pim[k] = (R)mi[k]; }
+    }
 
     const GradPlan p = grad_plan(N, h->ckpt_seg, h->ckpt_chunk);
     const size_t mats_vw = p.chunk_mats + p.top_mats;
@@ -788,9 +796,9 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     for (int l = p.seg + 1; l <= p.top; ++l)
         if (pa::launch_up_level<R>(vt(l - 1), p.n[l - 1], d, vt(l), false, st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (pa::launch_target_trace<R>(target_dev, vt(p.top), d, gsum, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (pa::launch_grad_root<R>(target_dev, vt(p.top), d, wt(p.top), gsum, st) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 1);
-    if (pa::launch_conj_transpose<R>(target_dev, d, wt(p.top), st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
     for (int l = p.top - 1; l >= p.seg; --l)
         if (pa::launch_down_level<R>(vt(l), p.n[l], d, wt(l + 1), wt(l), st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;

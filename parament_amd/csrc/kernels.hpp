@@ -20,6 +20,9 @@ template <class R> struct ChebArgs {
     R are[kMaxCheb], aim[kMaxCheb];   // a_k = (-i)^k J_k(tau Hnorm); are[0] holds a_0 - 1
     cplx<R>* out;          // N x d x d, E_j = U_j - I (out[0] is slice j0)
     int j0 = 0;            // first slice
+    // gradient only: the same polynomial in powers of Y = tau X (tau Hnorm <= 1), m_k = pre/pim[k]
+    int ps = 0;
+    R pre[kMaxCheb], pim[kMaxCheb];
 };
 
 // Mixed precision: FP64 operators / coefficients; output: the level-F nodes E = U - I of the
@@ -111,11 +114,11 @@ template <class R> hipError_t launch_up_level(const cplx<R>* in, int n_in, int d
                                               hipStream_t st);
 template <class R> hipError_t launch_down_level(const cplx<R>* V, int n_child, int d, const cplx<R>* Wp, cplx<R>* Wc,
                                                 hipStream_t st);
-template <class R> hipError_t launch_conj_transpose(const cplx<R>* t, int d, cplx<R>* out, hipStream_t st);
 template <class R> hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx<R>* W1,
                                                  cplx<R>* dg, int dg_stride, hipStream_t st);
-template <class R> hipError_t launch_target_trace(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* out,
-                                                  hipStream_t st);
+// root costate wroot = t^H and g = Tr(t^H u), one launch
+template <class R> hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g,
+                                               hipStream_t st);
 template <class R> hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pts, int amps,
                                                  int mode, int d, R* grad, R* fid, hipStream_t st);
 

@@ -43,6 +43,20 @@ def test_gradient_matches_block_expm_oracle(mode, d):
     assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
 
 
+@pytest.mark.parametrize("d", [5, 32])
+@pytest.mark.parametrize("dt", [0.02, 0.3])
+def test_gradient_both_derivative_forms_match_oracle(d, dt):
+    """tau Hnorm below 1 (derivative of the power form) and above 1 (forward-mode Clenshaw
+    derivative): both are the exact derivative of the propagator."""
+    h0, hk, c, t = _problem(1500 + d, d, 3, 17)
+    hnorm = sum(np.abs(m).sum(axis=1).max() for m in (h0, *hk))
+    assert (dt * hnorm < 1) == (dt == 0.02)
+    f, grad = gpu_grad("fp64", h0, hk, c, dt, "none", t)
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, dt, "none", t)
+    assert abs(f - f_ref) < 1e-12
+    assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
+
+
 @pytest.mark.parametrize("mode", ["none", "midpoint", "simpson"])
 @pytest.mark.parametrize("d", [3, 16, 32])
 def test_complex64_gradient_matches_block_expm_oracle(mode, d):
