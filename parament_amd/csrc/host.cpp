@@ -138,6 +138,17 @@ int mixed_order(int m_fp32, double x, int n_slices) {
 
 // ----- context -----
 
+// Everything a gradient call passes to its kernels, pointers and values; zero-filled before
+// use so that byte comparison is exact.
+struct GradKey {
+    const void *carr, *target, *fid, *grad, *ops, *E, *V, *W, *dg, *aux;
+    double dt, hnorm;
+    unsigned pts, amps, amps_set, dim, seg, chunk;
+    int quad, ncoef, M, device;
+    GradKey() { memset(this, 0, sizeof(*this)); }
+    bool operator==(const GradKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
 template <class R> struct Context {
     typedef cplx<R> C;
     Parament_ErrorCode last_error = PARAMENT_STATUS_SUCCESS;
@@ -177,6 +188,11 @@ template <class R> struct Context {
     void* d_gaux = nullptr; size_t cap_gaux = 0;
     unsigned ckpt_seg = 0, ckpt_chunk = 0;
     size_t grad_workspace = 0;  // device bytes the last gradient call worked in
+    // gradient launch sequence captured as a graph once two consecutive calls agree on every
+    // argument that reaches a kernel (GradKey), replayed while they keep agreeing
+    GradKey grad_key, grad_prev;
+    hipGraphExec_t grad_exec = nullptr;
+    hipStream_t cap_stream = nullptr;
 };
 
 template <class R> void free_hamiltonian(Context<R>* h) {
@@ -266,6 +282,8 @@ template <class R> Parament_ErrorCode destroy(Context<R>* h) {
         free_working(h);
         for (auto& e : h->ev)
             if (e) hipEventDestroy(e);
+        if (h->grad_exec) hipGraphExecDestroy(h->grad_exec);
+        if (h->cap_stream) hipStreamDestroy(h->cap_stream);
         hipStreamDestroy(h->own_stream);
     }
     delete h;
@@ -774,7 +792,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     auto wc = [&](int l) { return W + p.coff[l] * dd; };
     auto vt = [&](int l) { return V + (p.chunk_mats + p.toff[l]) * dd; };
     auto wt = [&](int l) { return W + (p.chunk_mats + p.toff[l]) * dd; };
-    hipStream_t st = h->stream;
+    hipStream_t st = h->stream;   // the stream the launches below go to (a capture stream when recording)
     auto count = [](int cnt, int l) { return (int)(((long)cnt + (1L << l) - 1) >> l); };
 
     // Slices of chunk c and their products up to level `upto` (level seg goes to the top arrays).
@@ -790,6 +808,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
         return true;
     };
 
+    auto enqueue = [&]() -> Parament_ErrorCode {
     mark(h, 0);
     for (int c = 0; c < p.nchunks; ++c)
         if (!forward_chunk(c, p.seg)) return PARAMENT_STATUS_CUBLAS_FAILED;
@@ -818,6 +837,46 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
         return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 2);
     return PARAMENT_STATUS_SUCCESS;
+    };
+
+    // Some thirty launches, most of them short: a repeated call replays them as one graph.
+    if (h->timing) return enqueue();
+    GradKey key;
+    key.carr = carr_dev; key.target = target_dev; key.fid = fid_dev; key.grad = grad_dev; key.ops = h->d_ops;
+    key.E = E; key.V = V; key.W = W; key.dg = dg; key.aux = gsum;
+    key.dt = dt; key.hnorm = h->hnorm; key.pts = pts; key.amps = amps; key.amps_set = h->amps; key.dim = h->dim;
+    key.seg = h->ckpt_seg; key.chunk = h->ckpt_chunk; key.quad = h->quad; key.ncoef = h->ncoef; key.M = M;
+    key.device = h->device;
+    if (h->grad_exec && key == h->grad_key)
+        return hipGraphLaunch(h->grad_exec, h->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+    if (!(key == h->grad_prev)) {   // first call with these arguments: launch directly
+        h->grad_prev = key;
+        return enqueue();
+    }
+    if (!h->cap_stream && hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        hipGetLastError(); h->cap_stream = nullptr;
+        return enqueue();
+    }
+    if (h->grad_exec) { hipGraphExecDestroy(h->grad_exec); h->grad_exec = nullptr; }
+    hipGraph_t g = nullptr;
+    st = h->cap_stream;
+    Parament_ErrorCode err = PARAMENT_STATUS_CUBLAS_FAILED;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        err = enqueue();
+        const hipError_t r = hipStreamEndCapture(st, &g);
+        if (err == PARAMENT_STATUS_SUCCESS && r == hipSuccess && g &&
+            hipGraphInstantiate(&h->grad_exec, g, nullptr, nullptr, 0) != hipSuccess)
+            h->grad_exec = nullptr;
+        if (g) hipGraphDestroy(g);
+    }
+    hipGetLastError();
+    st = h->stream;
+    if (err != PARAMENT_STATUS_SUCCESS || !h->grad_exec) {   // capture unavailable: launch directly
+        h->grad_exec = nullptr;
+        return enqueue();
+    }
+    h->grad_key = key;
+    return hipGraphLaunch(h->grad_exec, h->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
 }
 
 template <class R>

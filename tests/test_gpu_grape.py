@@ -182,6 +182,51 @@ def test_gradient_argument_errors(prec):
             p.gate_fidelity_gradient(0.1, np.zeros(4), target=big)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("segment", [0, 1])
+def test_repeated_gradient_calls_replay_identically(prec, segment):
+    """A call repeated with the same arguments is captured once and then replayed as a graph:
+    every repeat equals the first (direct) call bit for bit, and a replay reads the buffers'
+    current contents (controls changed in place match a fresh context's direct call)."""
+    import torch
+    import parament_amd as parament
+    h0, hk, c, t = _problem(77, 16, 3, 300)
+    ct = torch.complex128 if prec == "fp64" else torch.complex64
+    rt = torch.float64 if prec == "fp64" else torch.float32
+    dev = torch.device("cuda:0")
+    carr = torch.tensor(c, dtype=ct, device=dev).contiguous()
+    tgt = torch.tensor(t, dtype=ct, device=dev).contiguous()
+
+    def run(p, n):
+        out = []
+        f = torch.empty(1, dtype=rt, device=dev)
+        g = torch.empty(c.shape, dtype=rt, device=dev)
+        for _ in range(n):
+            p.gate_fidelity_gradient_device(0.02, carr.data_ptr(), c.shape[1], c.shape[0], tgt.data_ptr(),
+                                            f.data_ptr(), g.data_ptr())
+            torch.cuda.synchronize()
+            out.append((f.item(), g.cpu().numpy().copy()))
+        return out
+
+    with parament.Parament(prec) as p:
+        p.set_hamiltonian(h0, *hk, quadrature_mode="simpson")
+        p.set_stream(torch.cuda.current_stream().cuda_stream)
+        p.set_gradient_checkpointing(segment, 0)
+        first, *rest = run(p, 4)
+        for f, g in rest:
+            assert f == first[0] and np.array_equal(g, first[1])
+        carr.mul_(0.5)
+        replayed = run(p, 2)
+    with parament.Parament(prec) as q:
+        q.set_hamiltonian(h0, *hk, quadrature_mode="simpson")
+        q.set_stream(torch.cuda.current_stream().cuda_stream)
+        q.set_gradient_checkpointing(segment, 0)
+        fresh = run(q, 1)[0]
+    for f, g in replayed:
+        assert f == fresh[0] and np.array_equal(g, fresh[1])
+    assert not np.array_equal(fresh[1], first[1])
+
+
 def _torch_problem(prec, seed, d, amps, pts):
     import torch
     h0, hk, c, t = _problem(seed, d, amps, pts)
