@@ -115,11 +115,13 @@ __device__ __forceinline__ void cheb_slice(const cplx<R>* __restrict__ ops, int 
 }
 
 // Block blk of a tree level with the wave split SH: reduces nodes [blk B, blk B + B) of src (n_in
-// nodes) and writes one node to dst (+ I when final_level).  LDS planes Ar, Ai (stride
+// nodes) and writes one node to dst (+ I when final_level), and, given store, every node it
+// completes below that (the levels a gradient sweep reads back).  LDS planes Ar, Ai (stride
 // D + Pad::A), Br, Bi (stride D + Pad::B).
 template <class R, int D, int LOGB, class SrcT, class SH>
 __device__ __forceinline__ void tree_block(const SrcT* __restrict__ src, int n_in, int d, cplx<R>* __restrict__ dst,
-                                           int final_level, long blk, R* Ar, R* Ai, R* Br, R* Bi) {
+                                           int final_level, long blk, R* Ar, R* Ai, R* Br, R* Bi,
+                                           const TreeStore<R>* store = nullptr) {
     constexpr int G = SH::G, T = SH::T;
     constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
     constexpr int B = 1 << LOGB;
@@ -167,6 +169,20 @@ __device__ __forceinline__ void tree_block(const SrcT* __restrict__ src, int n_i
         __syncthreads();
     };
 
+    // node `idx` of level k (1 .. LOGB - 1) to the caller's array, when it exists at that level
+    auto keep = [&](const CTile<R, T, T>& t, int k, long idx) {
+        if (!store || idx >= store->n[k]) return;
+        cplx<R>* o = store->lev[k] + (size_t)idx * dd;
+#pragma unroll
+        for (int m = 0; m < T; ++m)
+#pragma unroll
+            for (int n = 0; n < T; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                    if (row < d && col < d) o[row * d + col] = mk<R>(t.re[m][n][v], t.im[m][n][v]);
+                }
+    };
     CTile<R, T, T> st[LOGB > 1 ? LOGB : 1];  // st[l]: completed level-l node waiting for its sibling
     CTile<R, T, T> cur;
 #pragma unroll
@@ -174,12 +190,14 @@ __device__ __forceinline__ void tree_block(const SrcT* __restrict__ src, int n_i
         stage(base + 2 * p + 1, Ar, Ai, SA);
         stage(base + 2 * p, Br, Bi, SB);
         combine_staged(cur);  // level-1 node p
+        if (LOGB > 1) keep(cur, 1, blk * (B / 2) + p);
 #pragma unroll
         for (int l = 1; l < LOGB; ++l) {
             if (((p >> (l - 1)) & 1) == 0) { st[l] = cur; break; }
             tile_to_lds(cur, Ar, Ai, SA);
             tile_to_lds(st[l], Br, Bi, SB);
-            combine_staged(cur);
+            combine_staged(cur);   // level-(l+1) node p >> l
+            if (l + 1 < LOGB) keep(cur, l + 1, blk * (B >> (l + 1)) + (p >> l));
         }
     }
     cplx<R>* out = dst;

@@ -20,6 +20,7 @@
 // W and its sibling's E = U - I.  The quadrature's chain rule then maps dF/dc'_tj to the raw
 // samples.
 #include "kernels.hpp"
+#include "fused_device.hpp"
 
 namespace pa {
 
@@ -131,50 +132,22 @@ __device__ __forceinline__ void wave_step_products(const R* __restrict__ Xr, con
 
 }  // namespace
 
-// Up-sweep level: out[i] = in[2i+1] in[2i] (the later node on the left), or in[2i] alone for
-// an odd last node.  Leaves arrive as E = U - I (eform = 1) and the product is formed as
-// (I + A)(I + B) = I + A + B + A B; every level above the leaves holds U itself.
-template <class R, int D>
-__global__ __launch_bounds__(64 * Shape<D>::G) void up_level_kernel(const cplx<R>* __restrict__ in, int n_in, int d,
-                                                                       cplx<R>* __restrict__ out, int eform) {
-    constexpr int G = GL<R, D>::G, T = GL<R, D>::T, SA = GL<R, D>::SA, SB = GL<R, D>::SB;
+// Up-sweep, LOGB levels per launch: block w reduces input nodes [w 2^LOGB, (w+1) 2^LOGB) with
+// the forward tree's block body (pairing, odd carry and deviation-from-identity form are the
+// same), keeping every node of the intermediate levels; the last level goes to st.lev[LOGB].
+// All levels hold E = V - I.
+template <class R, int D, int LOGB>
+__global__ __launch_bounds__(64 * Shape<D>::G) void up_block_kernel(const cplx<R>* __restrict__ in, int n_in, int d,
+                                                                       TreeStore<R> st) {
+    constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
     __shared__ R Ar[D * SA], Ai[D * SA], Br[D * SB], Bi[D * SB];
-    const int i = blockIdx.x, dd = d * d, wave = threadIdx.x >> 6;
-    const R one = eform ? R(1) : R(0);
-    cplx<R>* dst = out + (size_t)i * dd;
-    if (2 * i + 1 >= n_in) {
-        const cplx<R>* src = in + (size_t)(2 * i) * dd;
-        for (int e = threadIdx.x; e < dd; e += 64 * G) {
-            cplx<R> v = src[e];
-            if (e / d == e % d) v.x += one;
-            dst[e] = v;
-        }
-        return;
-    }
-    stage<R, D>(in + (size_t)(2 * i + 1) * dd, d, Ar, Ai, SA);
-    stage<R, D>(in + (size_t)(2 * i) * dd, d, Br, Bi, SB);
-    __syncthreads();
-    CTile<R, T, T> w;
-    const int row0 = Shape<D>::row0(wave), col0 = Shape<D>::col0(wave);
-    wave_cgemm<R, T, T, D, SA, SB>(Ar, Ai, Br, Bi, row0, col0, w);
-    if (eform) {
-#pragma unroll
-        for (int m = 0; m < T; ++m)
-#pragma unroll
-            for (int n = 0; n < T; ++n)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                    w.re[m][n][v] += Ar[row * SA + col] + Br[row * SB + col] + (row == col ? R(1) : R(0));
-                    w.im[m][n][v] += Ai[row * SA + col] + Bi[row * SB + col];
-                }
-    }
-    tile_store<R, D>(w, d, dst);
+    tree_block<R, D, LOGB, cplx<R>, Shape<D>>(in, n_in, d, st.lev[LOGB] + (size_t)blockIdx.x * d * d, 0, blockIdx.x,
+                                              Ar, Ai, Br, Bi, &st);
 }
 
 // Down-sweep level: child c of parent c/2 gets W(c) = W(parent) V(c+1) if it is the earlier
 // child (c even), W(c) = V(c-1) W(parent) if it is the later one, and W(parent) if it has no
-// sibling.  One work-group per child.
+// sibling (V = I + E from the stored deviations).  One work-group per child.
 template <class R, int D>
 __global__ __launch_bounds__(64 * Shape<D>::G) void down_level_kernel(const cplx<R>* __restrict__ V, int n_child, int d,
                                                                          const cplx<R>* __restrict__ Wp,
@@ -188,9 +161,9 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void down_level_kernel(const cplx
         for (int e = threadIdx.x; e < dd; e += 64 * G) dst[e] = P[e];
         return;
     }
-    const cplx<R>* S = V + (size_t)sib * dd;
-    if ((c & 1) == 0) { stage<R, D>(P, d, Ar, Ai, SA); stage<R, D>(S, d, Br, Bi, SB); }
-    else { stage<R, D>(S, d, Ar, Ai, SA); stage<R, D>(P, d, Br, Bi, SB); }
+    const cplx<R>* S = V + (size_t)sib * dd;   // E = V - I of the sibling
+    if ((c & 1) == 0) { stage<R, D>(P, d, Ar, Ai, SA); stage<R, D>(S, d, Br, Bi, SB, R(1)); }
+    else { stage<R, D>(S, d, Ar, Ai, SA, R(1)); stage<R, D>(P, d, Br, Bi, SB); }
     __syncthreads();
     CTile<R, T, T> w;
     wave_cgemm<R, T, T, D, SA, SB>(Ar, Ai, Br, Bi, Shape<D>::row0(wave), Shape<D>::col0(wave), w);
@@ -437,7 +410,8 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void grad_slice_kernel(ChebArgs<R
 }
 
 // Root of the gradient sweep, one launch: blocks 0 .. nb-1 write the root costate W = t^H, block nb
-// forms g = Tr(t^H u) (FP64 partial sums, fixed reduction order) into gout[0].
+// forms g = Tr(t^H (I + e)) from the root's deviation e (FP64 partial sums, fixed reduction
+// order) into gout[0].
 template <class R>
 __global__ __launch_bounds__(256) void grad_root_kernel(const cplx<R>* __restrict__ t, const cplx<R>* __restrict__ u,
                                                         int d, int nb, cplx<R>* __restrict__ wroot,
@@ -453,9 +427,10 @@ __global__ __launch_bounds__(256) void grad_root_kernel(const cplx<R>* __restric
     }
     double pr = 0.0, pi = 0.0;
     for (int e = threadIdx.x; e < d * d; e += 256) {
-        const cplx<R> a = t[e], b = u[e];
-        pr += (double)a.x * b.x + (double)a.y * b.y;   // conj(a) b
-        pi += (double)a.x * b.y - (double)a.y * b.x;
+        const cplx<R> a = t[e];
+        const double bx = (double)u[e].x + (e / d == e % d ? 1.0 : 0.0), by = (double)u[e].y;
+        pr += (double)a.x * bx + (double)a.y * by;   // conj(a) b
+        pi += (double)a.x * by - (double)a.y * bx;
     }
     sr[threadIdx.x] = pr; si[threadIdx.x] = pi;
     __syncthreads();
@@ -494,11 +469,15 @@ __global__ void grad_finish_kernel(const cplx<R>* __restrict__ dg, const cplx<R>
 bool grad_supported(int d) { return d >= 1 && d <= 32; }
 
 template <class R>
-hipError_t launch_up_level(const cplx<R>* in, int n_in, int d, cplx<R>* out, bool leaves_eform, hipStream_t st) {
-    if (d <= 0 || d > 32 || n_in <= 0) return hipErrorInvalidValue;
-    const int grid = (n_in + 1) / 2, e = leaves_eform ? 1 : 0;
-    if (d <= 16) hipLaunchKernelGGL((up_level_kernel<R, 16>), dim3(grid), dim3(64 * Shape<16>::G), 0, st, in, n_in, d, out, e);
-    else hipLaunchKernelGGL((up_level_kernel<R, 32>), dim3(grid), dim3(64 * Shape<32>::G), 0, st, in, n_in, d, out, e);
+hipError_t launch_up_block(const cplx<R>* in, int n_in, int d, int logb, const TreeStore<R>& st, hipStream_t s) {
+    if (d <= 0 || d > 32 || n_in <= 0 || logb < 1 || logb > 3) return hipErrorInvalidValue;
+    const int grid = (n_in + (1 << logb) - 1) >> logb;
+#define PA_UP_BLOCK(DD)                                                                                        \
+    if (logb == 1) hipLaunchKernelGGL((up_block_kernel<R, DD, 1>), dim3(grid), dim3(64 * Shape<DD>::G), 0, s, in, n_in, d, st); \
+    else if (logb == 2) hipLaunchKernelGGL((up_block_kernel<R, DD, 2>), dim3(grid), dim3(64 * Shape<DD>::G), 0, s, in, n_in, d, st); \
+    else hipLaunchKernelGGL((up_block_kernel<R, DD, 3>), dim3(grid), dim3(64 * Shape<DD>::G), 0, s, in, n_in, d, st);
+    if (d <= 16) { PA_UP_BLOCK(16) } else { PA_UP_BLOCK(32) }
+#undef PA_UP_BLOCK
     return hipGetLastError();
 }
 
@@ -543,7 +522,7 @@ hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pt
 }
 
 #define PA_GRAD_INSTANTIATE(R)                                                                                       \
-    template hipError_t launch_up_level<R>(const cplx<R>*, int, int, cplx<R>*, bool, hipStream_t);                  \
+    template hipError_t launch_up_block<R>(const cplx<R>*, int, int, int, const TreeStore<R>&, hipStream_t);         \
     template hipError_t launch_down_level<R>(const cplx<R>*, int, int, const cplx<R>*, cplx<R>*, hipStream_t);      \
     template hipError_t launch_grad_slices<R>(const ChebArgs<R>&, const cplx<R>*, const cplx<R>*, cplx<R>*, int,     \
                                               hipStream_t);                                                          \

@@ -795,26 +795,34 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     hipStream_t st = h->stream;   // the stream the launches below go to (a capture stream when recording)
     auto count = [](int cnt, int l) { return (int)(((long)cnt + (1L << l) - 1) >> l); };
 
+    // Up-sweep from level lo (n_lo nodes at `in`) to level hi; out(l) is the array of level l.
+    // Every node is kept as E = V - I.  Three levels per launch while that still leaves a
+    // work-group per CU (a block's products run one after the other), one level per launch above.
+    auto up_levels = [&](const C* in, int lo, int hi, int n_lo, auto out) -> bool {
+        for (int l = lo, n = n_lo; l < hi;) {
+            const int k = (n >= 8 * 256) ? std::min(3, hi - l) : 1;
+            pa::TreeStore<R> ts;
+            for (int i = 1; i <= k; ++i) { n = (n + 1) / 2; ts.lev[i] = out(l + i); ts.n[i] = n; }
+            if (pa::launch_up_block<R>(in, n_lo, d, k, ts, st) != hipSuccess) return false;
+            in = ts.lev[k]; n_lo = n; l += k;
+        }
+        return true;
+    };
     // Slices of chunk c and their products up to level `upto` (level seg goes to the top arrays).
     auto forward_chunk = [&](int c, int upto) -> bool {
         const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
         ca.j0 = j0; ca.N = cnt; ca.out = E;
         if (pa::launch_cheb_fused<R>(ca, st) != hipSuccess) return false;
-        for (int l = 1; l <= upto; ++l) {
-            const C* in = (l == 1) ? E : vc(l - 1);
-            C* out = (l == p.seg) ? vt(p.seg) + (size_t)(j0 >> p.seg) * dd : vc(l);
-            if (pa::launch_up_level<R>(in, count(cnt, l - 1), d, out, l == 1, st) != hipSuccess) return false;
-        }
-        return true;
+        return up_levels(E, 0, upto, cnt,
+                         [&](int l) { return (l == p.seg) ? vt(p.seg) + (size_t)(j0 >> p.seg) * dd : vc(l); });
     };
 
     auto enqueue = [&]() -> Parament_ErrorCode {
     mark(h, 0);
     for (int c = 0; c < p.nchunks; ++c)
         if (!forward_chunk(c, p.seg)) return PARAMENT_STATUS_CUBLAS_FAILED;
-    for (int l = p.seg + 1; l <= p.top; ++l)
-        if (pa::launch_up_level<R>(vt(l - 1), p.n[l - 1], d, vt(l), false, st) != hipSuccess)
-            return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (!up_levels(vt(p.seg), p.seg, p.top, p.n[p.seg], [&](int l) { return vt(l); }))
+        return PARAMENT_STATUS_CUBLAS_FAILED;
     if (pa::launch_grad_root<R>(target_dev, vt(p.top), d, wt(p.top), gsum, st) != hipSuccess)
         return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 1);

@@ -110,8 +110,16 @@ hipError_t launch_ens_tree(int D, int T, int logb, const EnsLevel* lv, const int
 // Adjoint gate-fidelity gradient (grad.hip), dims <= 32.  Tree levels are stored as
 // consecutive d x d matrices; see grad.hip for the scan.
 bool grad_supported(int d);
-template <class R> hipError_t launch_up_level(const cplx<R>* in, int n_in, int d, cplx<R>* out, bool leaves_eform,
-                                              hipStream_t st);
+// Intermediate levels of a block reduction that the caller keeps: lev[k], n[k] = the array and
+// node count of level k above the input (k = 1 .. LOGB - 1; level LOGB is the block's output).
+template <class R> struct TreeStore {
+    cplx<R>* lev[4];
+    int n[4];
+};
+// up-sweep of logb (1..3) levels in one launch, every node kept as E = V - I (st: arrays and
+// counts of the levels above the input, the last one included)
+template <class R> hipError_t launch_up_block(const cplx<R>* in, int n_in, int d, int logb, const TreeStore<R>& st,
+                                              hipStream_t s);
 template <class R> hipError_t launch_down_level(const cplx<R>* V, int n_child, int d, const cplx<R>* Wp, cplx<R>* Wc,
                                                 hipStream_t st);
 template <class R> hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx<R>* W1,
