@@ -201,6 +201,32 @@ def precision_variant(calls):
     return out
 
 
+def small_dim_bench(steps, warmup):
+    """Dim 8, one GUE control, no quadrature (the reference's CPU-sized configuration, 4096 steps,
+    and the same system over 1,048,576 steps): time per equipropDevice call in complex64 and
+    complex128, the slice kernel and tree times, and the algorithmic HBM traffic -- the controls
+    read once and each slice propagator written once and read once by the tree,
+    sizeof(c) * (A P + 2 N d^2) -- over the call and over the slice kernel."""
+    from parament_amd import workloads
+    res = {}
+    for pts in (4096, 1 << 20):
+        h0, hk, c, dt, quad = workloads.dim8_gue(pts=pts)
+        d, (amps, _) = h0.shape[0], c.shape
+        for prec, csize in (("fp32", 8), ("fp64", 16)):
+            r = run_path(prec, prec, False, h0, hk, c, dt, quad, steps, warmup)
+            n = pts   # no quadrature: one slice per step
+            call_bytes = csize * (amps * pts + 2 * n * d * d)
+            slice_bytes = csize * (amps * pts + n * d * d)
+            res[f"{prec}_{pts}"] = {
+                "ms_per_call": r["s_per_call"] * 1e3, "steps_per_s": pts / r["s_per_call"], "M": r["M"],
+                "slice_kernel_ms": r["slice_ms"], "tree_ms": r["tree_ms"],
+                "algorithmic_bytes": call_bytes,
+                "call_GB_per_s": call_bytes / r["s_per_call"] / 1e9,
+                "slice_kernel_GB_per_s": slice_bytes / (r["slice_ms"] * 1e-3) / 1e9}
+    res["workload"] = "dim8_gue: dim 8, 1 GUE control, dt 1e-3, no quadrature; 4096 and 1,048,576 steps"
+    return res
+
+
 def ensemble_bench(calls):
     """512 independent complex64 systems (dims round(exp(U[ln 4, ln 128])), 1-4 controls, 256-8192
     steps, dt 1e-2): time per solve of the whole ensemble as one tuned hipGraph and as the
@@ -258,7 +284,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample-pts", type=int, default=49153)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "fp64", "mixed", "fp32", "grape", "precision", "ensemble"], default="all",
+    ap.add_argument("--only", choices=["all", "fp64", "mixed", "fp32", "grape", "precision", "ensemble", "smalldim"],
+                    default="all",
                     help="run one part only (for profiling); prints that part's measurements")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for calibrating "
@@ -293,6 +320,8 @@ def main():
             part = grape_bench(args.steps, args.warmup)
         elif args.only == "precision":
             part = precision_variant(args.steps)
+        elif args.only == "smalldim":
+            part = small_dim_bench(args.steps, args.warmup)
         else:
             part = ensemble_bench(args.steps)
         print(json.dumps({"only": args.only, **part}), flush=True)
@@ -335,6 +364,7 @@ def main():
     line["grape"] = grape
     line["precision_variant_dim256"] = precision_variant(3)
     line["ensemble"] = ensemble_bench(3)
+    line["small_dim"] = small_dim_bench(args.steps, args.warmup)
     if not args.no_cpu:
         ucpu, cb = cpu_baseline(h0, hk, c, dt, quad, args.cpu_sample_pts)
         line["cpu_baseline"] = cb
