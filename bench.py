@@ -204,9 +204,9 @@ def precision_variant(calls):
 def small_dim_bench(steps, warmup):
     """Dim 8, one GUE control, no quadrature (the reference's CPU-sized configuration, 4096 steps,
     and the same system over 1,048,576 steps): time per equipropDevice call in complex64 and
-    complex128, the slice kernel and tree times, and the algorithmic HBM traffic -- the controls
-    read once and each slice propagator written once and read once by the tree,
-    sizeof(c) * (A P + 2 N d^2) -- over the call and over the slice kernel."""
+    complex128, the slice kernel (with its folded tree levels) and tree times, and the algorithmic
+    HBM traffic -- the controls read once and each slice propagator written once and read once by
+    the tree, sizeof(c) * (A P + 2 N d^2) -- over the whole call."""
     from parament_amd import workloads
     res = {}
     for pts in (4096, 1 << 20):
@@ -216,13 +216,11 @@ def small_dim_bench(steps, warmup):
             r = run_path(prec, prec, False, h0, hk, c, dt, quad, steps, warmup)
             n = pts   # no quadrature: one slice per step
             call_bytes = csize * (amps * pts + 2 * n * d * d)
-            slice_bytes = csize * (amps * pts + n * d * d)
             res[f"{prec}_{pts}"] = {
                 "ms_per_call": r["s_per_call"] * 1e3, "steps_per_s": pts / r["s_per_call"], "M": r["M"],
                 "slice_kernel_ms": r["slice_ms"], "tree_ms": r["tree_ms"],
                 "algorithmic_bytes": call_bytes,
-                "call_GB_per_s": call_bytes / r["s_per_call"] / 1e9,
-                "slice_kernel_GB_per_s": slice_bytes / (r["slice_ms"] * 1e-3) / 1e9}
+                "call_GB_per_s": call_bytes / r["s_per_call"] / 1e9}
     res["workload"] = "dim8_gue: dim 8, 1 GUE control, dt 1e-3, no quadrature; 4096 and 1,048,576 steps"
     return res
 

@@ -15,21 +15,20 @@ template <int D, int T_> struct TileShape {
     static __device__ __forceinline__ int col0(int w) { return 16 * T_ * (w % W); }
 };
 
-// One slice of the fused kernel (see cheb_fused_kernel) with the wave split SH (G waves,
-// T x T blocks of 16x16 each): E_j = U_j - I of slice j into out (d x d).  are / aim hold the
-// Chebyshev coefficients with are[0] = a_0 - 1.  LDS: planes Xr, Xi (stride D + Pad::A),
-// Br, Bi (stride D + Pad::B), coef[ncoef].
+// One slice of the fused kernels with the wave split SH (G waves, T x T blocks of 16x16 each),
+// from its operator coefficients already in coef[]: returns E_j = U_j - I in the accumulator
+// layout.  ops (global or LDS) holds H0 and the operators; are / aim hold the Chebyshev
+// coefficients with are[0] = a_0 - 1.  LDS: planes Xr, Xi (stride D + Pad::A), Br, Bi (stride
+// D + Pad::B).  Ends with a barrier: the planes and coef[] are free afterwards.
 template <class R, int D, class SH>
-__device__ __forceinline__ void cheb_slice(const cplx<R>* __restrict__ ops, int ncoef, int d, const Controls<R>& ctl,
-                                           int j, int M, R s, const R* are, const R* aim, R* Xr, R* Xi, R* Br,
-                                           R* Bi, cplx<R>* coef, cplx<R>* __restrict__ out) {
+__device__ __forceinline__ CTile<R, SH::T, SH::T> cheb_slice_tile(const cplx<R>* __restrict__ ops, int ncoef, int d,
+                                                                  int M, R s, const R* are, const R* aim, R* Xr,
+                                                                  R* Xi, R* Br, R* Bi, const cplx<R>* coef) {
     constexpr int G = SH::G, T = SH::T;
     constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
     const int tid = threadIdx.x, wave = tid >> 6;
     const int row0 = SH::row0(wave), col0 = SH::col0(wave);
     const int dd = d * d;
-    for (int t = tid; t < ncoef; t += 64 * G) coef[t] = slice_coefficient(ctl, t, j);
-    __syncthreads();
     for (int e = tid; e < D * D; e += 64 * G) {
         const int i = e / D, k = e % D;
         R xr = R(0), xi = R(0);
@@ -101,7 +100,22 @@ __device__ __forceinline__ void cheb_slice(const cplx<R>* __restrict__ ops, int 
             __syncthreads();
         }
     }
-    cplx<R>* dst = out;
+    __syncthreads();
+    return b1;
+}
+
+// One slice of the fused kernel (see cheb_fused_kernel): E_j = U_j - I of slice j into out
+// (d x d); LDS as for cheb_slice_tile, plus coef[ncoef].
+template <class R, int D, class SH>
+__device__ __forceinline__ void cheb_slice(const cplx<R>* __restrict__ ops, int ncoef, int d, const Controls<R>& ctl,
+                                           int j, int M, R s, const R* are, const R* aim, R* Xr, R* Xi, R* Br,
+                                           R* Bi, cplx<R>* coef, cplx<R>* __restrict__ out) {
+    constexpr int G = SH::G, T = SH::T;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int row0 = SH::row0(wave), col0 = SH::col0(wave);
+    for (int t = tid; t < ncoef; t += 64 * G) coef[t] = slice_coefficient(ctl, t, j);
+    __syncthreads();
+    const CTile<R, T, T> e = cheb_slice_tile<R, D, SH>(ops, ncoef, d, M, s, are, aim, Xr, Xi, Br, Bi, coef);
 #pragma unroll
     for (int m = 0; m < T; ++m)
 #pragma unroll
@@ -109,9 +123,8 @@ __device__ __forceinline__ void cheb_slice(const cplx<R>* __restrict__ ops, int 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                if (row < d && col < d) dst[row * d + col] = mk<R>(b1.re[m][n][v], b1.im[m][n][v]);
+                if (row < d && col < d) out[row * d + col] = mk<R>(e.re[m][n][v], e.im[m][n][v]);
             }
-    __syncthreads();
 }
 
 // Block blk of a tree level with the wave split SH: reduces nodes [blk B, blk B + B) of src (n_in

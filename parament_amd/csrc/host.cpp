@@ -567,16 +567,22 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     ca.are[0] = (R)a0m1;
 
     if (fused) {
-        const size_t nodes = (size_t)pa::tree_scratch_nodes<R>(N);
-        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)N * dd * sizeof(C)) ||
+        // dims <= 16: wavefront-resident slices with the lowest tree levels folded in
+        const bool small = pa::small_fold_supported((int)d, h->ncoef, sizeof(C));
+        const int F = small ? pa::small_fold_levels(N) : 0;
+        const int n_nodes = (N + (1 << F) - 1) >> F;
+        const size_t nodes = (size_t)pa::tree_scratch_nodes<R>(n_nodes);
+        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)n_nodes * dd * sizeof(C)) ||
             !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(C)))
             return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
         ca.out = reinterpret_cast<C*>(h->d_leaves);
         mark(h, 0);
-        if (pa::launch_cheb_fused<R>(ca, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        if ((small ? pa::launch_small_fold<R>(ca, h->stream) : pa::launch_cheb_fused<R>(ca, h->stream)) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
         mark(h, 1);
         C* nd = reinterpret_cast<C*>(h->d_nodes);
-        if (pa::launch_tree<R>(h->d_leaves, false, N, (int)d, nd, nd + nodes * dd, out_dev, h->stream) != hipSuccess)
+        if (pa::launch_tree<R>(h->d_leaves, false, n_nodes, (int)d, nd, nd + nodes * dd, out_dev, h->stream) !=
+            hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         mark(h, 2);
         return PARAMENT_STATUS_SUCCESS;

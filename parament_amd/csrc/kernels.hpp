@@ -44,6 +44,13 @@ struct MixedArgs {
 // Fused Chebyshev slice kernel, dims <= 64 (padded to 16 / 32 / 64).
 template <class R> hipError_t launch_cheb_fused(const ChebArgs<R>& a, hipStream_t st);
 hipError_t launch_cheb_mixed(const MixedArgs& a, hipStream_t st);
+
+// Small dims (d <= 16, ncoef <= 64, operators within 48 KB): the slices of a.j0 .. a.j0 + N - 1
+// with the lowest small_fold_levels(N) tree levels folded in; a.out receives the level-F nodes
+// (E = U - I), ceil(N / 2^F) of them.
+bool small_fold_supported(int d, int ncoef, size_t elem);
+int small_fold_levels(int N);
+template <class R> hipError_t launch_small_fold(const ChebArgs<R>& a, hipStream_t st);
 bool mixed_supported(int d);
 int mixed_fused_levels(int N);   // tree levels the mixed slice kernel folds in
 

@@ -42,6 +42,22 @@ def case(seed, d, amps, pts):
     return h[0], h[1:], c
 
 
+# ---- dims <= 16: the wavefront-resident kernel folds the lowest tree levels on chip ----
+
+@pytest.mark.parametrize("prec, d, amps, pts, mode", [
+    ("fp32", 8, 1, 8195, "none"),            # one folded level, ragged
+    ("fp64", 8, 1, 16389, "none"),           # two folded levels, ragged
+    ("fp64", 5, 2, 131075, "none"),          # five folded levels, ragged
+    ("fp64", 16, 3, 2 * 16389 + 1, "simpson"),
+    ("fp64", 3, 2, 20001, "midpoint"),
+])
+def test_small_dim_folded_tree_levels(prec, d, amps, pts, mode):
+    """Same pairing and odd carry as the tree kernels: the folded nodes feed the tree above."""
+    h0, hks, c = case(4000 + d + pts, d, amps, pts)
+    u, m = run_gpu(prec, h0, hks, c, 1e-3, mode)
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 1e-3, mode, cycles=m)) < TOL[prec]
+
+
 # ---- the reference's own numerics tests (test_numerics.py), run through the HIP path ----
 
 @pytest.mark.parametrize("dim", [2, 16])
