@@ -174,7 +174,7 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void down_level_kernel(const cplx
 // its parent and the E = U - I of its sibling (E holds this launch's leaves, W1 their
 // parents), then Q_j = Dp(X_j)[W_j] and dg[t * dg_stride + j] = Tr(Q_j O_t).
 template <class R, int D, bool PS>
-__global__ __launch_bounds__(64 * Shape<D>::G) void grad_slice_kernel(ChebArgs<R> a, const cplx<R>* __restrict__ E,
+__global__ __launch_bounds__(64 * Shape<D>::G, 2) void grad_slice_kernel(ChebArgs<R> a, const cplx<R>* __restrict__ E,
                                                                          const cplx<R>* __restrict__ W1,
                                                                          cplx<R>* __restrict__ dg, int dg_stride) {
     constexpr int G = GL<R, D>::G, T = GL<R, D>::T, SA = GL<R, D>::SA, SB = GL<R, D>::SB;
@@ -371,31 +371,40 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void grad_slice_kernel(ChebArgs<R
             step(b1, db1, k - 1);   // b1 <- b_{k-1}, db1 <- db_{k-1}
         }
         }
-        // db1 = Q_j; dg[t][j] = sum_{r,c} Q[r][c] O_t[c][r]: each wave's partial sums for every
-        // control, reduced across lanes, then across waves through LDS with one barrier
-        for (int t = 0; t < a.ncoef; ++t) {
-            const cplx<R>* O = a.ops + (size_t)(t + 1) * dd;
-            R pr = R(0), pi = R(0);
+        // db1 = Q_j; dg[t][j] = sum_{r,c} Q[r][c] O_t[c][r] = sum Q[r][c] O_t^T[r][c]: the transposed
+        // operators give coalesced loads, two controls' loads are issued together, and each
+        // wave's partial sums are reduced across lanes, then across waves through LDS
+        for (int t0 = 0; t0 < a.ncoef; t0 += 2) {
+            cplx<R> o[2][4];
 #pragma unroll
-            for (int m = 0; m < T; ++m)
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-                for (int n = 0; n < T; ++n)
+                for (int v = 0; v < 4; ++v) {
+                    const int row = acc_row<R>(row0, 0, v), col = acc_col(col0, 0);
+                    o[q][v] = (t0 + q < a.ncoef && row < d && col < d)
+                                  ? a.opsT[(size_t)(t0 + q) * dd + row * d + col] : mk<R>(0, 0);
+                }
+            R pr[2], pi[2];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
-                        if (row < d && col < d) {
-                            const cplx<R> o = O[col * d + row];
-                            pr += db1.re[m][n][v] * o.x - db1.im[m][n][v] * o.y;
-                            pi += db1.re[m][n][v] * o.y + db1.im[m][n][v] * o.x;
-                        }
-                    }
+            for (int q = 0; q < 2; ++q) {
+                pr[q] = R(0); pi[q] = R(0);
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) { pr += __shfl_xor(pr, off, 64); pi += __shfl_xor(pi, off, 64); }
-            if (G == 1) {
-                if (lane == 0) dg[(size_t)t * dg_stride + j] = mk<R>(pr, pi);
-            } else if (lane == 0) {
-                red[t * G + wave] = mk<R>(pr, pi);
+                for (int v = 0; v < 4; ++v) {
+                    pr[q] += db1.re[0][0][v] * o[q][v].x - db1.im[0][0][v] * o[q][v].y;
+                    pi[q] += db1.re[0][0][v] * o[q][v].y + db1.im[0][0][v] * o[q][v].x;
+                }
             }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) { pr[q] += __shfl_xor(pr[q], off, 64); pi[q] += __shfl_xor(pi[q], off, 64); }
+            if (lane == 0)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    if (t0 + q >= a.ncoef) break;
+                    if (G == 1) dg[(size_t)(t0 + q) * dg_stride + j] = mk<R>(pr[q], pi[q]);
+                    else red[(t0 + q) * G + wave] = mk<R>(pr[q], pi[q]);
+                }
         }
         if (G > 1) {
             __syncthreads();
@@ -506,6 +515,23 @@ hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx
 }
 
 template <class R>
+__global__ __launch_bounds__(256) void transpose_kernel(const cplx<R>* __restrict__ in, int count, int d,
+                                                        cplx<R>* __restrict__ out) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x, dd = (long)d * d;
+    if (e >= count * dd) return;
+    const long t = e / dd, r = (e % dd) / d, c = e % d;
+    out[e] = in[t * dd + c * d + r];
+}
+
+template <class R>
+hipError_t launch_transpose(const cplx<R>* in, int count, int d, cplx<R>* out, hipStream_t st) {
+    const long total = (long)count * d * d;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL((transpose_kernel<R>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, in, count, d, out);
+    return hipGetLastError();
+}
+
+template <class R>
 hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g, hipStream_t st) {
     const int nb = (d * d + 255) / 256;
     hipLaunchKernelGGL((grad_root_kernel<R>), dim3(nb + 1), dim3(256), 0, st, t, u, d, nb, wroot, g);
@@ -527,6 +553,7 @@ hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pt
     template hipError_t launch_grad_slices<R>(const ChebArgs<R>&, const cplx<R>*, const cplx<R>*, cplx<R>*, int,     \
                                               hipStream_t);                                                          \
     template hipError_t launch_grad_root<R>(const cplx<R>*, const cplx<R>*, int, cplx<R>*, cplx<R>*, hipStream_t);   \
+    template hipError_t launch_transpose<R>(const cplx<R>*, int, int, cplx<R>*, hipStream_t);                        \
     template hipError_t launch_grad_finish<R>(const cplx<R>*, const cplx<R>*, int, int, int, int, int, R*, R*,       \
                                               hipStream_t);
 PA_GRAD_INSTANTIATE(float)

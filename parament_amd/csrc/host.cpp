@@ -333,7 +333,8 @@ Parament_ErrorCode set_hamiltonian(Context<R>* h, const typename ApiCplx<R>::typ
 
     const size_t dd = (size_t)dim * dim;
     const size_t nops = 1 + (size_t)ncoef;
-    if (hipMalloc(&h->d_ops, nops * dd * sizeof(cplx<R>)) != hipSuccess) {
+    // H0 and the ncoef operators, then the operators transposed (read by the gradient's traces)
+    if (hipMalloc(&h->d_ops, (nops + ncoef) * dd * sizeof(cplx<R>)) != hipSuccess) {
         hipGetLastError();
         h->d_ops = nullptr;
         return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
@@ -351,6 +352,10 @@ Parament_ErrorCode set_hamiltonian(Context<R>* h, const typename ApiCplx<R>::typ
     h->hnorm = one_norm(H0, dim);
     for (unsigned k = 0; k < amps; ++k) h->hnorm += one_norm(H1 + k * dd, dim);
     if (use_magnus && magnus_commutators(h) != hipSuccess) {
+        free_hamiltonian(h);
+        return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    }
+    if (pa::launch_transpose<R>(h->d_ops + dd, ncoef, (int)dim, h->d_ops + nops * dd, h->stream) != hipSuccess) {
         free_hamiltonian(h);
         return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
     }
@@ -767,6 +772,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
 
     pa::ChebArgs<R> ca;
     ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = d; ca.M = M; ca.s = (R)(2.0 / h->hnorm);
+    ca.opsT = h->d_ops + (size_t)(1 + h->ncoef) * dd;
     ca.ctl.c = carr_dev; ca.ctl.pts = (int)pts; ca.ctl.amps_in = (int)amps; ca.ctl.amps_set = (int)h->amps;
     ca.ctl.mode = h->quad; ca.ctl.magnus = 0; ca.ctl.tau = (R)tau;
     for (int k = 0; k <= M; ++k) { ca.are[k] = (R)are[k]; ca.aim[k] = (R)aim[k]; }

@@ -23,6 +23,7 @@ template <class R> struct ChebArgs {
     // gradient only: the same polynomial in powers of Y = tau X (tau Hnorm <= 1), m_k = pre/pim[k]
     int ps = 0;
     R pre[kMaxCheb], pim[kMaxCheb];
+    const cplx<R>* opsT = nullptr;   // gradient only: the ncoef operators after H0, each transposed
 };
 
 // Mixed precision: FP64 operators / coefficients; output: the level-F nodes E = U - I of the
@@ -131,6 +132,8 @@ template <class R> hipError_t launch_down_level(const cplx<R>* V, int n_child, i
                                                 hipStream_t st);
 template <class R> hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx<R>* W1,
                                                  cplx<R>* dg, int dg_stride, hipStream_t st);
+// out[t] = in[t]^T for t < count d x d matrices
+template <class R> hipError_t launch_transpose(const cplx<R>* in, int count, int d, cplx<R>* out, hipStream_t st);
 // root costate wroot = t^H and g = Tr(t^H u), one launch
 template <class R> hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g,
                                                hipStream_t st);
