@@ -251,6 +251,29 @@ def ensemble_bench(calls):
         t = (time.perf_counter() - t0) / calls
         res[mode] = {"ms_per_solve": t * 1e3, "steps_per_s": float(pts.sum()) / t, "first_call_s": first}
     res["graph_speedup_vs_loop"] = res["loop"]["ms_per_solve"] / res["graph"]["ms_per_solve"]
+    # forward plus gradient of every instance's gate fidelity (Haar targets)
+    rng = np.random.default_rng(20260006)
+    tgt = torch.from_numpy(np.concatenate([np.ravel(workloads.haar_unitary(rng, int(d))) for d in dims])
+                           .astype(np.complex64)).to("cuda:0")
+    fid = torch.empty(len(dims), dtype=torch.float32, device="cuda:0")
+    grad = torch.empty(int((amps * pts).sum()), dtype=torch.float32, device="cuda:0")
+    gres = {}
+    for mode, reps in (("loop", 1), ("graph", calls)):
+        e.set_execution(mode)
+        t0 = time.perf_counter()
+        e.gate_fidelity_gradient_device(carr.data_ptr(), tgt.data_ptr(), fid.data_ptr(), grad.data_ptr())
+        e.synchronize()
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e.gate_fidelity_gradient_device(carr.data_ptr(), tgt.data_ptr(), fid.data_ptr(), grad.data_ptr())
+        e.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        gres[mode] = {"ms_per_gradient": t * 1e3, "first_call_s": first}
+    gres["graph_speedup_vs_loop"] = gres["loop"]["ms_per_gradient"] / gres["graph"]["ms_per_gradient"]
+    gres["ratio_to_forward"] = gres["graph"]["ms_per_gradient"] / res["graph"]["ms_per_solve"]
+    gres["mean_fidelity"] = float(fid.mean().item())
+    res["gradient"] = gres
     res["tuning"] = {str(k): [int(x) for x in v] if isinstance(v, tuple) else v for k, v in e.tuning.items()}
     res["workload"] = (f"ensemble512: {len(dims)} systems, dims 4-128 ({int((dims > 64).sum())} above 64), "
                        f"{int(pts.sum())} steps in total, complex64")

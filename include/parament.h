@@ -183,8 +183,8 @@ LIBSPEC Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64 *ha
  * propagator U that Parament_equiprop[_fp64] computes, with respect to every control sample
  * (carr layout, real controls: the imaginary parts must be zero).  grad receives amps x pts
  * values dF/dc[k*pts + j] (quadrature chain rule included), fidelity (optional) receives F.
- * For callers optimising controls (GRAPE loops, the PyTorch autograd wrapper).  dim <= 32,
- * no Magnus.  The Device variants take device pointers and are asynchronous on the
+ * For callers optimising controls (GRAPE loops, the PyTorch autograd wrapper).  Any dim (dims
+ * above 32 run on batched GEMMs and keep the whole product tree), no Magnus.  The Device variants take device pointers and are asynchronous on the
  * context's stream; they read only the real parts of carr_dev. */
 LIBSPEC Parament_ErrorCode Parament_gateFidelityGradient(Parament_Context_f32 *handle, const cuComplex *carr,
     double dt, unsigned int pts, unsigned int amps, const cuComplex *target, float *fidelity, float *grad);
@@ -205,7 +205,9 @@ LIBSPEC Parament_ErrorCode Parament_gateFidelityGradientDevice_fp64(Parament_Con
  * slices are processed in chunks of chunk_slices (0: max(segment, 2048); rounded up to a
  * multiple of the segment) whose propagators are computed again in the backward pass, so
  * the working set is O(N / segment + chunk) matrices for the cost of one more forward pass.
- * Results are identical with and without checkpointing.  getGradientWorkspace returns the
+ * Results are identical with and without checkpointing.  Dims above 32 always keep the whole
+ * tree and process their per-slice arrays in chunks of chunk_slices slices (0: as many as fit
+ * in 4 GiB); segment_slices does not apply to them.  getGradientWorkspace returns the
  * device bytes the last gradient call worked in. */
 LIBSPEC Parament_ErrorCode Parament_setGradientCheckpointing(Parament_Context_f32 *handle, unsigned int segment_slices,
     unsigned int chunk_slices);
@@ -243,6 +245,17 @@ LIBSPEC Parament_ErrorCode Parament_ensembleEquipropDevice(Parament_Ensemble *en
     cuComplex *out_dev);
 LIBSPEC Parament_ErrorCode Parament_ensembleSynchronize(Parament_Ensemble *ensemble);
 LIBSPEC Parament_ErrorCode Parament_ensemblePeekAtLastError(Parament_Ensemble *ensemble);
+/* Gate-fidelity gradients of every instance (as Parament_gateFidelityGradient computes them
+ * for it in complex64; real controls).  targets packed like out (dims[i] x dims[i] row-major
+ * each), fidelity[i] = F_i (optional), grad packed like carr: amps[i] x pts[i] real values
+ * dF_i/dc per instance.  setExecution(0): the instances on concurrent branches (fused kernels
+ * for dims <= 32, batched GEMMs above), captured as one HIP graph per (targets, fidelity,
+ * grad) pointer triple and replayed; setExecution(1): one instance after the other.  The
+ * Device variant takes device pointers and is asynchronous on the ensemble's stream. */
+LIBSPEC Parament_ErrorCode Parament_ensembleGateFidelityGradient(Parament_Ensemble *ensemble, const cuComplex *carr,
+    const cuComplex *targets, float *fidelity, float *grad);
+LIBSPEC Parament_ErrorCode Parament_ensembleGateFidelityGradientDevice(Parament_Ensemble *ensemble,
+    const cuComplex *carr_dev, const cuComplex *targets_dev, float *fidelity_dev, float *grad_dev);
 
 #ifdef __cplusplus
 }

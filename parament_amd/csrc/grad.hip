@@ -475,6 +475,57 @@ __global__ void grad_finish_kernel(const cplx<R>* __restrict__ dg, const cplx<R>
     grad[idx] = r;
 }
 
+// Traces of the step-wise gradient: dg[t * dg_stride + j0 + j] = Tr(Q_j O_t) for the n slices
+// of Q (one work-group per slice), from the transposed operators, so that both operands are
+// read along rows.  Eight operators per pass over Q_j.
+template <class R>
+__global__ __launch_bounds__(256) void grad_trace_kernel(const cplx<R>* __restrict__ Q, int d,
+                                                         const cplx<R>* __restrict__ opsT, int ncoef,
+                                                         cplx<R>* __restrict__ dg, int dg_stride, int j0) {
+    constexpr int KB = 8;
+    __shared__ R sr[KB][4], si[KB][4];
+    const long dd = (long)d * d;
+    const int j = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const cplx<R>* q = Q + (size_t)j * dd;
+    for (int t0 = 0; t0 < ncoef; t0 += KB) {
+        R pr[KB], pi[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) { pr[u] = R(0); pi[u] = R(0); }
+        for (long e = threadIdx.x; e < dd; e += 256) {
+            const cplx<R> x = q[e];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                if (t0 + u >= ncoef) break;
+                const cplx<R> o = opsT[(size_t)(t0 + u) * dd + e];
+                pr[u] += x.x * o.x - x.y * o.y;
+                pi[u] += x.x * o.y + x.y * o.x;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+            for (int u = 0; u < KB; ++u) { pr[u] += __shfl_xor(pr[u], off, 64); pi[u] += __shfl_xor(pi[u], off, 64); }
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < KB; ++u) { sr[u][wave] = pr[u]; si[u][wave] = pi[u]; }
+        __syncthreads();
+        if (threadIdx.x < KB && t0 + (int)threadIdx.x < ncoef) {
+            const int u = threadIdx.x;
+            dg[(size_t)(t0 + u) * dg_stride + j0 + j] =
+                mk<R>((sr[u][0] + sr[u][1]) + (sr[u][2] + sr[u][3]), (si[u][0] + si[u][1]) + (si[u][2] + si[u][3]));
+        }
+        __syncthreads();
+    }
+}
+
+template <class R>
+hipError_t launch_grad_trace(const cplx<R>* Q, int n, int d, const cplx<R>* opsT, int ncoef, cplx<R>* dg, int dg_stride,
+                             int j0, hipStream_t st) {
+    if (n <= 0 || d <= 0 || ncoef <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((grad_trace_kernel<R>), dim3(n), dim3(256), 0, st, Q, d, opsT, ncoef, dg, dg_stride, j0);
+    return hipGetLastError();
+}
+
 bool grad_supported(int d) { return d >= 1 && d <= 32; }
 
 template <class R>
@@ -554,6 +605,8 @@ hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pt
                                               hipStream_t);                                                          \
     template hipError_t launch_grad_root<R>(const cplx<R>*, const cplx<R>*, int, cplx<R>*, cplx<R>*, hipStream_t);   \
     template hipError_t launch_transpose<R>(const cplx<R>*, int, int, cplx<R>*, hipStream_t);                        \
+    template hipError_t launch_grad_trace<R>(const cplx<R>*, int, int, const cplx<R>*, int, cplx<R>*, int, int,     \
+                                             hipStream_t);                                                           \
     template hipError_t launch_grad_finish<R>(const cplx<R>*, const cplx<R>*, int, int, int, int, int, R*, R*,       \
                                               hipStream_t);
 PA_GRAD_INSTANTIATE(float)

@@ -744,6 +744,199 @@ GradPlan grad_plan(int N, unsigned seg_req, unsigned chunk_req) {
     return p;
 }
 
+// Step-wise gradient (any dim, batched GEMMs).  Level l of the pairwise tree has n[l] nodes
+// (level 0: the N slices, kept as E = U - I in the leaf array); V holds levels 1 .. top and W the
+// costates of levels 0 .. top, all kept.  The per-slice arrays (X, two Clenshaw and two
+// derivative buffers) cover `chunk` slices at a time.
+struct StepGradPlan {
+    int N = 0, top = 0, chunk = 1;
+    std::vector<int> n;
+    std::vector<size_t> voff, woff;   // matrix offsets of level l in V (l >= 1) and in W
+    size_t vmats = 0, wmats = 0;
+};
+
+StepGradPlan step_grad_plan(int N, int d, size_t elem_bytes, unsigned chunk_req = 0) {
+    StepGradPlan p;
+    p.N = N;
+    while ((1L << p.top) < N) ++p.top;
+    p.voff.assign(p.top + 1, 0);
+    p.woff.assign(p.top + 1, 0);
+    for (int l = 0; l <= p.top; ++l) {
+        p.n.push_back((int)(((long)N + (1L << l) - 1) >> l));
+        if (l >= 1) { p.voff[l] = p.vmats; p.vmats += (size_t)p.n[l]; }
+        p.woff[l] = p.wmats; p.wmats += (size_t)p.n[l];
+    }
+    // the five per-slice arrays within 4 GiB
+    const size_t per_slice = 5 * (size_t)d * d * elem_bytes;
+    size_t chunk = std::max<size_t>(1, std::min<size_t>((size_t)N, ((size_t)4 << 30) / per_slice));
+    if (chunk_req) chunk = std::min<size_t>(chunk, chunk_req);
+    p.chunk = (int)chunk;
+    return p;
+}
+
+// Buffers of one step-wise gradient: E (N slices), work (5 x chunk slices), V (vmats), W (wmats),
+// dg (ncoef x N values), gsum (one value).
+template <class R> struct StepGradBufs {
+    cplx<R> *E = nullptr, *work = nullptr, *V = nullptr, *W = nullptr, *dg = nullptr, *gsum = nullptr;
+};
+
+// The adjoint gradient on batched GEMMs: slices E_j by the Clenshaw recurrence (as equiprop's
+// step-wise path), the up-sweep keeping every node (E' = A + B + A B), the root costate t^H, the
+// down-sweep (earlier child W V(sibling), later child V(sibling) W, a lone child W), then per
+// slice the derivative of the same recurrence in the direction of the slice's costate,
+//   db_k = s (W b_{k+1} + X db_{k+1}) - db_{k+2},   Q = s (W b_1 + X db_1) - 2 db_2,
+// so that Tr(W dU_j) = Tr(Q_j dX_j) (the polynomial's Frechet derivative is self-adjoint under
+// the trace pairing), and dg[t][j] = Tr(Q_j O_t).  ca carries the controls, M, s and the
+// coefficients with are[0] = a_0 - 1; ca.opsT the transposed control operators.
+template <class R>
+hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, const StepGradBufs<R>& b,
+                                 const cplx<R>* target, R* fid, R* grad, hipStream_t st) {
+    typedef cplx<R> C;
+    const int d = ca.d, M = ca.M, N = p.N;
+    const size_t dd = (size_t)d * d;
+    const C zero = pa::mk<R>(0, 0), one = pa::mk<R>(1, 0);
+    const C aM = pa::mk<R>(ca.are[M], ca.aim[M]);
+    const C saM = pa::cmul(pa::mk<R>(ca.s, 0), aM);
+    const C aM1 = pa::mk<R>(ca.are[M - 1], ca.aim[M - 1]);
+    const double s = (double)ca.s;
+    C* X = b.work;
+    C* w[4];
+    for (int i = 0; i < 4; ++i) w[i] = b.work + (size_t)(1 + i) * p.chunk * dd;
+    hipError_t r = hipSuccess;
+    // slices
+    for (int j0 = 0; r == hipSuccess && j0 < N; j0 += p.chunk) {
+        const int cnt = std::min(p.chunk, N - j0);
+        ca.j0 = j0; ca.N = cnt;
+        C* b1 = w[0];
+        C* b2 = w[1];
+        r = pa::launch_assemble<R>(ca, X, st);
+        if (r == hipSuccess) r = pa::launch_fill<R>(nullptr, zero, aM, d, cnt, b2, st);
+        if (r == hipSuccess) r = pa::launch_fill<R>(X, saM, aM1, d, cnt, b1, st);
+        for (int k = M - 2; r == hipSuccess && k >= 0; --k) {
+            C* out = k == 0 ? b.E + (size_t)j0 * dd : b2;
+            r = zgemm_native<R>(X, 1, b1, 1, b2, 1, out, 1, c2(s), c2(k == 0 ? -2.0 : -1.0), c2(ca.are[k], ca.aim[k]),
+                                d, cnt, st);
+            std::swap(b1, b2);
+        }
+    }
+    // up-sweep
+    const C* rd = b.E;
+    for (int l = 1; r == hipSuccess && l <= p.top; ++l) {
+        const int n = p.n[l - 1], half = n / 2;
+        C* wr = b.V + p.voff[l] * dd;
+        if (half) r = zgemm_native<R>(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, c2(1), c2(0), c2(0), d, half, st, true);
+        if (r == hipSuccess && (n & 1)) r = pa::launch_fill<R>(rd + (size_t)(n - 1) * dd, one, zero, d, 1, wr + half * dd, st);
+        rd = wr;
+    }
+    if (r == hipSuccess) r = pa::launch_grad_root<R>(target, rd, d, b.W + p.woff[p.top] * dd, b.gsum, st);
+    // down-sweep
+    for (int l = p.top - 1; r == hipSuccess && l >= 0; --l) {
+        const C* Vl = l == 0 ? b.E : b.V + p.voff[l] * dd;
+        const C* Wp = b.W + p.woff[l + 1] * dd;
+        C* Wc = b.W + p.woff[l] * dd;
+        const int n = p.n[l], half = n / 2;
+        if (half) {
+            r = zgemm_native<R>(Wp, 1, Vl + dd, 2, Wp, 1, Wc, 2, c2(1), c2(1), c2(0), d, half, st);
+            if (r == hipSuccess) r = zgemm_native<R>(Vl, 2, Wp, 1, Wp, 1, Wc + dd, 2, c2(1), c2(1), c2(0), d, half, st);
+        }
+        if (r == hipSuccess && (n & 1))
+            r = pa::launch_fill<R>(Wp + (size_t)(n / 2) * dd, one, zero, d, 1, Wc + (size_t)(n - 1) * dd, st);
+    }
+    // per-slice derivatives and traces
+    for (int j0 = 0; r == hipSuccess && j0 < N; j0 += p.chunk) {
+        const int cnt = std::min(p.chunk, N - j0);
+        ca.j0 = j0; ca.N = cnt;
+        const C* Wj = b.W + (p.woff[0] + (size_t)j0) * dd;
+        C* b1 = w[0];
+        C* b2 = w[1];
+        C* d1 = w[2];
+        C* d2 = w[3];
+        r = pa::launch_assemble<R>(ca, X, st);
+        if (r == hipSuccess) r = pa::launch_fill<R>(nullptr, zero, aM, d, cnt, b2, st);
+        if (r == hipSuccess) r = pa::launch_fill<R>(X, saM, aM1, d, cnt, b1, st);
+        if (r == hipSuccess) r = pa::launch_fill<R>(nullptr, zero, zero, d, cnt, d2, st);
+        if (r == hipSuccess) r = pa::launch_fill<R>(Wj, saM, zero, d, cnt, d1, st);
+        for (int k = M - 2; r == hipSuccess && k >= 0; --k) {
+            r = zgemm_native<R>(X, 1, d1, 1, d2, 1, d2, 1, c2(s), c2(k == 0 ? -2.0 : -1.0), c2(0), d, cnt, st);
+            if (r == hipSuccess) r = zgemm_native<R>(Wj, 1, b1, 1, d2, 1, d2, 1, c2(s), c2(1), c2(0), d, cnt, st);
+            if (k == 0) break;
+            if (r == hipSuccess)
+                r = zgemm_native<R>(X, 1, b1, 1, b2, 1, b2, 1, c2(s), c2(-1.0), c2(ca.are[k], ca.aim[k]), d, cnt, st);
+            std::swap(b1, b2);
+            std::swap(d1, d2);
+        }
+        if (r == hipSuccess) r = pa::launch_grad_trace<R>(d2, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
+    }
+    if (r == hipSuccess)
+        r = pa::launch_grad_finish<R>(b.dg, b.gsum, N, ca.ctl.pts, ca.ctl.amps_in, ca.ctl.mode, d, grad, fid, st);
+    return r;
+}
+
+// Buffers of one gradient on the fused kernels (layout of GradPlan): E (chunk slices), V and W
+// (chunk levels, then top levels), dg (ncoef x N values), gsum (one value).
+template <class R> struct FusedGradBufs { cplx<R> *E, *V, *W, *dg, *gsum; };
+
+// The adjoint gradient on the fused kernels (dims <= 32): slices and up-sweep chunk by chunk,
+// the levels above the segment, the root, the down-sweep, then per chunk (recomputed when
+// checkpointed) the lower down-sweep levels and the per-slice derivatives; the chain rule to
+// the samples last.  h (optional) receives timing marks 0 (start) and 1 (root done).
+template <class R>
+hipError_t grad_fused_enqueue(pa::ChebArgs<R> ca, const GradPlan& p, const FusedGradBufs<R>& b, const cplx<R>* target,
+                              R* fid, R* grad, hipStream_t st, Context<R>* h) {
+    typedef cplx<R> C;
+    const int d = ca.d, N = p.N;
+    const size_t dd = (size_t)d * d;
+    auto vc = [&](int l) { return b.V + p.coff[l] * dd; };
+    auto wc = [&](int l) { return b.W + p.coff[l] * dd; };
+    auto vt = [&](int l) { return b.V + (p.chunk_mats + p.toff[l]) * dd; };
+    auto wt = [&](int l) { return b.W + (p.chunk_mats + p.toff[l]) * dd; };
+    auto count = [](int cnt, int l) { return (int)(((long)cnt + (1L << l) - 1) >> l); };
+    // Up-sweep from level lo (n_lo nodes at `in`) to level hi; out(l) is the array of level l.
+    // Every node is kept as E = V - I.  Three levels per launch while that still leaves a
+    // work-group per CU (a block's products run one after the other), one level per launch above.
+    auto up_levels = [&](const C* in, int lo, int hi, int n_lo, auto out) -> hipError_t {
+        for (int l = lo, n = n_lo; l < hi;) {
+            const int k = (n >= 8 * 256) ? std::min(3, hi - l) : 1;
+            pa::TreeStore<R> ts;
+            for (int i = 1; i <= k; ++i) { n = (n + 1) / 2; ts.lev[i] = out(l + i); ts.n[i] = n; }
+            const hipError_t r = pa::launch_up_block<R>(in, n_lo, d, k, ts, st);
+            if (r != hipSuccess) return r;
+            in = ts.lev[k]; n_lo = n; l += k;
+        }
+        return hipSuccess;
+    };
+    // Slices of chunk c and their products up to level `upto` (level seg goes to the top arrays).
+    auto forward_chunk = [&](int c, int upto) -> hipError_t {
+        const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
+        ca.j0 = j0; ca.N = cnt; ca.out = b.E;
+        const hipError_t r = pa::launch_cheb_fused<R>(ca, st);
+        if (r != hipSuccess) return r;
+        return up_levels(b.E, 0, upto, cnt,
+                         [&](int l) { return (l == p.seg) ? vt(p.seg) + (size_t)(j0 >> p.seg) * dd : vc(l); });
+    };
+    if (h) mark(h, 0);
+    hipError_t r = hipSuccess;
+    for (int c = 0; r == hipSuccess && c < p.nchunks; ++c) r = forward_chunk(c, p.seg);
+    if (r == hipSuccess) r = up_levels(vt(p.seg), p.seg, p.top, p.n[p.seg], [&](int l) { return vt(l); });
+    if (r == hipSuccess) r = pa::launch_grad_root<R>(target, vt(p.top), d, wt(p.top), b.gsum, st);
+    if (h) mark(h, 1);
+    for (int l = p.top - 1; r == hipSuccess && l >= p.seg; --l) r = pa::launch_down_level<R>(vt(l), p.n[l], d, wt(l + 1), wt(l), st);
+    for (int c = 0; r == hipSuccess && c < p.nchunks; ++c) {
+        const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
+        if (p.nchunks > 1) r = forward_chunk(c, p.seg - 1);
+        for (int l = p.seg - 1; r == hipSuccess && l >= 1; --l) {
+            const C* parent = (l + 1 == p.seg) ? wt(p.seg) + (size_t)(j0 >> p.seg) * dd : wc(l + 1);
+            r = pa::launch_down_level<R>(vc(l), count(cnt, l), d, parent, wc(l), st);
+        }
+        const C* W1 = (p.seg == 1) ? wt(1) + (size_t)(j0 >> 1) * dd : wc(1);
+        ca.j0 = j0; ca.N = cnt;
+        if (r == hipSuccess) r = pa::launch_grad_slices<R>(ca, b.E, W1, b.dg, N, st);
+    }
+    if (r == hipSuccess)
+        r = pa::launch_grad_finish<R>(b.dg, b.gsum, N, ca.ctl.pts, ca.ctl.amps_in, ca.ctl.mode, d, grad, fid, st);
+    return r;
+}
+
 // All device pointers; target is d x d row-major, grad receives amps x pts values of dF/dc
 // (real controls: only the real parts of carr are read), fid (optional) F = |Tr(U_t^H U)|^2 / d^2.
 template <class R>
@@ -751,7 +944,9 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
                             const cplx<R>* target_dev, R* fid_dev, R* grad_dev) {
     typedef cplx<R> C;
     if (!h->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
-    if (h->magnus || !pa::grad_supported((int)h->dim) || h->ncoef > pa::kGradCoeffs) return PARAMENT_STATUS_INVALID_VALUE;
+    // dims <= 32: the fused kernels (on-chip slices and tree levels); larger dims: batched GEMMs
+    const bool stepwise = !pa::grad_supported((int)h->dim);
+    if (h->magnus || (!stepwise && h->ncoef > pa::kGradCoeffs)) return PARAMENT_STATUS_INVALID_VALUE;
     if (amps == 0 || amps > h->amps) return PARAMENT_STATUS_INVALID_VALUE;
     if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
     int N = 0;
@@ -786,77 +981,38 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
         for (int k = 0; k <= M; ++k) { ca.pre[k] = (R)mr[k]; ca.pim[k] = (R)mi[k]; }
     }
 
-    const GradPlan p = grad_plan(N, h->ckpt_seg, h->ckpt_chunk);
+    // the fused path's tree layout (checkpointed or not); the step-wise path keeps the whole tree
+    const GradPlan p = grad_plan(stepwise ? 1 : N, h->ckpt_seg, h->ckpt_chunk);
+    const StepGradPlan sp = step_grad_plan(stepwise ? N : 1, d, sizeof(C), h->ckpt_chunk);
     const size_t mats_vw = p.chunk_mats + p.top_mats;
-    const size_t bytes_e = (size_t)p.chunk * dd * sizeof(C), bytes_vw = mats_vw * dd * sizeof(C);
+    const size_t bytes_e = stepwise ? ((size_t)N + 5 * (size_t)sp.chunk) * dd * sizeof(C) : (size_t)p.chunk * dd * sizeof(C);
+    const size_t bytes_v = (stepwise ? std::max<size_t>(sp.vmats, 1) : mats_vw) * dd * sizeof(C);
+    const size_t bytes_w = (stepwise ? sp.wmats : mats_vw) * dd * sizeof(C);
     const size_t bytes_dg = (size_t)h->ncoef * N * sizeof(C);
-    if (!grow(&h->d_gE, &h->cap_gE, bytes_e) || !grow(&h->d_gV, &h->cap_gV, bytes_vw) ||
-        !grow(&h->d_gW, &h->cap_gW, bytes_vw) || !grow(&h->d_gdg, &h->cap_gdg, bytes_dg) ||
+    if (!grow(&h->d_gE, &h->cap_gE, bytes_e) || !grow(&h->d_gV, &h->cap_gV, bytes_v) ||
+        !grow(&h->d_gW, &h->cap_gW, bytes_w) || !grow(&h->d_gdg, &h->cap_gdg, bytes_dg) ||
         !grow(&h->d_gaux, &h->cap_gaux, 4 * sizeof(C)))
         return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
-    h->grad_workspace = bytes_e + 2 * bytes_vw + bytes_dg + 4 * sizeof(C);
+    h->grad_workspace = bytes_e + bytes_v + bytes_w + bytes_dg + 4 * sizeof(C);
     C* E = reinterpret_cast<C*>(h->d_gE);
     C* V = reinterpret_cast<C*>(h->d_gV);
     C* W = reinterpret_cast<C*>(h->d_gW);
     C* dg = reinterpret_cast<C*>(h->d_gdg);
     C* gsum = reinterpret_cast<C*>(h->d_gaux);
-    auto vc = [&](int l) { return V + p.coff[l] * dd; };
-    auto wc = [&](int l) { return W + p.coff[l] * dd; };
-    auto vt = [&](int l) { return V + (p.chunk_mats + p.toff[l]) * dd; };
-    auto wt = [&](int l) { return W + (p.chunk_mats + p.toff[l]) * dd; };
     hipStream_t st = h->stream;   // the stream the launches below go to (a capture stream when recording)
-    auto count = [](int cnt, int l) { return (int)(((long)cnt + (1L << l) - 1) >> l); };
-
-    // Up-sweep from level lo (n_lo nodes at `in`) to level hi; out(l) is the array of level l.
-    // Every node is kept as E = V - I.  Three levels per launch while that still leaves a
-    // work-group per CU (a block's products run one after the other), one level per launch above.
-    auto up_levels = [&](const C* in, int lo, int hi, int n_lo, auto out) -> bool {
-        for (int l = lo, n = n_lo; l < hi;) {
-            const int k = (n >= 8 * 256) ? std::min(3, hi - l) : 1;
-            pa::TreeStore<R> ts;
-            for (int i = 1; i <= k; ++i) { n = (n + 1) / 2; ts.lev[i] = out(l + i); ts.n[i] = n; }
-            if (pa::launch_up_block<R>(in, n_lo, d, k, ts, st) != hipSuccess) return false;
-            in = ts.lev[k]; n_lo = n; l += k;
-        }
-        return true;
-    };
-    // Slices of chunk c and their products up to level `upto` (level seg goes to the top arrays).
-    auto forward_chunk = [&](int c, int upto) -> bool {
-        const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
-        ca.j0 = j0; ca.N = cnt; ca.out = E;
-        if (pa::launch_cheb_fused<R>(ca, st) != hipSuccess) return false;
-        return up_levels(E, 0, upto, cnt,
-                         [&](int l) { return (l == p.seg) ? vt(p.seg) + (size_t)(j0 >> p.seg) * dd : vc(l); });
-    };
-
     auto enqueue = [&]() -> Parament_ErrorCode {
-    mark(h, 0);
-    for (int c = 0; c < p.nchunks; ++c)
-        if (!forward_chunk(c, p.seg)) return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (!up_levels(vt(p.seg), p.seg, p.top, p.n[p.seg], [&](int l) { return vt(l); }))
-        return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (pa::launch_grad_root<R>(target_dev, vt(p.top), d, wt(p.top), gsum, st) != hipSuccess)
-        return PARAMENT_STATUS_CUBLAS_FAILED;
-    mark(h, 1);
-    for (int l = p.top - 1; l >= p.seg; --l)
-        if (pa::launch_down_level<R>(vt(l), p.n[l], d, wt(l + 1), wt(l), st) != hipSuccess)
-            return PARAMENT_STATUS_CUBLAS_FAILED;
-    for (int c = 0; c < p.nchunks; ++c) {
-        const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
-        if (p.nchunks > 1 && !forward_chunk(c, p.seg - 1)) return PARAMENT_STATUS_CUBLAS_FAILED;
-        for (int l = p.seg - 1; l >= 1; --l) {
-            const C* parent = (l + 1 == p.seg) ? wt(p.seg) + (size_t)(j0 >> p.seg) * dd : wc(l + 1);
-            if (pa::launch_down_level<R>(vc(l), count(cnt, l), d, parent, wc(l), st) != hipSuccess)
-                return PARAMENT_STATUS_CUBLAS_FAILED;
+        hipError_t r;
+        if (stepwise) {
+            StepGradBufs<R> sb;
+            sb.E = E; sb.work = E + (size_t)N * dd; sb.V = V; sb.W = W; sb.dg = dg; sb.gsum = gsum;
+            mark(h, 0);
+            r = grad_stepwise_enqueue<R>(ca, sp, sb, target_dev, fid_dev, grad_dev, st);
+            mark(h, 1);
+        } else {
+            r = grad_fused_enqueue<R>(ca, p, FusedGradBufs<R>{E, V, W, dg, gsum}, target_dev, fid_dev, grad_dev, st, h);
         }
-        const C* W1 = (p.seg == 1) ? wt(1) + (size_t)(j0 >> 1) * dd : wc(1);
-        ca.j0 = j0; ca.N = cnt;
-        if (pa::launch_grad_slices<R>(ca, E, W1, dg, N, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
-    }
-    if (pa::launch_grad_finish<R>(dg, gsum, N, (int)pts, (int)amps, h->quad, d, grad_dev, fid_dev, st) != hipSuccess)
-        return PARAMENT_STATUS_CUBLAS_FAILED;
-    mark(h, 2);
-    return PARAMENT_STATUS_SUCCESS;
+        mark(h, 2);
+        return r == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
     };
 
     // Some thirty launches, most of them short: a repeated call replays them as one graph.
@@ -988,6 +1144,7 @@ struct EnsInst {
     int tile = 0;                               // batched-GEMM output tile (tuned; 0 = by volume)
     size_t ops_off = 0, c_off = 0, out_off = 0; // element offsets in the packed arrays
     size_t work_off = 0;                        // element offset of its working memory
+    size_t opsT_off = 0;                        // element offset of its transposed controls (gradient)
     double are[pa::kMaxCheb], aim[pa::kMaxCheb];
 };
 
@@ -1021,6 +1178,16 @@ struct Ensemble {
     int exec_mode = 0;              // 0: captured graph, 1: per-instance loop
     hipGraphExec_t gexec = nullptr;
     std::vector<void*> allocs;      // small device tables
+    // gradient: transposed control operators, instance -> branch, per-branch workspaces, the
+    // captured graph and the pointers it was captured with, staging for the host-array call
+    float2* d_opsT = nullptr;
+    std::vector<int> gbranch;
+    float2* d_gws[kEnsSide] = {};
+    bool grad_ready = false;
+    hipGraphExec_t gexec_grad = nullptr;
+    const void* gkey[3] = {};
+    float2* d_tgt = nullptr;
+    float *d_fid = nullptr, *d_grad = nullptr;
 };
 
 void ens_free_plan(Ensemble* e) {
@@ -1031,6 +1198,11 @@ void ens_free_plan(Ensemble* e) {
         for (int b = 0; b < 2; ++b) { c.d_lv[b].clear(); c.d_lt[b].clear(); c.n_lt[b].clear(); }
     }
     if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+    if (e->gexec_grad) { hipGraphExecDestroy(e->gexec_grad); e->gexec_grad = nullptr; }
+    for (auto& w : e->d_gws) { hipFree(w); w = nullptr; }
+    hipFree(e->d_opsT); e->d_opsT = nullptr;
+    e->gbranch.clear();
+    e->grad_ready = false;
     hipFree(e->d_work); e->d_work = nullptr; e->n_work = 0;
     e->d_inst = nullptr;   // one of allocs
     e->large.clear();
@@ -1090,6 +1262,7 @@ Parament_ErrorCode ens_destroy(Ensemble* e) {
     if (e->stream) hipStreamSynchronize(e->stream);
     ens_free_plan(e);
     hipFree(e->d_ops); hipFree(e->d_c); hipFree(e->d_out);
+    hipFree(e->d_tgt); hipFree(e->d_fid); hipFree(e->d_grad);
     for (int k = 0; k < kEnsSide; ++k) {
         if (e->side[k]) hipStreamDestroy(e->side[k]);
         if (e->join_ev[k]) hipEventDestroy(e->join_ev[k]);
@@ -1370,6 +1543,165 @@ Parament_ErrorCode ens_run(Ensemble* e) {
     return hipGraphLaunch(e->gexec, e->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
 }
 
+// ----- ensemble gradients -----
+// Each instance's gate-fidelity gradient runs the single-problem gradient: the fused kernels
+// for dims <= 32 (whole tree kept), batched GEMMs above.  Instances are spread over the side
+// streams by longest-first greedy balancing of their arithmetic; one branch reuses one
+// workspace for its instances, one after the other.  The whole solve is captured as a graph.
+struct EnsGradLayout {
+    bool fused = false;
+    GradPlan p;
+    StepGradPlan sp;
+    size_t off[5] = {};   // element offsets: E (+ per-slice arrays), V, W, dg, gsum
+    size_t total = 0;
+};
+
+EnsGradLayout ens_grad_layout(const EnsInst& x) {
+    EnsGradLayout L;
+    const size_t dd = (size_t)x.d * x.d;
+    auto al = [](size_t n) { return (n + 63) / 64 * 64; };
+    L.fused = pa::grad_supported((int)x.d) && (int)x.amps <= pa::kGradCoeffs;
+    size_t sz[5];
+    if (L.fused) {
+        L.p = grad_plan(x.N, 0, 0);
+        const size_t vw = L.p.chunk_mats + L.p.top_mats;
+        sz[0] = (size_t)L.p.chunk * dd; sz[1] = vw * dd; sz[2] = vw * dd;
+    } else {
+        // per-slice arrays within 1 GiB per instance
+        const unsigned cap = (unsigned)std::max<size_t>(1, ((size_t)1 << 30) / (5 * dd * sizeof(float2)));
+        L.sp = step_grad_plan(x.N, (int)x.d, sizeof(float2), cap);
+        sz[0] = ((size_t)x.N + 5 * (size_t)L.sp.chunk) * dd;
+        sz[1] = std::max<size_t>(L.sp.vmats, 1) * dd;
+        sz[2] = L.sp.wmats * dd;
+    }
+    sz[3] = (size_t)x.amps * x.N;
+    sz[4] = 4;
+    for (int k = 0; k < 5; ++k) { L.off[k] = L.total; L.total += al(sz[k]); }
+    return L;
+}
+
+Parament_ErrorCode ens_grad_setup(Ensemble* e) {
+    if (e->grad_ready) return PARAMENT_STATUS_SUCCESS;
+    size_t n_opsT = 0;
+    for (EnsInst& x : e->in) { x.opsT_off = n_opsT; n_opsT += (size_t)x.amps * x.d * x.d; }
+    if (hipMalloc(&e->d_opsT, n_opsT * sizeof(float2)) != hipSuccess) {
+        hipGetLastError(); e->d_opsT = nullptr;
+        return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    for (const EnsInst& x : e->in)
+        if (pa::launch_transpose<float>(e->d_ops + x.ops_off + (size_t)x.d * x.d, (int)x.amps, (int)x.d,
+                                        e->d_opsT + x.opsT_off, e->stream) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+    // longest first onto the least loaded branch
+    const int nb = kEnsSide;
+    std::vector<int> order(e->in.size());
+    std::vector<double> cost(e->in.size());
+    for (size_t i = 0; i < e->in.size(); ++i) {
+        const EnsInst& x = e->in[i];
+        order[i] = (int)i;
+        cost[i] = (double)x.d * x.d * x.d * x.N * (x.M + (x.d <= 32 ? 2.0 : 3.0 * x.M));
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    e->gbranch.assign(e->in.size(), 0);
+    double load[kEnsSide] = {};
+    size_t need[kEnsSide] = {};
+    for (int i : order) {
+        int k = 0;
+        for (int q = 1; q < nb; ++q)
+            if (load[q] < load[k]) k = q;
+        e->gbranch[i] = k;
+        load[k] += cost[i];
+        need[k] = std::max(need[k], ens_grad_layout(e->in[i]).total);
+    }
+    for (int k = 0; k < nb; ++k) {
+        if (!need[k]) continue;
+        if (hipMalloc(&e->d_gws[k], need[k] * sizeof(float2)) != hipSuccess) {
+            hipGetLastError(); e->d_gws[k] = nullptr;
+            return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+        }
+    }
+    e->grad_ready = true;
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+// Instance i's gradient on stream st in workspace ws; controls from e->d_c.
+hipError_t ens_grad_instance(Ensemble* e, int i, float2* ws, const float2* tgt, float* fid, float* grad, hipStream_t st) {
+    const EnsInst& x = e->in[i];
+    const EnsGradLayout L = ens_grad_layout(x);
+    pa::ChebArgs<float> ca;
+    ca.ops = e->d_ops + x.ops_off; ca.ncoef = (int)x.amps; ca.d = (int)x.d; ca.N = x.N; ca.M = x.M;
+    ca.s = (float)(2.0 / x.hnorm);
+    ca.opsT = e->d_opsT + x.opsT_off;
+    ca.ctl.c = e->d_c + x.c_off; ca.ctl.pts = (int)x.pts; ca.ctl.amps_in = (int)x.amps; ca.ctl.amps_set = (int)x.amps;
+    ca.ctl.mode = e->quad; ca.ctl.magnus = 0; ca.ctl.tau = (float)x.tau;
+    for (int k = 0; k <= x.M; ++k) { ca.are[k] = (float)x.are[k]; ca.aim[k] = (float)x.aim[k]; }
+    ca.are[0] = (float)(x.are[0] - 1.0);
+    if (L.fused && x.tau * x.hnorm <= 1.0) {
+        double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
+        power_coefficients(x.M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
+        ca.ps = 1;
+        for (int k = 0; k <= x.M; ++k) { ca.pre[k] = (float)mr[k]; ca.pim[k] = (float)mi[k]; }
+    }
+    const float2* t = tgt + x.out_off;
+    float* f = fid ? fid + i : nullptr;
+    float* g = grad + x.c_off;
+    if (L.fused)
+        return grad_fused_enqueue<float>(ca, L.p, FusedGradBufs<float>{ws + L.off[0], ws + L.off[1], ws + L.off[2], ws + L.off[3],
+                                                                      ws + L.off[4]},
+                                         t, f, g, st, nullptr);
+    StepGradBufs<float> sb;
+    sb.E = ws + L.off[0]; sb.work = sb.E + (size_t)x.N * x.d * x.d; sb.V = ws + L.off[1]; sb.W = ws + L.off[2];
+    sb.dg = ws + L.off[3]; sb.gsum = ws + L.off[4];
+    return grad_stepwise_enqueue<float>(ca, L.sp, sb, t, f, g, st);
+}
+
+// All instances: branches forked from and joined to e->stream (graph mode), or one after the
+// other on e->stream (loop mode).
+hipError_t ens_grad_enqueue(Ensemble* e, const float2* tgt, float* fid, float* grad) {
+    if (e->exec_mode == 1) {
+        hipError_t r = hipSuccess;
+        for (size_t i = 0; r == hipSuccess && i < e->in.size(); ++i)
+            r = ens_grad_instance(e, (int)i, e->d_gws[e->gbranch[i]], tgt, fid, grad, e->stream);
+        return r;
+    }
+    hipError_t r = hipEventRecord(e->fork_ev, e->stream);
+    bool used[kEnsSide] = {};
+    for (size_t i = 0; r == hipSuccess && i < e->in.size(); ++i) {
+        const int k = e->gbranch[i];
+        if (!used[k]) { r = hipStreamWaitEvent(e->side[k], e->fork_ev, 0); used[k] = true; }
+        if (r == hipSuccess) r = ens_grad_instance(e, (int)i, e->d_gws[k], tgt, fid, grad, e->side[k]);
+    }
+    for (int k = 0; r == hipSuccess && k < kEnsSide; ++k) {
+        if (!used[k]) continue;
+        r = hipEventRecord(e->join_ev[k], e->side[k]);
+        if (r == hipSuccess) r = hipStreamWaitEvent(e->stream, e->join_ev[k], 0);
+    }
+    return r;
+}
+
+// Gradients of all instances (controls already in e->d_c).  Graph mode captures the solve once
+// per (targets, fidelity, gradient) pointer triple and replays it.
+Parament_ErrorCode ens_grad_run(Ensemble* e, const float2* tgt, float* fid, float* grad) {
+    if (!e->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
+    Parament_ErrorCode err = ens_grad_setup(e);
+    if (err != PARAMENT_STATUS_SUCCESS) return err;
+    if (e->exec_mode == 1)
+        return ens_grad_enqueue(e, tgt, fid, grad) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+    if (!e->gexec_grad || e->gkey[0] != tgt || e->gkey[1] != fid || e->gkey[2] != grad) {
+        if (e->gexec_grad) { hipGraphExecDestroy(e->gexec_grad); e->gexec_grad = nullptr; }
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        const hipError_t r = ens_grad_enqueue(e, tgt, fid, grad);
+        const hipError_t r2 = hipStreamEndCapture(e->stream, &g);
+        if (r != hipSuccess || r2 != hipSuccess || !g) { hipGetLastError(); if (g) hipGraphDestroy(g); return PARAMENT_STATUS_CUBLAS_FAILED; }
+        const hipError_t r3 = hipGraphInstantiate(&e->gexec_grad, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (r3 != hipSuccess) { hipGetLastError(); e->gexec_grad = nullptr; return PARAMENT_STATUS_CUBLAS_FAILED; }
+        e->gkey[0] = tgt; e->gkey[1] = fid; e->gkey[2] = grad;
+    }
+    return hipGraphLaunch(e->gexec_grad, e->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1636,6 +1968,42 @@ Parament_ErrorCode Parament_ensembleSynchronize(Parament_Ensemble* ensemble) {
 }
 Parament_ErrorCode Parament_ensemblePeekAtLastError(Parament_Ensemble* ensemble) {
     return ensemble ? reinterpret_cast<Ensemble*>(ensemble)->last_error : PARAMENT_STATUS_INVALID_VALUE;
+}
+Parament_ErrorCode Parament_ensembleGateFidelityGradient(Parament_Ensemble* ensemble, const cuComplex* carr,
+                                                         const cuComplex* targets, float* fidelity, float* grad) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    if (!carr || !targets || !grad) return e->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    if (hipSetDevice(e->device) != hipSuccess) return e->last_error = PARAMENT_FAIL;
+    if ((!e->d_tgt && hipMalloc(&e->d_tgt, e->n_out * sizeof(float2)) != hipSuccess) ||
+        (!e->d_fid && hipMalloc(&e->d_fid, e->in.size() * sizeof(float)) != hipSuccess) ||
+        (!e->d_grad && hipMalloc(&e->d_grad, std::max<size_t>(e->n_c, 1) * sizeof(float)) != hipSuccess)) {
+        hipGetLastError();
+        return e->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    if ((e->n_c && hipMemcpyAsync(e->d_c, carr, e->n_c * sizeof(float2), hipMemcpyHostToDevice, e->stream) != hipSuccess) ||
+        hipMemcpyAsync(e->d_tgt, targets, e->n_out * sizeof(float2), hipMemcpyHostToDevice, e->stream) != hipSuccess)
+        return e->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    Parament_ErrorCode err = ens_grad_run(e, e->d_tgt, e->d_fid, e->d_grad);
+    if (err == PARAMENT_STATUS_SUCCESS &&
+        (hipMemcpyAsync(grad, e->d_grad, e->n_c * sizeof(float), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+         (fidelity && hipMemcpyAsync(fidelity, e->d_fid, e->in.size() * sizeof(float), hipMemcpyDeviceToHost, e->stream) !=
+                          hipSuccess) ||
+         hipStreamSynchronize(e->stream) != hipSuccess))
+        err = PARAMENT_STATUS_CUBLAS_FAILED;
+    if (err != PARAMENT_STATUS_SUCCESS) hipStreamSynchronize(e->stream);
+    return e->last_error = err;
+}
+Parament_ErrorCode Parament_ensembleGateFidelityGradientDevice(Parament_Ensemble* ensemble, const cuComplex* carr_dev,
+                                                               const cuComplex* targets_dev, float* fidelity_dev,
+                                                               float* grad_dev) {
+    if (!ensemble) return PARAMENT_STATUS_INVALID_VALUE;
+    auto* e = reinterpret_cast<Ensemble*>(ensemble);
+    if (!carr_dev || !targets_dev || !grad_dev) return e->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    if (hipSetDevice(e->device) != hipSuccess) return e->last_error = PARAMENT_FAIL;
+    if (e->n_c && hipMemcpyAsync(e->d_c, carr_dev, e->n_c * sizeof(float2), hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+        return e->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    return e->last_error = ens_grad_run(e, reinterpret_cast<const float2*>(targets_dev), fidelity_dev, grad_dev);
 }
 
 void device_info(void) {

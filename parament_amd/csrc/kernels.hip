@@ -302,7 +302,7 @@ template <class R, class TO>
 __global__ __launch_bounds__(256) void assemble_kernel(ChebArgs<R> a, TO* __restrict__ X, int blocks_per) {
     __shared__ cplx<R> coef[kMaxCoeffs];
     const int j = blockIdx.x / blocks_per, b = blockIdx.x % blocks_per;
-    for (int t = threadIdx.x; t < a.ncoef; t += blockDim.x) coef[t] = slice_coefficient(a.ctl, t, j);
+    for (int t = threadIdx.x; t < a.ncoef; t += blockDim.x) coef[t] = slice_coefficient(a.ctl, t, a.j0 + j);
     __syncthreads();
     const int dd = a.d * a.d;
     const int e = b * 256 + threadIdx.x;

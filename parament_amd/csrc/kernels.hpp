@@ -134,6 +134,9 @@ template <class R> hipError_t launch_grad_slices(const ChebArgs<R>& a, const cpl
                                                  cplx<R>* dg, int dg_stride, hipStream_t st);
 // out[t] = in[t]^T for t < count d x d matrices
 template <class R> hipError_t launch_transpose(const cplx<R>* in, int count, int d, cplx<R>* out, hipStream_t st);
+// dg[t * dg_stride + j0 + j] = Tr(Q_j O_t), O_t^T = opsT[t], for the n matrices Q_j (any d)
+template <class R> hipError_t launch_grad_trace(const cplx<R>* Q, int n, int d, const cplx<R>* opsT, int ncoef,
+                                                cplx<R>* dg, int dg_stride, int j0, hipStream_t st);
 // root costate wroot = t^H and g = Tr(t^H u), one launch
 template <class R> hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g,
                                                hipStream_t st);

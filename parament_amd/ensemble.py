@@ -82,6 +82,37 @@ class Ensemble:
             off += d * d
         return res
 
+    def gate_fidelity_gradient(self, controls, targets):
+        """Per instance F_i = |Tr(targets[i]^H U_i)|^2 / d_i^2 and dF_i/dc (adjoint method, real
+        controls): returns (array of F_i, list of (amps[i], pts[i]) float32 gradients)."""
+        carr = self._pack_controls(controls)
+        if np.any(carr.imag != 0):
+            raise ValueError("gradients need real control amplitudes")
+        if len(targets) != len(self.dims):
+            raise ValueError("one target per instance")
+        parts = []
+        for i, t in enumerate(targets):
+            t = np.asarray(t, dtype=np.complex64)
+            d = int(self.dims[i])
+            if t.shape != (d, d):
+                raise ValueError(f"instance {i}: target must have shape ({d}, {d})")
+            parts.append(np.ravel(t, order="C"))
+        tgt = np.ascontiguousarray(np.concatenate(parts))
+        fid = np.zeros(len(self.dims), dtype=np.float32)
+        grad = np.zeros(carr.size, dtype=np.float32)
+        self._check(self._lib.Parament_ensembleGateFidelityGradient(self._handle, carr, tgt, fid, grad))
+        res, off = [], 0
+        for a, p in zip(self.amps.astype(int), self.pts.astype(int)):
+            res.append(grad[off:off + a * p].reshape(a, p))
+            off += a * p
+        return fid, res
+
+    def gate_fidelity_gradient_device(self, carr_ptr, targets_ptr, fid_ptr, grad_ptr):
+        """Device-pointer variant (packed layouts; fid_ptr may be 0); asynchronous."""
+        self._check(self._lib.Parament_ensembleGateFidelityGradientDevice(
+            self._handle, ctypes.c_void_p(carr_ptr), ctypes.c_void_p(targets_ptr),
+            ctypes.c_void_p(fid_ptr) if fid_ptr else None, ctypes.c_void_p(grad_ptr)))
+
     def equiprop_device(self, carr_ptr, out_ptr):
         """Device-pointer variant on the ensemble's own stream (packed layouts); asynchronous."""
         self._check(self._lib.Parament_ensembleEquipropDevice(self._handle, ctypes.c_void_p(carr_ptr),

@@ -106,6 +106,11 @@ lib.Parament_ensembleEquiprop.argtypes = [c_ParamentContext_p, c_cuComplex_p, c_
 lib.Parament_ensembleEquipropDevice.argtypes = [c_ParamentContext_p, ctypes.c_void_p, ctypes.c_void_p]
 lib.Parament_ensembleSynchronize.argtypes = [c_ParamentContext_p]
 lib.Parament_ensemblePeekAtLastError.argtypes = [c_ParamentContext_p]
+lib.Parament_ensembleGateFidelityGradient.argtypes = [c_ParamentContext_p, c_cuComplex_p, c_cuComplex_p,
+                                                      np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS"),
+                                                      np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
+lib.Parament_ensembleGateFidelityGradientDevice.argtypes = [c_ParamentContext_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                            ctypes.c_void_p, ctypes.c_void_p]
 lib.Parament_errorMessage.argtypes = [ctypes.c_int]
 lib.Parament_errorMessage.restype = ctypes.c_char_p
 lib.Parament_selectIterationCycles_fp32.argtypes = [ctypes.c_double, ctypes.c_double]

@@ -79,3 +79,106 @@ def test_ensemble_errors():
         rng = np.random.default_rng(1)
         with pytest.raises(RuntimeError, match="Timestep too large"):
             e.set_hamiltonians([[gue(rng, 4) * 50, gue(rng, 4)]])
+
+
+GDIMS = [3, 16, 32, 33, 48, 70, 5]
+GAMPS = [2, 1, 3, 2, 1, 2, 4]
+GPTS = [21, 40, 17, 9, 23, 11, 64]
+
+
+def _targets(seed, dims):
+    from parament_amd.workloads import haar_unitary
+    rng = np.random.default_rng(seed)
+    return [haar_unitary(rng, d) for d in dims]
+
+
+@pytest.mark.parametrize("quad", ["none", "simpson"])
+def test_ensemble_gradient_matches_block_expm_oracle(quad):
+    """Every instance's F and dF/dc (fused kernels up to dim 32, batched GEMMs above) against
+    the block-matrix-expm gradient of its own problem, to complex64 accuracy."""
+    from oracle import grape_oracle as go
+    from parament_amd.ensemble import Ensemble
+    pts = [p + 2 for p in GPTS] if quad == "simpson" else GPTS
+    dt = [0.03 + 0.01 * i for i in range(len(GDIMS))]
+    hs, cs = make_ensemble(11, GDIMS, GAMPS, pts, dt)
+    ts = _targets(12, GDIMS)
+    with Ensemble(GDIMS, GAMPS, pts, dt, quad) as e:
+        e.set_hamiltonians(hs)
+        fid, grads = e.gate_fidelity_gradient(cs, ts)
+    for i, (h, c, t) in enumerate(zip(hs, cs, ts)):
+        f_ref, g_ref = go.fidelity_gradient(h[0], h[1:], c, dt[i], quad, t)
+        assert abs(fid[i] - f_ref) < 1e-5 * max(f_ref, 1e-3), (i, GDIMS[i])
+        assert grads[i].shape == c.shape and grads[i].dtype == np.float32
+        err = np.abs(grads[i] - g_ref).max() / np.abs(g_ref).max()
+        assert err < 1e-4, (i, GDIMS[i], err)
+
+
+def test_ensemble_gradient_graph_loop_and_context_agree():
+    """The captured graph equals the per-instance loop bit for bit, replays repeat it, new
+    controls go through the same graph, and each instance matches a complex64 context."""
+    import parament_amd as parament
+    from parament_amd.ensemble import Ensemble
+    dt = [0.05] * len(GDIMS)
+    hs, cs = make_ensemble(13, GDIMS, GAMPS, GPTS, dt)
+    ts = _targets(14, GDIMS)
+    with Ensemble(GDIMS, GAMPS, GPTS, dt) as e:
+        e.set_hamiltonians(hs)
+        e.set_execution("loop")
+        f_loop, g_loop = e.gate_fidelity_gradient(cs, ts)
+        e.set_execution("graph")
+        f_graph, g_graph = e.gate_fidelity_gradient(cs, ts)
+        assert np.array_equal(f_loop, f_graph)
+        for a, b in zip(g_loop, g_graph):
+            assert np.array_equal(a, b)
+        for _ in range(2):
+            np.sum(np.random.default_rng(0).normal(size=1000))   # host work between replays
+            f2, g2 = e.gate_fidelity_gradient(cs, ts)
+            assert np.array_equal(f2, f_graph)
+            for a, b in zip(g2, g_graph):
+                assert np.array_equal(a, b)
+        cs2 = [-c for c in cs]
+        f3, g3 = e.gate_fidelity_gradient(cs2, ts)
+        e.set_execution("loop")
+        f4, g4 = e.gate_fidelity_gradient(cs2, ts)
+        assert np.array_equal(f3, f4)
+        for a, b in zip(g3, g4):
+            assert np.array_equal(a, b)
+        assert not np.array_equal(g3[0], g_graph[0])
+    for i, (h, c, t) in enumerate(zip(hs, cs, ts)):
+        with parament.Parament("fp32") as p:
+            p.set_hamiltonian(h[0], *h[1:])
+            f, g = p.gate_fidelity_gradient(dt[i], *c, target=t)
+        assert abs(f - f_graph[i]) <= 1e-6 * max(f, 1e-3)
+        assert np.abs(g - g_graph[i]).max() <= 1e-6 * np.abs(g).max(), i
+
+
+def test_ensemble_gradient_device_pointers():
+    import torch
+    from parament_amd.ensemble import Ensemble
+    dt = [0.04] * len(GDIMS)
+    hs, cs = make_ensemble(15, GDIMS, GAMPS, GPTS, dt)
+    ts = _targets(16, GDIMS)
+    dev = torch.device("cuda:0")
+    carr = torch.tensor(np.concatenate([np.ravel(c) for c in cs]), dtype=torch.complex64, device=dev)
+    tgt = torch.tensor(np.concatenate([np.ravel(t) for t in ts]), dtype=torch.complex64, device=dev)
+    fid = torch.zeros(len(GDIMS), dtype=torch.float32, device=dev)
+    grad = torch.zeros(carr.numel(), dtype=torch.float32, device=dev)
+    with Ensemble(GDIMS, GAMPS, GPTS, dt) as e:
+        e.set_hamiltonians(hs)
+        f_host, g_host = e.gate_fidelity_gradient(cs, ts)
+        e.gate_fidelity_gradient_device(carr.data_ptr(), tgt.data_ptr(), fid.data_ptr(), grad.data_ptr())
+        e.synchronize()
+    assert np.array_equal(fid.cpu().numpy(), f_host)
+    assert np.array_equal(grad.cpu().numpy(), np.concatenate([np.ravel(g) for g in g_host]))
+
+
+def test_ensemble_gradient_errors():
+    from parament_amd.ensemble import Ensemble
+    with Ensemble([4], [1], [10], 0.1) as e:
+        with pytest.raises(RuntimeError, match="No hamiltonian"):
+            e.gate_fidelity_gradient([np.zeros((1, 10))], [np.eye(4)])
+        e.set_hamiltonians([[np.eye(4), np.eye(4)]])
+        with pytest.raises(ValueError):
+            e.gate_fidelity_gradient([np.zeros((1, 10)) + 1j], [np.eye(4)])
+        with pytest.raises(ValueError):
+            e.gate_fidelity_gradient([np.zeros((1, 10))], [np.eye(3)])

@@ -69,6 +69,50 @@ def test_complex64_gradient_matches_block_expm_oracle(mode, d):
     assert err < 1e-4, err
 
 
+@pytest.mark.parametrize("mode", ["none", "midpoint", "simpson"])
+@pytest.mark.parametrize("d", [33, 48, 70])
+def test_stepwise_gradient_matches_block_expm_oracle(mode, d):
+    """Dims above 32 take the batched-GEMM gradient (whole tree kept, Clenshaw derivative in the
+    direction of each slice's costate, one trace per control)."""
+    h0, hk, c, t = _problem(4000 + d, d, 2, 21)
+    f, grad = gpu_grad("fp64", h0, hk, c, 0.05, mode, t)
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, 0.05, mode, t)
+    assert abs(f - f_ref) < 1e-12
+    assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
+
+
+@pytest.mark.parametrize("d,dt", [(40, 0.02), (40, 0.4), (65, 0.1)])
+def test_complex64_stepwise_gradient_matches_block_expm_oracle(d, dt):
+    h0, hk, c, t = _problem(4100 + d, d, 3, 41)
+    f, grad = gpu_grad("fp32", h0, hk, c, dt, "simpson", t)
+    assert grad.dtype == np.float32
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, dt, "simpson", t)
+    assert abs(f - f_ref) < 1e-5 * max(f_ref, 1e-3)
+    err = np.abs(grad - grad_ref).max() / np.abs(grad_ref).max()
+    assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("pts,mode", [(1, "none"), (2, "none"), (3, "simpson"), (5, "midpoint"), (33, "none")])
+def test_stepwise_gradient_few_slices(pts, mode):
+    """One slice (no tree), two, and odd counts that carry a node up a level."""
+    h0, hk, c, t = _problem(4200 + pts, 36, 2, pts)
+    f, grad = gpu_grad("fp64", h0, hk, c, 0.07, mode, t)
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, 0.07, mode, t)
+    assert abs(f - f_ref) < 1e-12
+    assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_stepwise_gradient_chunks_agree(prec):
+    """Per-slice arrays processed 7 slices at a time give the whole-batch result."""
+    h0, hk, c, t = _problem(4300, 40, 2, 61)
+    f0, g0 = gpu_grad(prec, h0, hk, c, 0.05, "none", t)
+    f1, g1 = gpu_grad(prec, h0, hk, c, 0.05, "none", t, segment=1, chunk=7)
+    tol = 1e-13 if prec == "fp64" else 1e-6
+    assert abs(f0 - f1) <= tol
+    assert np.abs(g0 - g1).max() <= tol * max(1.0, np.abs(g0).max())
+
+
 def test_gradient_matches_central_differences_dim32_4096_steps():
     """The GRAPE workload at full size: dim 32, 4 GUE controls, 4096 steps, dt 2e-3, Haar
     target; central differences (h = 1e-6) of the FP64 equiprop at 16 random entries."""
@@ -178,19 +222,22 @@ def test_gradient_argument_errors(prec):
             p.gate_fidelity_gradient(0.1, np.zeros(4), np.zeros(4), target=t)
         big = np.eye(33)
         p.set_hamiltonian(big, big)
-        with pytest.raises(ValueError, match="Invalid value"):
-            p.gate_fidelity_gradient(0.1, np.zeros(4), target=big)
+        with pytest.raises(ValueError, match="dim x dim"):
+            p.gate_fidelity_gradient(0.1, np.zeros(4), target=t)
+        # dims above 32 (batched GEMMs): a control on the identity only moves the global phase
+        f, g = p.gate_fidelity_gradient(0.1, np.zeros(4), target=big)
+        assert abs(f - 1) < 1e-5 and np.abs(g).max() < 1e-4
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("segment", [0, 1])
-def test_repeated_gradient_calls_replay_identically(prec, segment):
+@pytest.mark.parametrize("segment,d", [(0, 16), (1, 16), (0, 40)])
+def test_repeated_gradient_calls_replay_identically(prec, segment, d):
     """A call repeated with the same arguments is captured once and then replayed as a graph:
     every repeat equals the first (direct) call bit for bit, and a replay reads the buffers'
     current contents (controls changed in place match a fresh context's direct call)."""
     import torch
     import parament_amd as parament
-    h0, hk, c, t = _problem(77, 16, 3, 300)
+    h0, hk, c, t = _problem(77, d, 3, 300)
     ct = torch.complex128 if prec == "fp64" else torch.complex64
     rt = torch.float64 if prec == "fp64" else torch.float32
     dev = torch.device("cuda:0")
