@@ -58,10 +58,12 @@ struct EpiIO {
 };
 
 // FP64 / FP32 matrix-core tiles: A chunk [64][32] (stride SA), B chunk [32][64] (stride SB).
+// The next K chunk is fetched into registers while the matrix cores work on the current one.
 template <class R, int BT, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
     constexpr int TW = BT / 32;   // 16x16 blocks per wave side (4 waves, 2 x 2)
     constexpr int SA = KT + Pad<R>::A, SB = BT + Pad<R>::B;
+    constexpr int NE = BT * KT / 256;   // chunk elements per thread, of A and of B
     __shared__ R Ar[BT * SA], Ai[BT * SA], Br[KT * SB], Bi[KT * SB];
     const int d = g.d, tiles = (d + BT - 1) / BT;
     const long total = (long)g.batch * tiles * tiles;
@@ -74,24 +76,36 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
     const TB* B = io.B + b * g.sb * dd;
     const int tid = threadIdx.x, wave = tid >> 6;
     const int wr = (BT / 2) * (wave >> 1), wc = (BT / 2) * (wave & 1);
+    TA ra[NE];
+    TB rb[NE];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u;
+            const int gi = ti * BT + e / KT, gk = k0 + e % KT;
+            ra[u].x = 0; ra[u].y = 0;
+            if (gi < d && gk < d) ra[u] = A[(long)gi * d + gk];
+        }
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u;
+            const int gk = k0 + e / BT, gj = tj * BT + e % BT;
+            rb[u].x = 0; rb[u].y = 0;
+            if (gk < d && gj < d) rb[u] = B[(long)gk * d + gj];
+        }
+    };
     CTile<R, TW, TW> acc;
     acc.zero();
+    fetch(0);
     for (int k0 = 0; k0 < d; k0 += KT) {
-        for (int e = tid; e < BT * KT; e += 256) {
-            const int r = e / KT, k = e % KT;
-            const int gi = ti * BT + r, gk = k0 + k;
-            double2 v = mk<double>(0, 0);
-            if (gi < d && gk < d) v = ld(A, (long)gi * d + gk);
-            Ar[r * SA + k] = (R)v.x; Ai[r * SA + k] = (R)v.y;
-        }
-        for (int e = tid; e < KT * BT; e += 256) {
-            const int k = e / BT, c = e % BT;
-            const int gk = k0 + k, gj = tj * BT + c;
-            double2 v = mk<double>(0, 0);
-            if (gk < d && gj < d) v = ld(B, (long)gk * d + gj);
-            Br[k * SB + c] = (R)v.x; Bi[k * SB + c] = (R)v.y;
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u;
+            Ar[(e / KT) * SA + e % KT] = (R)ra[u].x; Ai[(e / KT) * SA + e % KT] = (R)ra[u].y;
+            Br[(e / BT) * SB + e % BT] = (R)rb[u].x; Bi[(e / BT) * SB + e % BT] = (R)rb[u].y;
         }
         __syncthreads();
+        if (k0 + KT < d) fetch(k0 + KT);
         wave_cgemm<R, TW, TW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc);
         __syncthreads();
     }
@@ -163,24 +177,39 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             r -= (float)h;
         }
     };
-    for (int k0 = 0; k0 < d; k0 += KT) {
-        for (int e = tid; e < BT * KT; e += 256) {
-            const int r = e / KT, k = e % KT;
-            const int gi = ti * BT + r, gk = k0 + k;
-            double2 v = mk<double>(0, 0);
-            if (gi < d && gk < d) v = ld(A, (long)gi * d + gk);
-            split_store(Ap, r * S + k, (float)v.x, 0);
-            split_store(Ap, r * S + k, (float)v.y, 1);
+    // the next K chunk is fetched into registers while the matrix cores work on the current one
+    constexpr int NE = BT * KT / 256;
+    TA ra[NE];
+    TB rb[NE];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u;
+            const int gi = ti * BT + e / KT, gk = k0 + e % KT;
+            ra[u].x = 0; ra[u].y = 0;
+            if (gi < d && gk < d) ra[u] = A[(long)gi * d + gk];
         }
-        for (int e = tid; e < KT * BT; e += 256) {
-            const int k = e / BT, c = e % BT;
-            const int gk = k0 + k, gj = tj * BT + c;
-            double2 v = mk<double>(0, 0);
-            if (gk < d && gj < d) v = ld(B, (long)gk * d + gj);
-            split_store(Bp, c * S + k, (float)v.x, 0);
-            split_store(Bp, c * S + k, (float)v.y, 1);
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u;
+            const int gk = k0 + e / BT, gj = tj * BT + e % BT;
+            rb[u].x = 0; rb[u].y = 0;
+            if (gk < d && gj < d) rb[u] = B[(long)gk * d + gj];
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < d; k0 += KT) {
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u;
+            const int oa = (e / KT) * S + e % KT, ob = (e % BT) * S + e / BT;
+            split_store(Ap, oa, (float)ra[u].x, 0);
+            split_store(Ap, oa, (float)ra[u].y, 1);
+            split_store(Bp, ob, (float)rb[u].x, 0);
+            split_store(Bp, ob, (float)rb[u].y, 1);
         }
         __syncthreads();
+        if (k0 + KT < d) fetch(k0 + KT);
         bf16x8 br[TW][NS], bi[TW][NS];
 #pragma unroll
         for (int n = 0; n < TW; ++n)
