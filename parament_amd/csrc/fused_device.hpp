@@ -20,22 +20,30 @@ template <int D, int T_> struct TileShape {
 // layout.  ops (global or LDS) holds H0 and the operators; are / aim hold the Chebyshev
 // coefficients with are[0] = a_0 - 1.  LDS: planes Xr, Xi (stride D + Pad::A), Br, Bi (stride
 // D + Pad::B).  Ends with a barrier: the planes and coef[] are free afterwards.
-template <class R, int D, class SH>
+//
+// NS > 1 (d <= D / NS): NS slices at once as the diagonal blocks of one D x D tile, block q at
+// rows and columns [q D/NS, (q+1) D/NS) with coefficients coef[64 q ..].  Every step of the
+// recurrence maps block-diagonal matrices to block-diagonal matrices, and a block's products
+// only add exact zeros from the others, so each block holds what the slice alone would.
+template <class R, int D, class SH, int NS = 1>
 __device__ __forceinline__ CTile<R, SH::T, SH::T> cheb_slice_tile(const cplx<R>* __restrict__ ops, int ncoef, int d,
                                                                   int M, R s, const R* are, const R* aim, R* Xr,
                                                                   R* Xi, R* Br, R* Bi, const cplx<R>* coef) {
     constexpr int G = SH::G, T = SH::T;
     constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
+    constexpr int HB = D / NS;
     const int tid = threadIdx.x, wave = tid >> 6;
     const int row0 = SH::row0(wave), col0 = SH::col0(wave);
     const int dd = d * d;
     for (int e = tid; e < D * D; e += 64 * G) {
         const int i = e / D, k = e % D;
+        const int q = i / HB, ii = i % HB, kk = k % HB;
         R xr = R(0), xi = R(0);
-        if (i < d && k < d) {
-            cplx<R> x = ops[i * d + k];
+        if (k / HB == q && ii < d && kk < d) {
+            const cplx<R>* cq = coef + 64 * q;
+            cplx<R> x = ops[ii * d + kk];
             for (int t = 0; t < ncoef; ++t)
-                x = cadd(x, cmul(coef[t], ops[(size_t)(t + 1) * dd + i * d + k]));
+                x = cadd(x, cmul(cq[t], ops[(size_t)(t + 1) * dd + ii * d + kk]));
             xr = x.x; xi = x.y;
         }
         Xr[i * SA + k] = xr; Xi[i * SA + k] = xi;

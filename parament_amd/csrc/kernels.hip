@@ -47,19 +47,24 @@ template hipError_t launch_cheb_fused<double>(const ChebArgs<double>&, hipStream
 // next slice's coefficient t while the current slice runs; and the F lowest tree levels are
 // folded on chip in the reference's pairing (binary counter, later node on the left, on
 // deviations from identity, a slice beyond N acting as the zero node of the odd carry).  Node w
-// of level F covers slices [w 2^F, (w+1) 2^F) and goes to a.out as E = U - I.
-template <class R, int F>
+// of level F covers slices [w 2^F, (w+1) 2^F) and goes to a.out as E = U - I.  For d <= 16 / NS
+// a tile carries NS independent slices as diagonal blocks (cheb_slice_tile): group w folds
+// nodes w, w + groups, .. side by side, so the 16 x 16 matrix-core products that pad a
+// d <= 8 slice do the work of NS slices.
+template <class R, int F, int NS>
 __global__ __launch_bounds__(64) void small_fold_kernel(ChebArgs<R> a, int n_nodes) {
-    constexpr int D = 16, NB = 1 << F;
+    constexpr int D = 16, NB = 1 << F, HB = D / NS;
     constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
     typedef Shape<16> SH;
     typedef CTile<R, 1, 1> Tile;
     __shared__ R Xr[D * SA], Xi[D * SA], Br[D * SB], Bi[D * SB];
-    __shared__ cplx<R> coef[64];
+    __shared__ cplx<R> coef[64 * NS];
     extern __shared__ __align__(16) unsigned char small_ops_lds[];
     cplx<R>* ops = reinterpret_cast<cplx<R>*>(small_ops_lds);
     const int tid = threadIdx.x, d = a.d, dd = d * d;
     for (int e = tid; e < (1 + a.ncoef) * dd; e += 64) ops[e] = a.ops[e];
+    // tile group w holds nodes w + q groups (q < NS) in its diagonal blocks
+    const int groups = (n_nodes + NS - 1) / NS;
 
     // out = A + B + A B for the later node A and the earlier node B
     auto combine = [&](const Tile& A, const Tile& B, Tile& out) {
@@ -79,21 +84,47 @@ __global__ __launch_bounds__(64) void small_fold_kernel(ChebArgs<R> a, int n_nod
         }
         __syncthreads();
     };
-    auto first_of = [&](int w) { return w < n_nodes ? w * NB : -1; };
-    cplx<R> next = mk<R>(0, 0);   // lane t: coefficient t of the slice about to run
-    if (tid < a.ncoef && first_of(blockIdx.x) >= 0) next = slice_coefficient(a.ctl, tid, a.j0 + first_of(blockIdx.x));
-    for (int w = blockIdx.x; w < n_nodes; w += gridDim.x) {
+    // slice p of block q of group w (may lie beyond N: then a zero node)
+    auto slice_of = [&](int w, int q, int p) { return (w + q * groups) * NB + p; };
+    // lane t: coefficient t of each block's slice about to run (zero beyond N)
+    cplx<R> next[NS];
+    auto prefetch = [&](int w, int p) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            next[q] = mk<R>(0, 0);
+            const int j = (w < groups) ? slice_of(w, q, p) : a.N;
+            if (tid < a.ncoef && j < a.N) next[q] = slice_coefficient(a.ctl, tid, a.j0 + j);
+        }
+    };
+    prefetch(blockIdx.x, 0);
+    for (int w = blockIdx.x; w < groups; w += gridDim.x) {
         Tile stk[F > 0 ? F : 1];
         Tile node;
+        // steps below slice N of the block each accumulator element lies in
+        int lim[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lim[v] = a.N - (w + (acc_row<R>(0, 0, v) / HB) * groups) * NB;
+        // one slice per tile: steps up to slice N - 1; several: every step of every group
+        const int jcap = NS > 1 ? groups * NB : a.N;
 #pragma unroll
         for (int p = 0; p < NB; ++p) {
-            const int j = w * NB + p;
-            if (j < a.N) {   // uniform
-                if (tid < a.ncoef) coef[tid] = next;
-                const int jn = (j + 1 < a.N && p + 1 < NB) ? j + 1 : first_of(w + (int)gridDim.x);
-                if (tid < a.ncoef && jn >= 0 && jn < a.N) next = slice_coefficient(a.ctl, tid, a.j0 + jn);
+            const int j = slice_of(w, 0, p);
+            // one slice per tile: slices beyond N are skipped (uniform), the ragged block below;
+            // several: every block runs all NB steps, slices beyond N as zero nodes, which the
+            // binary counter combines exactly as the odd-carry rule
+            if (j < jcap) {
+                if (tid < a.ncoef)
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) coef[64 * q + tid] = next[q];
+                if (p + 1 < NB && (NS > 1 || j + 1 < a.N)) prefetch(w, p + 1);
+                else prefetch(w + (int)gridDim.x, 0);
                 __syncthreads();
-                Tile e = cheb_slice_tile<R, D, SH>(ops, a.ncoef, d, a.M, a.s, a.are, a.aim, Xr, Xi, Br, Bi, coef);
+                Tile e = cheb_slice_tile<R, D, SH, NS>(ops, a.ncoef, d, a.M, a.s, a.are, a.aim, Xr, Xi, Br, Bi, coef);
+                if (NS > 1) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (p >= lim[v]) { e.re[0][0][v] = R(0); e.im[0][0][v] = R(0); }
+                }
                 int merged = 0;
 #pragma unroll
                 for (int l = 0; l < F; ++l) {
@@ -109,22 +140,25 @@ __global__ __launch_bounds__(64) void small_fold_kernel(ChebArgs<R> a, int n_nod
                 }
             }
         }
-        const int live = min(a.N - w * NB, NB);
-        if (live < NB) {   // ragged block: the pending nodes fold from the lowest level up
-            bool have = false;
+        if (NS == 1) {
+            const int live = min(a.N - w * NB, NB);
+            if (live < NB) {   // ragged block: the pending nodes fold from the lowest level up
+                bool have = false;
 #pragma unroll
-            for (int l = 0; l < F; ++l) {
-                if ((live >> l) & 1) {
-                    if (!have) { node = stk[l]; have = true; }
-                    else { Tile m; combine(node, stk[l], m); node = m; }
+                for (int l = 0; l < F; ++l) {
+                    if ((live >> l) & 1) {
+                        if (!have) { node = stk[l]; have = true; }
+                        else { Tile m; combine(node, stk[l], m); node = m; }
+                    }
                 }
             }
         }
-        cplx<R>* out = a.out + (size_t)w * dd;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = acc_row<R>(0, 0, v), col = acc_col(0, 0);
-            if (row < d && col < d) out[row * d + col] = mk<R>(node.re[0][0][v], node.im[0][0][v]);
+            const int q = row / HB, i = row % HB, k = col % HB, nd = w + q * groups;
+            if (col / HB == q && i < d && k < d && nd < n_nodes)
+                a.out[(size_t)nd * dd + i * d + k] = mk<R>(node.re[0][0][v], node.im[0][0][v]);
         }
     }
 }
@@ -145,16 +179,24 @@ hipError_t launch_small_fold(const ChebArgs<R>& a, hipStream_t st) {
         return hipErrorInvalidValue;
     const int F = small_fold_levels(a.N);
     const int n_nodes = (a.N + (1 << F) - 1) >> F;
-    const int grid = n_nodes < 4096 ? n_nodes : 4096;
+    // slices per 16 x 16 tile: 4 for d <= 4, 2 for d <= 8
+    const int NS = a.d <= 4 ? 4 : (a.d <= 8 ? 2 : 1);
+    const int groups = (n_nodes + NS - 1) / NS;
+    const int grid = groups < 4096 ? groups : 4096;
     const size_t lds = (size_t)(1 + a.ncoef) * a.d * a.d * sizeof(cplx<R>);
+#define PA_SMALL(FF)                                                                                              \
+    if (NS == 4) hipLaunchKernelGGL((small_fold_kernel<R, FF, 4>), dim3(grid), dim3(64), lds, st, a, n_nodes);    \
+    else if (NS == 2) hipLaunchKernelGGL((small_fold_kernel<R, FF, 2>), dim3(grid), dim3(64), lds, st, a, n_nodes); \
+    else hipLaunchKernelGGL((small_fold_kernel<R, FF, 1>), dim3(grid), dim3(64), lds, st, a, n_nodes);
     switch (F) {
-        case 5: hipLaunchKernelGGL((small_fold_kernel<R, 5>), dim3(grid), dim3(64), lds, st, a, n_nodes); break;
-        case 4: hipLaunchKernelGGL((small_fold_kernel<R, 4>), dim3(grid), dim3(64), lds, st, a, n_nodes); break;
-        case 3: hipLaunchKernelGGL((small_fold_kernel<R, 3>), dim3(grid), dim3(64), lds, st, a, n_nodes); break;
-        case 2: hipLaunchKernelGGL((small_fold_kernel<R, 2>), dim3(grid), dim3(64), lds, st, a, n_nodes); break;
-        case 1: hipLaunchKernelGGL((small_fold_kernel<R, 1>), dim3(grid), dim3(64), lds, st, a, n_nodes); break;
-        default: hipLaunchKernelGGL((small_fold_kernel<R, 0>), dim3(grid), dim3(64), lds, st, a, n_nodes); break;
+        case 5: PA_SMALL(5) break;
+        case 4: PA_SMALL(4) break;
+        case 3: PA_SMALL(3) break;
+        case 2: PA_SMALL(2) break;
+        case 1: PA_SMALL(1) break;
+        default: PA_SMALL(0) break;
     }
+#undef PA_SMALL
     return hipGetLastError();
 }
 template hipError_t launch_small_fold<float>(const ChebArgs<float>&, hipStream_t);

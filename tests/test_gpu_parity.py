@@ -50,6 +50,12 @@ def case(seed, d, amps, pts):
     ("fp64", 5, 2, 131075, "none"),          # five folded levels, ragged
     ("fp64", 16, 3, 2 * 16389 + 1, "simpson"),
     ("fp64", 3, 2, 20001, "midpoint"),
+    # several slices per tile (d <= 8: two, d <= 4: four) with a partly empty last group
+    ("fp32", 4, 1, 4099, "none"),
+    ("fp64", 2, 1, 7, "none"),
+    ("fp32", 4, 2, 3 * 8192 + 5, "none"),
+    ("fp64", 7, 3, 2 * 8192 + 3, "simpson"),
+    ("fp32", 8, 2, 1 << 16, "none"),
 ])
 def test_small_dim_folded_tree_levels(prec, d, amps, pts, mode):
     """Same pairing and odd carry as the tree kernels: the folded nodes feed the tree above."""
