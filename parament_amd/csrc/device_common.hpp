@@ -121,16 +121,17 @@ template <class R, int TM, int TN> struct CTile {
 };
 
 // C(tile at row0, col0) = A * B over K = KD (ZERO), or C += A * B (!ZERO), operands in LDS
-// planes with row strides SA, SB.
+// planes with row strides SA, SB.  kd (a multiple of 4, at most KD) shortens the inner
+// dimension at run time, for a last chunk that is zero past kd.
 template <class R, int TM, int TN, int KD, int SA, int SB, bool ZERO = true>
 __device__ __forceinline__ void wave_cgemm(const R* __restrict__ Ar, const R* __restrict__ Ai,
                                            const R* __restrict__ Br, const R* __restrict__ Bi,
-                                           int row0, int col0, CTile<R, TM, TN>& c) {
+                                           int row0, int col0, CTile<R, TM, TN>& c, int kd = KD) {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, q = lane >> 4;
     if (ZERO) c.zero();
 #pragma unroll 4
-    for (int k0 = 0; k0 < KD; k0 += 4) {
+    for (int k0 = 0; k0 < kd; k0 += 4) {
         R ar[TM], ai[TM], an[TM], br[TN], bi[TN];
 #pragma unroll
         for (int m = 0; m < TM; ++m) {

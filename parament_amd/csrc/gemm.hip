@@ -106,7 +106,8 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
         }
         __syncthreads();
         if (k0 + KT < d) fetch(k0 + KT);
-        wave_cgemm<R, TW, TW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc);
+        // a last chunk runs only the k-steps that reach d (the rest of it is zero)
+        wave_cgemm<R, TW, TW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc, min(KT, (d - k0 + 3) & ~3));
         __syncthreads();
     }
     // epilogue in FP64
