@@ -247,6 +247,40 @@ hipError_t zgemm_native(const cplx<R>* A, long sa, const cplx<R>* B, long sb, co
                  beta, diag, d, batch, st, add_ab, tile);
 }
 inline double2 c2(double re, double im = 0.0) { return pa::mk<double>(re, im); }
+inline double2 cmul2(double2 x, double2 y) { return c2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x); }
+inline double2 cscale(double f, double2 x) { return c2(f * x.x, f * x.y); }
+
+// Slices E_j = p(X_j) - I of a batch of cnt slices by the Clenshaw recurrence on batched GEMMs
+// (parament.cpp:569-652): b_k = s X b_{k+1} - b_{k+2} + a_k I, E = s X b_1 - 2 b_2 + (a_0 - 1) I,
+// every step's diagonal and scaling in the GEMM epilogue.  The start values b_M = a_M I and
+// b_{M-1} = s a_M X + a_{M-1} I are affine in X, so the first two steps take them through X as
+// C_in instead of materialising them:
+//   b_{M-2} = s^2 a_M X X + s a_{M-1} X + (a_{M-2} - a_M) I,
+//   b_{M-3} = s X b_{M-2} + beta s a_M X + (a_{M-3} + beta a_{M-1}) I   (beta = -2 when M = 3).
+// a[k] = a_k (a[0] = a_0 - 1); M >= 3.  w0, w1: cnt slices each.  The result goes to out, or,
+// when out is null, stays in w0 or w1 (*result says which).
+template <class R>
+hipError_t clenshaw_batched(const cplx<R>* X, int d, int cnt, int M, double s, const double2* a, cplx<R>* w0,
+                            cplx<R>* w1, cplx<R>* out, cplx<R>** result, hipStream_t st, int tile = 0) {
+    const int k3 = M - 3;
+    const double beta3 = k3 == 0 ? -2.0 : -1.0;
+    hipError_t r = zgemm_native<R>(X, 1, X, 1, X, 1, w0, 1, cscale(s * s, a[M]), cscale(s, a[M - 1]),
+                                   c2(a[M - 2].x - a[M].x, a[M - 2].y - a[M].y), d, cnt, st, false, tile);
+    cplx<R>* dst = (k3 == 0 && out) ? out : w1;
+    if (r == hipSuccess)
+        r = zgemm_native<R>(X, 1, w0, 1, X, 1, dst, 1, c2(s), cscale(beta3 * s, a[M]),
+                            c2(a[k3].x + beta3 * a[M - 1].x, a[k3].y + beta3 * a[M - 1].y), d, cnt, st, false, tile);
+    cplx<R>* b1 = dst;
+    cplx<R>* b2 = w0;
+    for (int k = M - 4; r == hipSuccess && k >= 0; --k) {
+        cplx<R>* o = (k == 0 && out) ? out : b2;
+        r = zgemm_native<R>(X, 1, b1, 1, b2, 1, o, 1, c2(s), c2(k == 0 ? -2.0 : -1.0), a[k], d, cnt, st, false, tile);
+        b2 = b1;
+        b1 = o;
+    }
+    if (result) *result = b1;
+    return r;
+}
 
 template <class R> Parament_ErrorCode bind_device(Context<R>* h) {
     if (h->bound) {
@@ -568,7 +602,6 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = (int)d; ca.ctl = ctl; ca.N = N; ca.M = M;
     ca.s = (R)(2.0 / h->hnorm);  // the reference casts s and a_k to the working precision (parament.cpp:583-584, 406)
     for (int k = 0; k <= M; ++k) { ca.are[k] = (R)are[k]; ca.aim[k] = (R)aim[k]; }
-    const R a_m_top = (R)are[M];
     ca.are[0] = (R)a0m1;
 
     if (fused) {
@@ -613,28 +646,20 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
                                       reinterpret_cast<double2*>(out_dev));
     }
     C* X = h->d_gen[0];
-    C* b1 = h->d_gen[1];
-    C* b2 = h->d_gen[2];
-    const C aM = pa::mk<R>(a_m_top, ca.aim[M]);
-    const C saM = pa::cmul(pa::mk<R>(ca.s, 0), aM);
     mark(h, 0);
     if (pa::launch_assemble<R>(ca, X, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (pa::launch_fill<R>(nullptr, pa::mk<R>(0, 0), aM, (int)d, N, b2, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
-    if (pa::launch_fill<R>(X, saM, pa::mk<R>(ca.are[M - 1], ca.aim[M - 1]), (int)d, N, b1, st) != hipSuccess)
+    // the slices as E = U - I (a_0 - 1 on the diagonal of the last step)
+    double2 ak[pa::kMaxCheb];
+    for (int k = 0; k <= M; ++k) ak[k] = c2(k == 0 ? a0m1 : (double)ca.are[k], (double)ca.aim[k]);
+    C* leaves = nullptr;
+    if (clenshaw_batched<R>(X, (int)d, N, M, (double)ca.s, ak, h->d_gen[1], h->d_gen[2], nullptr, &leaves, st) !=
+        hipSuccess)
         return PARAMENT_STATUS_CUBLAS_FAILED;
-    for (int k = M - 2; k >= 0; --k) {
-        // b2 <- s X b1 + beta b2 + a_k I (k = 0: a_0 - 1, the slice as E = U - I), then swap roles
-        const double beta = (k == 0) ? -2.0 : -1.0;
-        const double akr = (k == 0) ? a0m1 : (double)ca.are[k], aki = ca.aim[k];
-        if (zgemm_native<R>(X, 1, b1, 1, b2, 1, b2, 1, c2((double)ca.s), c2(beta), c2(akr, aki), (int)d, N, st) != hipSuccess)
-            return PARAMENT_STATUS_CUBLAS_FAILED;
-        std::swap(b1, b2);
-    }
     mark(h, 1);
-    // b1 holds E_j = U_j - I for the N slices; reduce level by level (parament.cpp:675-714)
-    // on deviations from identity: E' = A + B + A B.
-    C* rd = b1;
-    C* wr = b2;
+    // E_j = U_j - I for the N slices; reduce level by level (parament.cpp:675-714) on deviations
+    // from identity: E' = A + B + A B.
+    C* rd = leaves;
+    C* wr = leaves == h->d_gen[1] ? h->d_gen[2] : h->d_gen[1];
     int r = N;
     while (r > 1) {
         const int half = r / 2;
@@ -795,10 +820,9 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
     const int d = ca.d, M = ca.M, N = p.N;
     const size_t dd = (size_t)d * d;
     const C zero = pa::mk<R>(0, 0), one = pa::mk<R>(1, 0);
-    const C aM = pa::mk<R>(ca.are[M], ca.aim[M]);
-    const C saM = pa::cmul(pa::mk<R>(ca.s, 0), aM);
-    const C aM1 = pa::mk<R>(ca.are[M - 1], ca.aim[M - 1]);
     const double s = (double)ca.s;
+    double2 a[pa::kMaxCheb];
+    for (int k = 0; k <= M; ++k) a[k] = c2((double)ca.are[k], (double)ca.aim[k]);
     C* X = b.work;
     C* w[4];
     for (int i = 0; i < 4; ++i) w[i] = b.work + (size_t)(1 + i) * p.chunk * dd;
@@ -807,17 +831,8 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
     for (int j0 = 0; r == hipSuccess && j0 < N; j0 += p.chunk) {
         const int cnt = std::min(p.chunk, N - j0);
         ca.j0 = j0; ca.N = cnt;
-        C* b1 = w[0];
-        C* b2 = w[1];
         r = pa::launch_assemble<R>(ca, X, st);
-        if (r == hipSuccess) r = pa::launch_fill<R>(nullptr, zero, aM, d, cnt, b2, st);
-        if (r == hipSuccess) r = pa::launch_fill<R>(X, saM, aM1, d, cnt, b1, st);
-        for (int k = M - 2; r == hipSuccess && k >= 0; --k) {
-            C* out = k == 0 ? b.E + (size_t)j0 * dd : b2;
-            r = zgemm_native<R>(X, 1, b1, 1, b2, 1, out, 1, c2(s), c2(k == 0 ? -2.0 : -1.0), c2(ca.are[k], ca.aim[k]),
-                                d, cnt, st);
-            std::swap(b1, b2);
-        }
+        if (r == hipSuccess) r = clenshaw_batched<R>(X, d, cnt, M, s, a, w[0], w[1], b.E + (size_t)j0 * dd, nullptr, st);
     }
     // up-sweep
     const C* rd = b.E;
@@ -842,30 +857,42 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         if (r == hipSuccess && (n & 1))
             r = pa::launch_fill<R>(Wp + (size_t)(n / 2) * dd, one, zero, d, 1, Wc + (size_t)(n - 1) * dd, st);
     }
-    // per-slice derivatives and traces
+    // per-slice derivatives and traces; the start values b_M, b_{M-1}, db_M = 0 and
+    // db_{M-1} = s a_M W are affine in X and W and enter the first two steps as C_in:
+    //   db_{M-2} = s^2 a_M (X W + W X) + s a_{M-1} W,
+    //   db_{M-3} = s (X db_{M-2} + W b_{M-2}) + beta s a_M W   (beta = -2 when M = 3)
+    const int k3 = M - 3;
+    const double beta3 = k3 == 0 ? -2.0 : -1.0;
     for (int j0 = 0; r == hipSuccess && j0 < N; j0 += p.chunk) {
         const int cnt = std::min(p.chunk, N - j0);
         ca.j0 = j0; ca.N = cnt;
         const C* Wj = b.W + (p.woff[0] + (size_t)j0) * dd;
-        C* b1 = w[0];
-        C* b2 = w[1];
-        C* d1 = w[2];
-        C* d2 = w[3];
-        r = pa::launch_assemble<R>(ca, X, st);
-        if (r == hipSuccess) r = pa::launch_fill<R>(nullptr, zero, aM, d, cnt, b2, st);
-        if (r == hipSuccess) r = pa::launch_fill<R>(X, saM, aM1, d, cnt, b1, st);
-        if (r == hipSuccess) r = pa::launch_fill<R>(nullptr, zero, zero, d, cnt, d2, st);
-        if (r == hipSuccess) r = pa::launch_fill<R>(Wj, saM, zero, d, cnt, d1, st);
-        for (int k = M - 2; r == hipSuccess && k >= 0; --k) {
-            r = zgemm_native<R>(X, 1, d1, 1, d2, 1, d2, 1, c2(s), c2(k == 0 ? -2.0 : -1.0), c2(0), d, cnt, st);
-            if (r == hipSuccess) r = zgemm_native<R>(Wj, 1, b1, 1, d2, 1, d2, 1, c2(s), c2(1), c2(0), d, cnt, st);
-            if (k == 0) break;
-            if (r == hipSuccess)
-                r = zgemm_native<R>(X, 1, b1, 1, b2, 1, b2, 1, c2(s), c2(-1.0), c2(ca.are[k], ca.aim[k]), d, cnt, st);
+        if (p.chunk < N) r = pa::launch_assemble<R>(ca, X, st);   // one chunk: X is still in place
+        auto gemm = [&](const C* A, const C* B, const C* Cin, C* out, double2 alpha, double2 beta, double2 diag) {
+            if (r == hipSuccess) r = zgemm_native<R>(A, 1, B, 1, Cin, 1, out, 1, alpha, beta, diag, d, cnt, st);
+        };
+        gemm(X, X, X, w[0], cscale(s * s, a[M]), cscale(s, a[M - 1]), c2(a[M - 2].x - a[M].x, a[M - 2].y - a[M].y));
+        gemm(X, Wj, Wj, w[2], cscale(s * s, a[M]), cscale(s, a[M - 1]), c2(0));
+        gemm(Wj, X, w[2], w[2], cscale(s * s, a[M]), c2(1), c2(0));
+        gemm(X, w[2], Wj, w[3], c2(s), cscale(beta3 * s, a[M]), c2(0));
+        gemm(Wj, w[0], w[3], w[3], c2(s), c2(1), c2(0));
+        if (k3 > 0)
+            gemm(X, w[0], X, w[1], c2(s), cscale(beta3 * s, a[M]),
+                 c2(a[k3].x + beta3 * a[M - 1].x, a[k3].y + beta3 * a[M - 1].y));
+        // b1 = b_{k+1}, b2 = b_{k+2}, d1 = db_{k+1}, d2 = db_{k+2}
+        C* b1 = w[1];
+        C* b2 = w[0];
+        C* d1 = w[3];
+        C* d2 = w[2];
+        for (int k = M - 4; k >= 0; --k) {
+            gemm(X, d1, d2, d2, c2(s), c2(k == 0 ? -2.0 : -1.0), c2(0));
+            gemm(Wj, b1, d2, d2, c2(s), c2(1), c2(0));
+            if (k > 0) gemm(X, b1, b2, b2, c2(s), c2(-1.0), a[k]);
             std::swap(b1, b2);
             std::swap(d1, d2);
         }
-        if (r == hipSuccess) r = pa::launch_grad_trace<R>(d2, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
+        // Q = Dp(X)[W] is in d1 (M = 3: the second step's w[3])
+        if (r == hipSuccess) r = pa::launch_grad_trace<R>(d1, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
     }
     if (r == hipSuccess)
         r = pa::launch_grad_finish<R>(b.dg, b.gsum, N, ca.ctl.pts, ca.ctl.amps_in, ca.ctl.mode, d, grad, fid, st);
@@ -1388,20 +1415,14 @@ hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st) {
     ca.ctl.mode = e->quad; ca.ctl.magnus = 0; ca.ctl.tau = (float)x.tau;
     for (int k = 0; k <= M; ++k) { ca.are[k] = (float)x.are[k]; ca.aim[k] = (float)x.aim[k]; }
     ca.out = nullptr;
-    const float2 aM = make_float2((float)x.are[M], (float)x.aim[M]);
-    const float2 saM = make_float2(ca.s * aM.x, ca.s * aM.y);
+    double2 ak[pa::kMaxCheb];
+    for (int k = 0; k <= M; ++k) ak[k] = c2(k == 0 ? x.are[0] - 1.0 : (double)ca.are[k], (double)ca.aim[k]);
+    float2* leaves = nullptr;
     hipError_t r = pa::launch_assemble<float>(ca, X, st);
-    if (r == hipSuccess) r = pa::launch_fill<float>(nullptr, make_float2(0, 0), aM, d, N, b2, st);
-    if (r == hipSuccess) r = pa::launch_fill<float>(X, saM, make_float2(ca.are[M - 1], ca.aim[M - 1]), d, N, b1, st);
-    for (int k = M - 2; r == hipSuccess && k >= 0; --k) {
-        const double akr = (k == 0) ? x.are[0] - 1.0 : (double)ca.are[k];
-        r = zgemm_native<float>(X, 1, b1, 1, b2, 1, b2, 1, c2((double)ca.s), c2(k == 0 ? -2.0 : -1.0), c2(akr, ca.aim[k]),
-                                d, N, st, false, x.tile);
-        std::swap(b1, b2);
-    }
-    // b1 holds the leaves E_j; b2 is free for the other half of the ping-pong
-    float2* rd = b1;
-    float2* wr = b2;
+    if (r == hipSuccess) r = clenshaw_batched<float>(X, d, N, M, (double)ca.s, ak, b1, b2, nullptr, &leaves, st, x.tile);
+    // the leaves E_j in one buffer, the other free for the other half of the ping-pong
+    float2* rd = leaves;
+    float2* wr = leaves == b1 ? b2 : b1;
     int n = N;
     while (r == hipSuccess && n > 1) {
         const int half = n / 2;
