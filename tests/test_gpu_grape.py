@@ -102,6 +102,27 @@ def test_stepwise_gradient_few_slices(pts, mode):
     assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
 
 
+@pytest.mark.parametrize("cycles", [3, 5])
+def test_stepwise_gradient_manual_cycles(cycles):
+    """A manual Chebyshev order (M = 3: the derivative's first two steps are all of it) against
+    central differences of the library's own FP64 propagator with the same order."""
+    import parament_amd as parament
+    h0, hk, c, t = _problem(4400 + cycles, 40, 2, 15)
+    dt, h = 0.01, 1e-6
+    with parament.Parament("fp64") as p:
+        p.set_hamiltonian(h0, *hk)
+        p.set_iteration_cycles(cycles)
+        f, grad = p.gate_fidelity_gradient(dt, *c, target=t)
+        assert p.iteration_cycles == cycles
+        assert abs(f - gpu_fidelity(p, c, dt, t)) < 1e-13
+        for k, j in [(0, 0), (1, 7), (0, 14), (1, 3)]:
+            cp, cm = c.copy(), c.copy()
+            cp[k, j] += h
+            cm[k, j] -= h
+            fd = (gpu_fidelity(p, cp, dt, t) - gpu_fidelity(p, cm, dt, t)) / (2 * h)
+            assert abs(fd - grad[k, j]) < 1e-8 * max(1.0, np.abs(grad).max()), (k, j, fd, grad[k, j])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_stepwise_gradient_chunks_agree(prec):
     """Per-slice arrays processed 7 slices at a time give the whole-batch result."""

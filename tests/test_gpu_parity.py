@@ -275,6 +275,17 @@ def test_stepwise_large_dims(prec, d, n):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("cycles", [3, 5, 7])
+def test_stepwise_manual_cycles(prec, cycles):
+    """The batched recurrence takes its start values b_M, b_(M-1) through C_in in its first two
+    steps; with M = 3 the second of them is already the last."""
+    h0, hks, c = case(1450 + cycles, 70, 2, 11)
+    u, m = run_gpu(prec, h0, hks, c, 0.01, "none", cycles=cycles)
+    assert m == cycles
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 0.01, "none", cycles=cycles)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("mode,magnus", MODES)
 def test_stepwise_forced_small_dim(prec, mode, magnus):
     h0, hks, c = case(1400, 24, 2, 33)
