@@ -1603,6 +1603,9 @@ EnsGradLayout ens_grad_layout(const EnsInst& x) {
 
 Parament_ErrorCode ens_grad_setup(Ensemble* e) {
     if (e->grad_ready) return PARAMENT_STATUS_SUCCESS;
+    // a set-up that failed part way may have left buffers behind
+    hipFree(e->d_opsT); e->d_opsT = nullptr;
+    for (auto& w : e->d_gws) { hipFree(w); w = nullptr; }
     size_t n_opsT = 0;
     for (EnsInst& x : e->in) { x.opsT_off = n_opsT; n_opsT += (size_t)x.amps * x.d * x.d; }
     if (hipMalloc(&e->d_opsT, n_opsT * sizeof(float2)) != hipSuccess) {
