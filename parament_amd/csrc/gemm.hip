@@ -137,6 +137,82 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
             }
 }
 
+// Whole-product FP32 kernel for dims 33-128 (complex64 operands): work-group b forms the full C_b with NW = ceil(d/16)
+// waves, wave w the 16-row band [16 w, 16 w + 16) over all 16 NW columns, so the padding is to a
+// multiple of 16 and every operand element is fetched once per product.  K in chunks of 32,
+// the next one fetched into registers while the matrix cores work on the current one.
+template <int NW, class TA, class TB, class TC, class TO>
+__global__ __launch_bounds__(64 * NW) void zgemm_row_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
+    typedef float R;
+    constexpr int DP = 16 * NW, NT = 64 * NW;
+    constexpr int SA = KT + Pad<R>::A, SB = DP + Pad<R>::B;
+    constexpr int NE = DP * KT / NT;   // 8 elements of A and of B per thread and chunk
+    __shared__ R Ar[DP * SA], Ai[DP * SA], Br[KT * SB], Bi[KT * SB];
+    const int d = g.d;
+    const long b = work_item(g.batch);
+    if (b < 0) return;
+    const long dd = (long)d * d;
+    const TA* A = io.A + b * g.sa * dd;
+    const TB* B = io.B + b * g.sb * dd;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int wr = 16 * wave;
+    TA ra[NE];
+    TB rb[NE];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + NT * u;
+            const int gi = e / KT, gk = k0 + e % KT;
+            ra[u].x = 0; ra[u].y = 0;
+            if (gi < d && gk < d) ra[u] = A[(long)gi * d + gk];
+        }
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + NT * u;
+            const int gk = k0 + e / DP, gj = e % DP;
+            rb[u].x = 0; rb[u].y = 0;
+            if (gk < d && gj < d) rb[u] = B[(long)gk * d + gj];
+        }
+    };
+    CTile<R, 1, NW> acc;
+    acc.zero();
+    fetch(0);
+    for (int k0 = 0; k0 < d; k0 += KT) {
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + NT * u;
+            Ar[(e / KT) * SA + e % KT] = (R)ra[u].x; Ai[(e / KT) * SA + e % KT] = (R)ra[u].y;
+            Br[(e / DP) * SB + e % DP] = (R)rb[u].x; Bi[(e / DP) * SB + e % DP] = (R)rb[u].y;
+        }
+        __syncthreads();
+        if (k0 + KT < d) fetch(k0 + KT);
+        wave_cgemm<R, 1, NW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, 0, acc, min(KT, (d - k0 + 3) & ~3));
+        __syncthreads();
+    }
+    const long cb = b * g.so * dd, ib = b * g.sc * dd;
+#pragma unroll
+    for (int n = 0; n < NW; ++n)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int gi = acc_row<R>(wr, 0, v), gj = acc_col(0, n);
+            if (gi >= d || gj >= d) continue;
+            const long o = (long)gi * d + gj;
+            const double xr = acc.re[0][n][v], xi = acc.im[0][n][v];
+            double2 y = mk<double>(g.alpha_re * xr - g.alpha_im * xi, g.alpha_re * xi + g.alpha_im * xr);
+            if (io.Cin) {
+                const double2 c = ld(io.Cin, ib + o);
+                y.x += g.beta_re * c.x - g.beta_im * c.y;
+                y.y += g.beta_re * c.y + g.beta_im * c.x;
+            }
+            if (g.add_ab) {
+                const double2 a = ld(A, o), bb = ld(B, o);
+                y.x += a.x + bb.x; y.y += a.y + bb.y;
+            }
+            if (gi == gj) { y.x += g.diag_re; y.y += g.diag_im; }
+            st(io.C, cb + o, y);
+        }
+}
+
 // bf16 split products: A chunk planes [slice][re/im][64][32 + 8] row-major, B chunk planes
 // transposed [slice][re/im][64 cols][32 + 8].  Operand fragments of the 16x16x32 instruction:
 // lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7]; accumulator element v is
@@ -289,10 +365,35 @@ hipError_t launch_tiled(const GemmArgs& g, int mode, const EpiIO<TA, TB, TC, TO>
 }
 
 template <class TA, class TB, class TC, class TO>
+hipError_t launch_row(const GemmArgs& g, const EpiIO<TA, TB, TC, TO>& io, hipStream_t st) {
+    const int nw = (g.d + 15) / 16;
+    if (g.batch <= 0) return hipSuccess;
+    const dim3 grid((unsigned)g.batch);
+    switch (nw) {
+        case 3: hipLaunchKernelGGL((zgemm_row_kernel<3, TA, TB, TC, TO>), grid, dim3(64 * 3), 0, st, g, io); break;
+        case 4: hipLaunchKernelGGL((zgemm_row_kernel<4, TA, TB, TC, TO>), grid, dim3(64 * 4), 0, st, g, io); break;
+        case 5: hipLaunchKernelGGL((zgemm_row_kernel<5, TA, TB, TC, TO>), grid, dim3(64 * 5), 0, st, g, io); break;
+        case 6: hipLaunchKernelGGL((zgemm_row_kernel<6, TA, TB, TC, TO>), grid, dim3(64 * 6), 0, st, g, io); break;
+        case 7: hipLaunchKernelGGL((zgemm_row_kernel<7, TA, TB, TC, TO>), grid, dim3(64 * 7), 0, st, g, io); break;
+        case 8: hipLaunchKernelGGL((zgemm_row_kernel<8, TA, TB, TC, TO>), grid, dim3(64 * 8), 0, st, g, io); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class TA, class TB, class TC, class TO>
 hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
     EpiIO<TA, TB, TC, TO> io;
     io.A = static_cast<const TA*>(g.A); io.B = static_cast<const TB*>(g.B);
     io.Cin = static_cast<const TC*>(g.Cin); io.C = static_cast<TO*>(g.C);
+    // complex64 FP32 products at dims 33-128: the whole-product kernel when asked for (tile 16),
+    // and by default where it measured faster than the 32/64 tiles (dims 33-48: 27-28%,
+    // 97-112: 5-6%; at 49-96 and 113-128 it was level or slower)
+    if constexpr (sizeof(TA) == 8 && sizeof(TB) == 8 && sizeof(TO) == 8) {
+        const bool row_default = (g.d > 32 && g.d <= 48) || (g.d > 96 && g.d <= 112);
+        if (mode == GEMM_F32 && g.d > 32 && g.d <= 128 && (g.tile == 16 || (g.tile == 0 && row_default)))
+            return launch_row(g, io, st);
+    }
     return gemm_tile(g) == 32 ? launch_tiled<32>(g, mode, io, st) : launch_tiled<64>(g, mode, io, st);
 }
 

@@ -1168,7 +1168,7 @@ struct EnsInst {
     unsigned d = 0, amps = 0, pts = 0;
     double dt = 0, tau = 0, hnorm = 0;
     int N = 0, M = 0, cls = -1;                // cls: class index, -1 for the batched-GEMM path
-    int tile = 0;                               // batched-GEMM output tile (tuned; 0 = by volume)
+    int tile = 0;                               // batched-GEMM tile (tuned: 64, 32 or 16 = whole product; 0 = automatic)
     size_t ops_off = 0, c_off = 0, out_off = 0; // element offsets in the packed arrays
     size_t work_off = 0;                        // element offset of its working memory
     size_t opsT_off = 0;                        // element offset of its transposed controls (gradient)
@@ -1518,7 +1518,7 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
     for (int i : e->large) {   // output tile of each large instance's batched GEMMs
         float best = 1e30f;
         int best_tile = 64;
-        for (int tile : {64, 32}) {
+        for (int tile : {64, 32, 16}) {   // 16: the whole-product kernel
             e->in[i].tile = tile;
             const float t = ens_time(e, [&] { return ens_enqueue_large(e, i, e->stream); }, 1);
             if (t < best) { best = t; best_tile = tile; }

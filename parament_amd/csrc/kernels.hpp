@@ -89,7 +89,7 @@ struct GemmArgs {
     double alpha_re, alpha_im, beta_re, beta_im, diag_re, diag_im;
     int d, batch, add_ab;
     int a_c128, b_c128, out_c128;
-    int tile = 0;          // output tile 32 or 64; 0 chooses by padded volume
+    int tile = 0;          // output tile 32 or 64, 16 = whole product per work-group (FP32, dims 33-128); 0 = automatic
 };
 hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st);
 

@@ -275,6 +275,16 @@ def test_stepwise_large_dims(prec, d, n):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("d,n", [(33, 9), (40, 4), (48, 17), (105, 7), (112, 3)])
+def test_stepwise_whole_product_dims(prec, d, n):
+    """Complex64 products at dims 33-48 and 97-112 run the whole-product GEMM kernel (one
+    work-group per slice, 16-row bands); complex128 the tiled kernel."""
+    h0, hks, c = case(1600 + d, d, 2, n)
+    u, m = run_gpu(prec, h0, hks, c, 0.02, "none", path="stepwise")
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 0.02, "none")) < TOL[prec], m
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("cycles", [3, 5, 7])
 def test_stepwise_manual_cycles(prec, cycles):
     """The batched recurrence takes its start values b_M, b_(M-1) through C_in in its first two
