@@ -182,3 +182,32 @@ def test_ensemble_gradient_errors():
             e.gate_fidelity_gradient([np.zeros((1, 10)) + 1j], [np.eye(4)])
         with pytest.raises(ValueError):
             e.gate_fidelity_gradient([np.zeros((1, 10))], [np.eye(3)])
+
+
+def test_full_size_ensemble512_forward_and_gradient():
+    """The ensemble workload at full size (512 systems, dims 4-128, 1-4 controls, 256-8192 steps;
+    one tuned, captured graph): every propagator unitary to complex64 accuracy, and the smallest,
+    the largest and two mid-sized systems equal a complex64 context's propagator and gradient."""
+    import parament_amd as parament
+    from parament_amd import workloads
+    from parament_amd.ensemble import Ensemble
+    dims, amps, pts, dt, hs, cs = workloads.ensemble512()
+    rng = np.random.default_rng(21)
+    ts = [workloads.haar_unitary(rng, int(d)) for d in dims]
+    with Ensemble(dims, amps, pts, dt) as e:
+        e.set_hamiltonians(hs)
+        us = e.equiprop(cs)
+        fid, grads = e.gate_fidelity_gradient(cs, ts)
+    for i, u in enumerate(us):
+        d, n = int(dims[i]), int(pts[i])
+        # complex64 rounding of n slices adding up as a random walk, with a wide margin
+        assert np.linalg.norm(u.conj().T @ u - np.eye(d)) < 1e-6 * d * np.sqrt(n), (i, d, n)
+    order = np.argsort(dims, kind="stable")
+    for i in (order[0], order[-1], order[len(order) // 3], order[2 * len(order) // 3]):
+        with parament.Parament("fp32") as p:
+            p.set_hamiltonian(hs[i][0], *hs[i][1:])
+            u = p.equiprop(dt, *cs[i])
+            f, g = p.gate_fidelity_gradient(dt, *cs[i], target=ts[i])
+        assert rel_err(us[i], u) < 1e-5, (i, int(dims[i]))
+        assert abs(fid[i] - f) <= 1e-5 * max(f, 1e-3), (i, int(dims[i]))
+        assert np.abs(grads[i] - g).max() <= 1e-4 * np.abs(g).max(), (i, int(dims[i]))

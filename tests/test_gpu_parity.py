@@ -262,6 +262,25 @@ def test_full_size_heisenberg6_mixed_within_gate():
     assert np.linalg.norm(um.conj().T @ um - eye) < 2 * 1e-5 * np.sqrt(64)
 
 
+def test_full_size_ising8_split_products():
+    """The precision-variant workload at full size (dim-256 transverse-field Ising chain, 8 local
+    sigma_x controls, 16,384 steps, no quadrature) on the batched GEMMs: bf16 three-way split
+    products and FP32 products, with FP64 first-order terms and upper tree levels, stay within
+    1e-6 of the FP64 path; the FP64 propagator is unitary to 1e-10."""
+    import parament_amd as parament
+    from parament_amd import workloads
+    h0, hk, c, dt, quad = workloads.ising8()
+    with parament.Parament("fp64") as p:
+        p.set_hamiltonian(h0, *hk, quadrature_mode=quad)
+        u64 = p.equiprop(dt, *c)
+        errs = {}
+        for mode in ("mixed", "mixed_fp32"):
+            p.set_precision_mode(mode)
+            errs[mode] = rel_err(p.equiprop(dt, *c), u64)
+    assert np.linalg.norm(u64.conj().T @ u64 - np.eye(256)) < 1e-10
+    assert errs["mixed"] < 1e-6 and errs["mixed_fp32"] < 1e-6, errs
+
+
 # ---- dims above the on-chip kernels: batched matrix-core GEMMs ----
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
