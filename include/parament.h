@@ -184,7 +184,8 @@ LIBSPEC Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64 *ha
  * (carr layout, real controls: the imaginary parts must be zero).  grad receives amps x pts
  * values dF/dc[k*pts + j] (quadrature chain rule included), fidelity (optional) receives F.
  * For callers optimising controls (GRAPE loops, the PyTorch autograd wrapper).  Any dim (dims
- * above 32 run on batched GEMMs and keep the whole product tree), no Magnus.  The Device variants take device pointers and are asynchronous on the
+ * above 32 run on batched GEMMs and keep the whole product tree), every quadrature mode
+ * including Magnus.  The Device variants take device pointers and are asynchronous on the
  * context's stream; they read only the real parts of carr_dev. */
 LIBSPEC Parament_ErrorCode Parament_gateFidelityGradient(Parament_Context_f32 *handle, const cuComplex *carr,
     double dt, unsigned int pts, unsigned int amps, const cuComplex *target, float *fidelity, float *grad);

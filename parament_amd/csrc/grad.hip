@@ -526,6 +526,65 @@ hipError_t launch_grad_trace(const cplx<R>* Q, int n, int d, const cplx<R>* opsT
     return hipGetLastError();
 }
 
+// Magnus chain rule (control_expansion.cu:27-99 differentiated): with G_t(j) = dg[t][j] the
+// derivative of g with respect to coefficient t of slice j, g is holomorphic in the
+// coefficients, so dg/dc_kp = sum_(t,j) G_t(j) dcoef_t(j)/dc_kp, and dF/dc_kp = 2 Re(conj(g) dg/dc_kp) / d^2.
+// Sample p of control k enters slice j = p/2 (p = 2j, even), j = (p-1)/2 (odd) and j = p/2 - 1
+// (p = 2j + 2, even):
+//   Simpson mean t = k:        1/6, 4/6, 1/6;
+//   gamma_k, t = A + k:        -i tau/12 at p = 2j, +i tau/12 at p = 2j + 2;
+//   delta_ab, t = 2A + pair:   i tau/12 (c_a[2j] c_b[2j+2] - c_a[2j+2] c_b[2j]) differentiated:
+//                              for k = a: +c_b[2j+2] at p = 2j, -c_b[2j] at p = 2j + 2;
+//                              for k = b: -c_a[2j+2] at p = 2j, +c_a[2j] at p = 2j + 2 (times i tau/12).
+template <class R>
+__global__ void grad_finish_magnus_kernel(const cplx<R>* __restrict__ dg, const cplx<R>* __restrict__ gptr,
+                                          Controls<R> q, int N, int d, R* __restrict__ grad, R* __restrict__ fid) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    const cplx<R> g = gptr[0];
+    const R inv = R(1) / ((R)d * (R)d);
+    if (idx == 0 && fid) fid[0] = (g.x * g.x + g.y * g.y) * inv;
+    if (idx >= (long)q.amps_in * q.pts) return;
+    const int k = (int)(idx / q.pts), p = (int)(idx % q.pts), A = q.amps_set;
+    const R f = q.tau / R(12);
+    // accumulate dg/dc_kp; (re, im) of the factor multiplying G
+    cplx<R> acc = mk<R>(0, 0);
+    auto add = [&](int t, int j, cplx<R> w) {
+        if (j < 0 || j >= N) return;
+        acc = cadd(acc, cmul(w, dg[(size_t)t * N + j]));
+    };
+    auto smp = [&](int b, int i) -> R { return (b < q.amps_in && i >= 0 && i < q.pts) ? q.c[(size_t)b * q.pts + i].x : R(0); };
+    if (p & 1) {
+        add(k, (p - 1) / 2, mk<R>(R(4) / R(6), 0));
+    } else {
+        const int j0 = p / 2, j1 = p / 2 - 1;   // p = 2 j0, p = 2 j1 + 2
+        add(k, j0, mk<R>(R(1) / R(6), 0));
+        add(k, j1, mk<R>(R(1) / R(6), 0));
+        add(A + k, j0, mk<R>(0, -f));
+        add(A + k, j1, mk<R>(0, f));
+        int t = 2 * A;
+        for (int a = 0; a < A; ++a)
+            for (int b = a + 1; b < A; ++b, ++t) {
+                if (k == a) {
+                    add(t, j0, mk<R>(0, f * smp(b, 2 * j0 + 2)));
+                    add(t, j1, mk<R>(0, -f * smp(b, 2 * j1)));
+                } else if (k == b) {
+                    add(t, j0, mk<R>(0, -f * smp(a, 2 * j0 + 2)));
+                    add(t, j1, mk<R>(0, f * smp(a, 2 * j1)));
+                }
+            }
+    }
+    grad[idx] = R(2) * (g.x * acc.x + g.y * acc.y) * inv;   // 2 Re(conj(g) acc) / d^2
+}
+
+template <class R>
+hipError_t launch_grad_finish_magnus(const cplx<R>* dg, const cplx<R>* g, const Controls<R>& q, int N, int d, R* grad,
+                                     R* fid, hipStream_t st) {
+    const long total = (long)q.amps_in * q.pts;
+    const unsigned grid = (unsigned)(total > 0 ? (total + 255) / 256 : 1);
+    hipLaunchKernelGGL((grad_finish_magnus_kernel<R>), dim3(grid), dim3(256), 0, st, dg, g, q, N, d, grad, fid);
+    return hipGetLastError();
+}
+
 bool grad_supported(int d) { return d >= 1 && d <= 32; }
 
 template <class R>
@@ -608,7 +667,9 @@ hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pt
     template hipError_t launch_grad_trace<R>(const cplx<R>*, int, int, const cplx<R>*, int, cplx<R>*, int, int,     \
                                              hipStream_t);                                                           \
     template hipError_t launch_grad_finish<R>(const cplx<R>*, const cplx<R>*, int, int, int, int, int, R*, R*,       \
-                                              hipStream_t);
+                                              hipStream_t);                                                          \
+    template hipError_t launch_grad_finish_magnus<R>(const cplx<R>*, const cplx<R>*, const Controls<R>&, int, int,  \
+                                                     R*, R*, hipStream_t);
 PA_GRAD_INSTANTIATE(float)
 PA_GRAD_INSTANTIATE(double)
 #undef PA_GRAD_INSTANTIATE

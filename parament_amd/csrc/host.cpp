@@ -769,6 +769,15 @@ GradPlan grad_plan(int N, unsigned seg_req, unsigned chunk_req) {
     return p;
 }
 
+// The chain rule from the per-slice coefficient derivatives dg[t][j] to the raw samples: the
+// quadrature weights, or with Magnus terms the derivatives of their coefficients.
+template <class R>
+hipError_t grad_finish(const pa::ChebArgs<R>& ca, const cplx<R>* dg, const cplx<R>* gsum, int N, R* grad, R* fid,
+                       hipStream_t st) {
+    if (ca.ctl.magnus) return pa::launch_grad_finish_magnus<R>(dg, gsum, ca.ctl, N, ca.d, grad, fid, st);
+    return pa::launch_grad_finish<R>(dg, gsum, N, ca.ctl.pts, ca.ctl.amps_in, ca.ctl.mode, ca.d, grad, fid, st);
+}
+
 // Step-wise gradient (any dim, batched GEMMs).  Level l of the pairwise tree has n[l] nodes
 // (level 0: the N slices, kept as E = U - I in the leaf array); V holds levels 1 .. top and W the
 // costates of levels 0 .. top, all kept.  The per-slice arrays (X, two Clenshaw and two
@@ -894,8 +903,7 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         // Q = Dp(X)[W] is in d1 (M = 3: the second step's w[3])
         if (r == hipSuccess) r = pa::launch_grad_trace<R>(d1, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
     }
-    if (r == hipSuccess)
-        r = pa::launch_grad_finish<R>(b.dg, b.gsum, N, ca.ctl.pts, ca.ctl.amps_in, ca.ctl.mode, d, grad, fid, st);
+    if (r == hipSuccess) r = grad_finish<R>(ca, b.dg, b.gsum, N, grad, fid, st);
     return r;
 }
 
@@ -959,8 +967,7 @@ hipError_t grad_fused_enqueue(pa::ChebArgs<R> ca, const GradPlan& p, const Fused
         ca.j0 = j0; ca.N = cnt;
         if (r == hipSuccess) r = pa::launch_grad_slices<R>(ca, b.E, W1, b.dg, N, st);
     }
-    if (r == hipSuccess)
-        r = pa::launch_grad_finish<R>(b.dg, b.gsum, N, ca.ctl.pts, ca.ctl.amps_in, ca.ctl.mode, d, grad, fid, st);
+    if (r == hipSuccess) r = grad_finish<R>(ca, b.dg, b.gsum, N, grad, fid, st);
     return r;
 }
 
@@ -973,7 +980,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     if (!h->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
     // dims <= 32: the fused kernels (on-chip slices and tree levels); larger dims: batched GEMMs
     const bool stepwise = !pa::grad_supported((int)h->dim);
-    if (h->magnus || (!stepwise && h->ncoef > pa::kGradCoeffs)) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!stepwise && h->ncoef > pa::kGradCoeffs) return PARAMENT_STATUS_INVALID_VALUE;
     if (amps == 0 || amps > h->amps) return PARAMENT_STATUS_INVALID_VALUE;
     if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
     int N = 0;
@@ -996,7 +1003,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = d; ca.M = M; ca.s = (R)(2.0 / h->hnorm);
     ca.opsT = h->d_ops + (size_t)(1 + h->ncoef) * dd;
     ca.ctl.c = carr_dev; ca.ctl.pts = (int)pts; ca.ctl.amps_in = (int)amps; ca.ctl.amps_set = (int)h->amps;
-    ca.ctl.mode = h->quad; ca.ctl.magnus = 0; ca.ctl.tau = (R)tau;
+    ca.ctl.mode = h->quad; ca.ctl.magnus = h->magnus ? 1 : 0; ca.ctl.tau = (R)tau;
     for (int k = 0; k <= M; ++k) { ca.are[k] = (R)are[k]; ca.aim[k] = (R)aim[k]; }
     ca.are[0] = (R)(are[0] - 1.0);  // slices are written as E_j = U_j - I
     // The derivative kernel evaluates the same polynomial's derivative in powers of tau X when

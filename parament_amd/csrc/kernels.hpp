@@ -142,5 +142,8 @@ template <class R> hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* 
                                                hipStream_t st);
 template <class R> hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pts, int amps,
                                                  int mode, int d, R* grad, R* fid, hipStream_t st);
+// the chain rule of the Magnus coefficients (commutator terms included) to the raw samples
+template <class R> hipError_t launch_grad_finish_magnus(const cplx<R>* dg, const cplx<R>* g, const Controls<R>& q, int N,
+                                                       int d, R* grad, R* fid, hipStream_t st);
 
 }  // namespace pa

@@ -28,7 +28,7 @@ class GateFidelity(torch.autograd.Function):
 
     controls: real (amps, pts) tensor on the GPU (float64 for an 'fp64' context, float32 for
     'fp32'); target: (dim, dim) complex tensor on the same device.  The context must hold a
-    Hamiltonian with at least ``amps`` controls and no Magnus terms.
+    Hamiltonian with at least ``amps`` controls.
     """
 
     @staticmethod

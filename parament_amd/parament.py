@@ -234,7 +234,7 @@ class Parament:
     def gate_fidelity_gradient(self, dt, *carr, target):
         """F = |Tr(target^H U)|^2 / dim^2 and dF/dc for every control sample (adjoint method).
 
-        Real controls, any dim, no Magnus.  Returns ``(F, grad)`` with ``grad`` shaped like
+        Real controls, any dim, every quadrature mode including Magnus.  Returns ``(F, grad)`` with ``grad`` shaped like
         the stacked controls, ``(amps, pts)``, in the context's precision.
         """
         if self.amps < 0:

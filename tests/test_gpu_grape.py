@@ -102,6 +102,44 @@ def test_stepwise_gradient_few_slices(pts, mode):
     assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
 
 
+def _fd_check(p, c, dt, t, grad, entries, h=1e-6, tol=1e-8):
+    for k, j in entries:
+        cp, cm = c.copy(), c.copy()
+        cp[k, j] += h
+        cm[k, j] -= h
+        fd = (gpu_fidelity(p, cp, dt, t) - gpu_fidelity(p, cm, dt, t)) / (2 * h)
+        assert abs(fd - grad[k, j]) < tol * max(1.0, np.abs(grad).max()), (k, j, fd, grad[k, j])
+
+
+@pytest.mark.parametrize("d", [5, 40])
+def test_magnus_gradient_matches_central_differences(d):
+    """Magnus slices (Simpson means plus commutator terms whose coefficients are differences and
+    products of samples) on the fused (d = 5) and batched-GEMM (d = 40) paths: central
+    differences of the library's own FP64 Magnus propagator, at even and odd samples, the first
+    and the last, of every control."""
+    import parament_amd as parament
+    h0, hk, c, t = _problem(4500 + d, d, 3, 21)
+    dt = 0.05
+    with parament.Parament("fp64") as p:
+        p.set_hamiltonian(h0, *hk, use_magnus=True, quadrature_mode="simpson")
+        f, grad = p.gate_fidelity_gradient(dt, *c, target=t)
+        assert abs(f - gpu_fidelity(p, c, dt, t)) < 1e-13
+        _fd_check(p, c, dt, t, grad, [(0, 0), (1, 1), (2, 2), (0, 7), (1, 10), (2, 13), (0, 19), (1, 20), (2, 20)])
+
+
+def test_complex64_magnus_gradient_matches_fp64():
+    import parament_amd as parament
+    h0, hk, c, t = _problem(4600, 12, 3, 33)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        with parament.Parament(prec) as p:
+            p.set_hamiltonian(h0, *hk, use_magnus=True, quadrature_mode="simpson")
+            out[prec] = p.gate_fidelity_gradient(0.05, *c, target=t)
+    (f64, g64), (f32, g32) = out["fp64"], out["fp32"]
+    assert abs(f32 - f64) < 1e-5 * max(f64, 1e-3)
+    assert np.abs(g32 - g64).max() < 1e-4 * np.abs(g64).max()
+
+
 @pytest.mark.parametrize("cycles", [3, 5])
 def test_stepwise_gradient_manual_cycles(cycles):
     """A manual Chebyshev order (M = 3: the derivative's first two steps are all of it) against
@@ -234,8 +272,8 @@ def test_gradient_argument_errors(prec):
         with pytest.raises(RuntimeError):
             p.gate_fidelity_gradient(0.1, np.zeros(4), target=t)
         p.set_hamiltonian(np.diag([1.0, -1.0]), sx, use_magnus=True, quadrature_mode="simpson")
-        with pytest.raises(ValueError, match="Invalid value"):
-            p.gate_fidelity_gradient(0.1, np.zeros(5), target=t)
+        f, g = p.gate_fidelity_gradient(0.1, np.zeros(5), target=t)   # Magnus contexts are supported
+        assert np.isfinite(f) and np.all(np.isfinite(g))
         p.set_hamiltonian(np.diag([1.0, -1.0]), sx)
         with pytest.raises(ValueError):
             p.gate_fidelity_gradient(0.1, np.zeros(4) + 1j, target=t)
