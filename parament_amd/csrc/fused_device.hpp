@@ -112,6 +112,97 @@ __device__ __forceinline__ CTile<R, SH::T, SH::T> cheb_slice_tile(const cplx<R>*
     return b1;
 }
 
+// The same slice in the power form, for tau Hnorm <= 1: p(X) = sum_k m_k Y^k with Y = tau X by
+// Horner's rule in S = Y^2,
+//   Q_J = m_{2J} I + m_{2J+1} Y,  Q_j = m_{2j} I + m_{2j+1} Y + S Q_{j+1},  p(X) = Q_0  (J = (M-1)/2),
+// (M+1)/2 products instead of the recurrence's M - 1.  pre[0] holds m_0 - 1, so Q_0 = E_j.  X goes
+// into both operand layouts; after S = Y Y the A planes hold S and the B planes Q, and Y is taken
+// from registers at the accumulator positions.  Ends with a barrier.
+template <class R, int D, class SH>
+__device__ __forceinline__ CTile<R, SH::T, SH::T> power_slice_tile(const cplx<R>* __restrict__ ops, int ncoef, int d,
+                                                                   int M, R tau, const R* pre, const R* pim, R* Xr,
+                                                                   R* Xi, R* Br, R* Bi, const cplx<R>* coef) {
+    constexpr int G = SH::G, T = SH::T;
+    constexpr int SA = D + Pad<R>::A, SB = D + Pad<R>::B;
+    typedef CTile<R, T, T> Tile;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int row0 = SH::row0(wave), col0 = SH::col0(wave);
+    const int dd = d * d;
+    for (int e = tid; e < D * D; e += 64 * G) {
+        const int i = e / D, k = e % D;
+        R xr = R(0), xi = R(0);
+        if (i < d && k < d) {
+            cplx<R> x = ops[i * d + k];
+            for (int t = 0; t < ncoef; ++t) x = cadd(x, cmul(coef[t], ops[(size_t)(t + 1) * dd + i * d + k]));
+            xr = x.x; xi = x.y;
+        }
+        Xr[i * SA + k] = xr; Xi[i * SA + k] = xi;
+        Br[i * SB + k] = xr; Bi[i * SB + k] = xi;
+    }
+    __syncthreads();
+    Tile xt, acc, q;
+#pragma unroll
+    for (int m = 0; m < T; ++m)
+#pragma unroll
+        for (int n = 0; n < T; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                xt.re[m][n][v] = Xr[row * SA + col]; xt.im[m][n][v] = Xi[row * SA + col];
+            }
+    wave_cgemm<R, T, T, D, SA, SB>(Xr, Xi, Br, Bi, row0, col0, acc);   // X X
+    // m_{2jj} I + m_{2jj+1} tau X (+ the product)
+    auto horner = [&](int jj, bool with_product) {
+        const cplx<R> mA = mk<R>(pre[2 * jj], pim[2 * jj]);
+        const cplx<R> mB = mk<R>(tau * pre[2 * jj + 1], tau * pim[2 * jj + 1]);
+#pragma unroll
+        for (int m = 0; m < T; ++m)
+#pragma unroll
+            for (int n = 0; n < T; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const bool dg = acc_row<R>(row0, m, v) == acc_col(col0, n);
+                    const cplx<R> y = cmul(mB, mk<R>(xt.re[m][n][v], xt.im[m][n][v]));
+                    q.re[m][n][v] = y.x + (dg ? mA.x : R(0)) + (with_product ? acc.re[m][n][v] : R(0));
+                    q.im[m][n][v] = y.y + (dg ? mA.y : R(0)) + (with_product ? acc.im[m][n][v] : R(0));
+                }
+    };
+    const int J = (M - 1) / 2;
+    const R tau2 = tau * tau;
+    horner(J, false);
+    __syncthreads();   // every wave has read the X planes
+#pragma unroll
+    for (int m = 0; m < T; ++m)
+#pragma unroll
+        for (int n = 0; n < T; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                Xr[row * SA + col] = tau2 * acc.re[m][n][v]; Xi[row * SA + col] = tau2 * acc.im[m][n][v];   // S
+                Br[row * SB + col] = q.re[m][n][v]; Bi[row * SB + col] = q.im[m][n][v];                     // Q_J
+            }
+    __syncthreads();
+    for (int jj = J - 1; jj >= 0; --jj) {
+        wave_cgemm<R, T, T, D, SA, SB>(Xr, Xi, Br, Bi, row0, col0, acc);   // S Q_{jj+1}
+        horner(jj, true);
+        if (jj > 0) {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < T; ++m)
+#pragma unroll
+                for (int n = 0; n < T; ++n)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = acc_row<R>(row0, m, v), col = acc_col(col0, n);
+                        Br[row * SB + col] = q.re[m][n][v]; Bi[row * SB + col] = q.im[m][n][v];
+                    }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    return q;
+}
+
 // One slice of the fused kernel (see cheb_fused_kernel): E_j = U_j - I of slice j into out
 // (d x d); LDS as for cheb_slice_tile, plus coef[ncoef].
 template <class R, int D, class SH>

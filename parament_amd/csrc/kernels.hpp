@@ -21,6 +21,7 @@ template <class R> struct ChebArgs {
     cplx<R>* out;          // N x d x d, E_j = U_j - I (out[0] is slice j0)
     int j0 = 0;            // first slice
     // gradient only: the same polynomial in powers of Y = tau X (tau Hnorm <= 1), m_k = pre/pim[k]
+    // with pre[0] = m_0 - 1; the fused slice kernel and the derivative kernel then use the power form
     int ps = 0;
     R pre[kMaxCheb], pim[kMaxCheb];
     const cplx<R>* opsT = nullptr;   // gradient only: the ncoef operators after H0, each transposed
