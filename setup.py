@@ -31,7 +31,7 @@ setup(
     version="0.1.0",
     description="Piecewise-constant Hamiltonian propagators (Chebyshev equiprop) on MI355X: C ABI, Python class, "
                 "mixed precision, adjoint gradients, ensembles",
-    packages=["parament_amd"],
+    packages=["parament_amd", "parament_amd.test"],
     package_data={"parament_amd": ["libparament.so"]},
     data_files=[("include", ["include/parament.h"])],
     python_requires=">=3.9",
