@@ -92,7 +92,7 @@ def _targets(seed, dims):
     return [haar_unitary(rng, d) for d in dims]
 
 
-@pytest.mark.parametrize("quad", ["none", "simpson"])
+@pytest.mark.parametrize("quad", ["none", "midpoint", "simpson"])
 def test_ensemble_gradient_matches_block_expm_oracle(quad):
     """Every instance's F and dF/dc (fused kernels up to dim 32, batched GEMMs above) against
     the block-matrix-expm gradient of its own problem, to complex64 accuracy."""
@@ -211,3 +211,23 @@ def test_full_size_ensemble512_forward_and_gradient():
         assert rel_err(us[i], u) < 1e-5, (i, int(dims[i]))
         assert abs(fid[i] - f) <= 1e-5 * max(f, 1e-3), (i, int(dims[i]))
         assert np.abs(grads[i] - g).max() <= 1e-4 * np.abs(g).max(), (i, int(dims[i]))
+
+
+@pytest.mark.parametrize("quad,pts", [("none", 1), ("none", 2), ("midpoint", 2), ("simpson", 3), ("simpson", 4)])
+def test_ensemble_gradient_few_samples(quad, pts):
+    """One or two slices per system (no tree above the leaves, or one product), on the fused
+    (dims 3, 20) and batched-GEMM (dim 40) paths."""
+    from oracle import grape_oracle as go
+    from parament_amd.ensemble import Ensemble
+    dims, amps = [3, 20, 40], [2, 1, 2]
+    ptss = [pts] * len(dims)
+    dt = [0.05] * len(dims)
+    hs, cs = make_ensemble(31 + pts, dims, amps, ptss, dt)
+    ts = _targets(32 + pts, dims)
+    with Ensemble(dims, amps, ptss, dt, quad) as e:
+        e.set_hamiltonians(hs)
+        fid, grads = e.gate_fidelity_gradient(cs, ts)
+    for i, (h, c, t) in enumerate(zip(hs, cs, ts)):
+        f_ref, g_ref = go.fidelity_gradient(h[0], h[1:], c, dt[i], quad, t)
+        assert abs(fid[i] - f_ref) < 1e-5 * max(f_ref, 1e-3), (i, dims[i])
+        assert np.abs(grads[i] - g_ref).max() <= 1e-4 * max(np.abs(g_ref).max(), 1e-3), (i, dims[i])
