@@ -1409,9 +1409,10 @@ hipError_t ens_enqueue_class(Ensemble* e, int c, const EnsCfg& cf, hipStream_t s
 }
 
 // One large instance: the step-wise path on batched FP32 matrix-core GEMMs (as Context<float>).
-hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st) {
+// n_cap > 0 (the tuner): only the first n_cap slices, which is enough to rank the GEMM tiles.
+hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) {
     const EnsInst& x = e->in[i];
-    const int d = (int)x.d, N = x.N, M = x.M;
+    const int d = (int)x.d, N = n_cap > 0 ? std::min(x.N, n_cap) : x.N, M = x.M;
     const size_t dd = (size_t)d * d;
     float2* X = e->d_work + x.work_off;
     float2* b1 = X + (size_t)N * dd;
@@ -1527,7 +1528,7 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
         int best_tile = 64;
         for (int tile : {64, 32, 16}) {   // 16: the whole-product kernel
             e->in[i].tile = tile;
-            const float t = ens_time(e, [&] { return ens_enqueue_large(e, i, e->stream); }, 1);
+            const float t = ens_time(e, [&] { return ens_enqueue_large(e, i, e->stream, 512); }, 1);
             if (t < best) { best = t; best_tile = tile; }
         }
         e->in[i].tile = best_tile;
