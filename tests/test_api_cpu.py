@@ -26,6 +26,40 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_header_compiles_as_c_and_a_c_program_links(tmp_path):
+    """include/parament.h is a C99 header: a C program that takes the address of every declared
+    entry point compiles with gcc -Wall -Werror, links against libparament.so, and runs the
+    host-only calls (no device work)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    names = _declared_symbols()
+    src = tmp_path / "abi.c"
+    src.write_text(
+        "#include <stdio.h>\n#include <string.h>\n#include \"parament.h\"\n"
+        "typedef void (*entry)(void);\n"
+        "static entry entries[] = {\n" + ",\n".join(f"    (entry){n}" for n in names) + "\n};\n"
+        "int main(void) {\n"
+        "    Parament_Context_f32 *h = NULL;\n"
+        "    if (Parament_create(&h) != PARAMENT_STATUS_SUCCESS || !h) return 1;\n"
+        "    if (Parament_destroy(h) != PARAMENT_STATUS_SUCCESS) return 2;\n"
+        "    if (strcmp(Parament_errorMessage(PARAMENT_STATUS_SUCCESS), \"Success\") != 0) return 3;\n"
+        "    if (Parament_selectIterationCycles_fp32(1.0, 0.01) < 3) return 4;\n"
+        "    printf(\"%d entries\\n\", (int)(sizeof entries / sizeof entries[0]));\n"
+        "    return entries[0] ? 0 : 5;\n"
+        "}\n")
+    libdir = os.path.join(ROOT, "parament_amd")
+    exe = tmp_path / "abi"
+    cmd = ["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+           "-L", libdir, "-lparament", f"-Wl,-rpath,{libdir}"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, (run.returncode, run.stdout, run.stderr)
+    assert run.stdout.strip() == f"{len(names)} entries"
+
+
 def test_reference_symbols_are_declared():
     # the 17 reference exports (parament.h:155-400) + device_info + OneNorm*
     ref = ["Parament_create", "Parament_destroy", "Parament_setHamiltonian", "Parament_equiprop",
