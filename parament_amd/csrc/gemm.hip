@@ -245,45 +245,55 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             for (int n = 0; n < TW; ++n)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) { accr[L][m][n][v] = 0.f; acci[L][m][n][v] = 0.f; }
-    auto split_store = [&](__bf16* P, int o, float x, int c) {
-        float r = x;
+    // two elements adjacent in k split together: each level's pair goes to LDS in one 4-byte store
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    auto split_store2 = [&](__bf16* P, int o, float x0, float x1, int c) {
+        float r0 = x0, r1 = x1;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const __bf16 h = (__bf16)r;
-            P[(2 * s + c) * PL + o] = h;
-            r -= (float)h;
+            bf16x2 h;
+            h[0] = (__bf16)r0; h[1] = (__bf16)r1;
+            *reinterpret_cast<bf16x2*>(P + (2 * s + c) * PL + o) = h;
+            r0 -= (float)h[0]; r1 -= (float)h[1];
         }
     };
-    // the next K chunk is fetched into registers while the matrix cores work on the current one
-    constexpr int NE = BT * KT / 256;
-    TA ra[NE];
-    TB rb[NE];
+    // the next K chunk is fetched into registers while the matrix cores work on the current one;
+    // thread element pairs: A (r, k), (r, k+1) and B (k, c), (k+1, c), k even
+    constexpr int NP = BT * KT / 512;
+    TA ra[NP][2];
+    TB rb[NP][2];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            const int e = tid + 256 * u;
-            const int gi = ti * BT + e / KT, gk = k0 + e % KT;
-            ra[u].x = 0; ra[u].y = 0;
-            if (gi < d && gk < d) ra[u] = A[(long)gi * d + gk];
+        for (int u = 0; u < NP; ++u) {
+            const int q = tid + 256 * u;
+            const int gi = ti * BT + q / (KT / 2), gk = k0 + 2 * (q % (KT / 2));
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                ra[u][h].x = 0; ra[u][h].y = 0;
+                if (gi < d && gk + h < d) ra[u][h] = A[(long)gi * d + gk + h];
+            }
         }
 #pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            const int e = tid + 256 * u;
-            const int gk = k0 + e / BT, gj = tj * BT + e % BT;
-            rb[u].x = 0; rb[u].y = 0;
-            if (gk < d && gj < d) rb[u] = B[(long)gk * d + gj];
+        for (int u = 0; u < NP; ++u) {
+            const int q = tid + 256 * u;
+            const int gk = k0 + 2 * (q / BT), gj = tj * BT + q % BT;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                rb[u][h].x = 0; rb[u][h].y = 0;
+                if (gk + h < d && gj < d) rb[u][h] = B[(long)(gk + h) * d + gj];
+            }
         }
     };
     fetch(0);
     for (int k0 = 0; k0 < d; k0 += KT) {
 #pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            const int e = tid + 256 * u;
-            const int oa = (e / KT) * S + e % KT, ob = (e % BT) * S + e / BT;
-            split_store(Ap, oa, (float)ra[u].x, 0);
-            split_store(Ap, oa, (float)ra[u].y, 1);
-            split_store(Bp, ob, (float)rb[u].x, 0);
-            split_store(Bp, ob, (float)rb[u].y, 1);
+        for (int u = 0; u < NP; ++u) {
+            const int q = tid + 256 * u;
+            const int oa = (q / (KT / 2)) * S + 2 * (q % (KT / 2)), ob = (q % BT) * S + 2 * (q / BT);
+            split_store2(Ap, oa, (float)ra[u][0].x, (float)ra[u][1].x, 0);
+            split_store2(Ap, oa, (float)ra[u][0].y, (float)ra[u][1].y, 1);
+            split_store2(Bp, ob, (float)rb[u][0].x, (float)rb[u][1].x, 0);
+            split_store2(Bp, ob, (float)rb[u][0].y, (float)rb[u][1].y, 1);
         }
         __syncthreads();
         if (k0 + KT < d) fetch(k0 + KT);
