@@ -36,10 +36,13 @@ test_numerics.py:32-97); the tree against brute-force sequential products and
 the reference's 10-step two-field test (test_numerics.py:100-117); the
 quadrature rules by closed forms (Simpson of constant controls equals 'none' at
 2 dt, midpoint of linear controls is the exact average); the docstring matrix
-of parament.py:61-68.  Magnus: parity is pinned only by the commuting-case
-invariant (all [H_a, H_b] = 0 reduces Magnus to Simpson) and by a second-order
-accuracy check against scipy expm of a finely resolved time-ordered product;
-the reference ships no Magnus test.
+of parament.py:61-68.  Magnus (the reference ships no Magnus test): with
+controls linear across every slice, H(t) = P + t Q, the step must equal
+exp(Omega1 + Omega2) with the closed forms Omega1 = -i tau (H(0) + H(tau))/2 and
+Omega2 = (tau^3/12)[P, Q], slice by slice to 1e-12; its global error must fall
+16x per halving of dt against a fine Gauss-Legendre Magnus product (Simpson
+means alone: 4x); with all [H_a, H_b] = 0 it reduces to Simpson.  A flipped
+sign or a wrong factor on either commutator row fails the first two.
 """
 from __future__ import annotations
 

@@ -204,6 +204,68 @@ def test_magnus_is_more_accurate_than_simpson():
     assert e_m < 0.5 * e_s
 
 
+def test_magnus_equals_closed_form_for_piecewise_linear_controls():
+    # With H(t) = P + t Q on a slice of length tau, the Magnus series stops after two terms up to
+    # O(tau^5), and those two are closed forms: Omega1 = -i tau (H(0) + H(tau)) / 2 and
+    # Omega2 = -1/2 int_0^tau int_0^t1 [H(t1), H(t2)] = (tau^3 / 12) [P, Q].  Controls that are
+    # linear across every slice (mid sample = mean of the ends) make the fourth-order step exact
+    # in those terms, so the oracle must equal the product of exp(Omega1 + Omega2) slice by slice.
+    rng = np.random.default_rng(12)
+    d, amps, n, dt = 4, 3, 5, 0.07
+    h0 = gue(rng, d)
+    hks = [gue(rng, d) for _ in range(amps)]
+    c = np.empty((amps, 2 * n + 1))
+    c[:, 0::2] = rng.uniform(-1.5, 1.5, (amps, n + 1))
+    c[:, 1::2] = 0.5 * (c[:, 0:-1:2] + c[:, 2::2])
+    tau = 2 * dt
+    ham = lambda col: h0 + sum(col[k] * hks[k] for k in range(amps))
+    u = np.eye(d, dtype=complex)
+    for j in range(n):
+        ha, hb = ham(c[:, 2 * j]), ham(c[:, 2 * j + 2])
+        p, q = ha, (hb - ha) / tau
+        omega = -1j * tau * 0.5 * (ha + hb) + tau ** 3 / 12 * (p @ q - q @ p)
+        u = scipy.linalg.expm(omega) @ u
+    got = eo.equiprop(h0, hks, c, dt, mode="simpson", use_magnus=True)
+    assert rel_err(got, u) < 1e-12
+    # the commutator terms are not negligible here: Simpson alone misses by far more
+    assert rel_err(eo.equiprop(h0, hks, c, dt, mode="simpson"), u) > 1e-4
+
+
+def test_magnus_converges_at_fourth_order():
+    # Global error of the fourth-order Magnus step falls by 2^4 per halving of dt, Simpson
+    # means alone (exp(-i tau Hbar)) by 2^2.  A wrong factor or sign on either commutator row
+    # leaves an O(tau^3) local error and shows as second order.
+    rng = np.random.default_rng(3)
+    h0, h1, h2 = gue(rng, 3), gue(rng, 3), gue(rng, 3)
+    f = lambda t: np.stack([np.sin(2 * t + 0.4), np.cos(3 * t) ** 2])
+    t_end = 0.8
+
+    def truth():
+        fine = 4096
+        h = t_end / fine
+        u = np.eye(3, dtype=complex)
+        for s in range(fine):   # Gauss-Legendre 2-point Magnus step, order 4, on a fine grid
+            ta, tb = (s + 0.5 - 0.5 / np.sqrt(3)) * h, (s + 0.5 + 0.5 / np.sqrt(3)) * h
+            ca, cb = f(ta), f(tb)
+            ha = h0 + ca[0] * h1 + ca[1] * h2
+            hb = h0 + cb[0] * h1 + cb[1] * h2
+            om = -1j * h * 0.5 * (ha + hb) + np.sqrt(3) / 12 * h * h * (ha @ hb - hb @ ha)
+            u = scipy.linalg.expm(om) @ u
+        return u
+
+    u = truth()
+    errs_m, errs_s = [], []
+    for pts in (9, 17, 33):
+        dt = t_end / (pts - 1)
+        c = f(np.arange(pts) * dt)
+        errs_m.append(np.linalg.norm(eo.equiprop(h0, [h1, h2], c, dt, mode="simpson", use_magnus=True) - u))
+        errs_s.append(np.linalg.norm(eo.equiprop(h0, [h1, h2], c, dt, mode="simpson") - u))
+    for a, b in zip(errs_m, errs_m[1:]):
+        assert 12 < a / b < 20
+    for a, b in zip(errs_s, errs_s[1:]):
+        assert 3 < a / b < 5
+
+
 # ---------------- whole pipeline vs the stepwise expm oracle ----------------
 
 @pytest.mark.parametrize("mode,mag", [("none", False), ("midpoint", False), ("simpson", False), ("simpson", True)])
