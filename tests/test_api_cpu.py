@@ -168,3 +168,80 @@ def test_precision_argument_checks():
         context.set_precision_mode('native')
         with pytest.raises(ValueError):
             context.set_precision_mode('bf16')
+
+
+# ---- additions: argument checks of the ensemble and gradient entry points (host only) ----
+
+def _raw(name, restype=ctypes.c_int):
+    """The entry point without the ctypes argument conversions, so that null pointers and
+    out-of-range values reach the C checks."""
+    from parament_amd import paramentlib
+    f = getattr(ctypes.CDLL(paramentlib.lib._name), name)
+    f.restype = restype
+    return f
+
+
+def test_ensemble_create_rejects_bad_arguments():
+    create = _raw("Parament_ensembleCreate")
+    u = lambda *v: (ctypes.c_uint * len(v))(*v)
+    dbl = lambda *v: (ctypes.c_double * len(v))(*v)
+    none, midpoint, simpson = 0, 0x01000000, 0x02000000
+    dims, amps, pts, dts = u(4, 8), u(1, 2), u(9, 9), dbl(0.01, 0.02)
+    cases = {
+        "empty ensemble": (0, dims, amps, pts, dts, none),
+        "null dims": (2, None, amps, pts, dts, none),
+        "null dt": (2, dims, amps, pts, None, none),
+        "unknown quadrature": (2, dims, amps, pts, dts, 7),
+        "Magnus flag": (2, dims, amps, pts, dts, simpson | 0x10000000),
+        "dim 0": (2, u(4, 0), amps, pts, dts, none),
+        "no controls": (2, dims, u(1, 0), pts, dts, none),
+        "more controls than a slice holds": (2, dims, u(1, 257), pts, dts, none),
+        "dt 0": (2, dims, amps, pts, dbl(0.01, 0.0), none),
+        "dt < 0": (2, dims, amps, pts, dbl(-0.01, 0.02), none),
+        "dt NaN": (2, dims, amps, pts, dbl(0.01, float("nan")), none),
+        "no samples": (2, dims, amps, u(9, 0), dts, none),
+        "midpoint of one sample": (2, dims, amps, u(9, 1), dts, midpoint),
+        "Simpson of two samples": (2, dims, amps, u(9, 2), dts, simpson),
+    }
+    for what, args in cases.items():
+        h = ctypes.c_void_p()
+        assert create(ctypes.byref(h), *args) == 50, what
+        assert not h.value, what
+    assert create(None, 2, dims, amps, pts, dts, none) == 50
+
+
+def test_ensemble_entry_points_reject_null_handle():
+    assert _raw("Parament_ensembleDestroy")(None) == 0
+    for name, nargs in (("Parament_ensembleSetHamiltonians", 1), ("Parament_ensembleSetExecution", 1),
+                        ("Parament_ensembleTune", 0), ("Parament_ensembleGetTuning", 1),
+                        ("Parament_ensembleEquiprop", 2), ("Parament_ensembleEquipropDevice", 2),
+                        ("Parament_ensembleSynchronize", 0), ("Parament_ensembleGateFidelityGradient", 4),
+                        ("Parament_ensembleGateFidelityGradientDevice", 4)):
+        assert _raw(name)(None, *([None] * nargs)) == 50, name
+
+
+@pytest.mark.parametrize("sfx,cplx", [("", np.complex64), ("_fp64", np.complex128)])
+def test_gradient_entry_points_resolve_on_host(sfx, cplx):
+    """No context, or a context without a Hamiltonian: the gradient returns the reference's
+    codes (INVALID_VALUE 50, NO_HAMILTONIAN 80) before any device work, and the workspace query
+    reads zero."""
+    grad = _raw("Parament_gateFidelityGradient" + sfx)
+    grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_uint, ctypes.c_uint,
+                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    real = np.float32 if cplx == np.complex64 else np.float64
+    c = np.zeros(8, cplx)
+    t = np.eye(2, dtype=cplx)
+    f = np.zeros(1, real)
+    g = np.zeros(8, real)
+    ptr = lambda a: a.ctypes.data
+    assert grad(None, ptr(c), 0.1, 8, 1, ptr(t), ptr(f), ptr(g)) == 50
+    create, destroy = _raw("Parament_create" + sfx), _raw("Parament_destroy" + sfx)
+    h = ctypes.c_void_p()
+    assert create(ctypes.byref(h)) == 0
+    try:
+        assert grad(h, ptr(c), 0.1, 8, 1, ptr(t), ptr(f), ptr(g)) == 80
+        workspace = _raw("Parament_getGradientWorkspace" + sfx, ctypes.c_size_t)
+        workspace.argtypes = [ctypes.c_void_p]
+        assert workspace(h) == 0 and workspace(None) == 0
+    finally:
+        assert destroy(h) == 0
