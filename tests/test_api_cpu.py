@@ -245,3 +245,24 @@ def test_gradient_entry_points_resolve_on_host(sfx, cplx):
         assert workspace(h) == 0 and workspace(None) == 0
     finally:
         assert destroy(h) == 0
+
+
+def test_ensemble_class_checks_arguments():
+    from parament_amd.ensemble import Ensemble
+    with pytest.raises(ValueError, match="same length"):
+        Ensemble([4, 8], [1], [9, 9], 0.01)
+    with pytest.raises(ValueError, match="Unknown quadrature"):
+        Ensemble([4], [1], [9], 0.01, "magnus")
+    with pytest.raises(ValueError, match="Invalid value"):
+        Ensemble([4, 0], [1, 1], [9, 9], 0.01)
+    with pytest.raises(ValueError, match="Invalid value"):
+        Ensemble([4], [1], [2], 0.01, "simpson")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "measures one GPU" in r.stderr and not r.stdout.strip()
