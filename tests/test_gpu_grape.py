@@ -161,6 +161,28 @@ def test_stepwise_gradient_manual_cycles(cycles):
             assert abs(fd - grad[k, j]) < 1e-8 * max(1.0, np.abs(grad).max()), (k, j, fd, grad[k, j])
 
 
+@pytest.mark.parametrize("d,cycles,dt", [(8, 3, 0.01), (8, 7, 0.01), (32, 5, 0.01), (16, 9, 0.1), (32, 21, 0.2)])
+def test_fused_gradient_manual_cycles(d, cycles, dt):
+    """A manual Chebyshev order on the fused kernels (the power-form derivative where tau Hnorm
+    <= 1, the forward-mode Clenshaw derivative above) against central differences of the
+    library's own FP64 propagator with the same order."""
+    import parament_amd as parament
+    h0, hk, c, t = _problem(4500 + d + cycles, d, 2, 15)
+    h = 1e-6
+    with parament.Parament("fp64") as p:
+        p.set_hamiltonian(h0, *hk)
+        p.set_iteration_cycles(cycles)
+        f, grad = p.gate_fidelity_gradient(dt, *c, target=t)
+        assert p.iteration_cycles == cycles
+        assert abs(f - gpu_fidelity(p, c, dt, t)) < 1e-12
+        for k, j in [(0, 0), (1, 7), (0, 14), (1, 3)]:
+            cp, cm = c.copy(), c.copy()
+            cp[k, j] += h
+            cm[k, j] -= h
+            fd = (gpu_fidelity(p, cp, dt, t) - gpu_fidelity(p, cm, dt, t)) / (2 * h)
+            assert abs(fd - grad[k, j]) < 1e-8 * max(1.0, np.abs(grad).max()), (k, j, fd, grad[k, j])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_stepwise_gradient_chunks_agree(prec):
     """Per-slice arrays processed 7 slices at a time give the whole-batch result."""
