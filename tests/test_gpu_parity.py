@@ -331,3 +331,47 @@ def test_mixed_stepwise_modes(variant, tol, d, n, mode):
     u, m = run_gpu("fp64", h0, hks, c, 0.01, mode, mixed=variant)
     ref = eo.equiprop(h0, hks, c, 0.01, mode=mode, cycles=m)
     assert rel_err(u, ref) < tol
+
+
+def test_caller_streams_and_concurrent_contexts():
+    """Two contexts (a fused and a step-wise dim) on two caller streams at once: the stream a
+    context reports is the one it was given, each result equals the oracle once its own stream
+    is synchronised, and passing back the handle read first restores the context's own stream."""
+    import torch
+    import parament_amd as parament
+    dev = torch.device("cuda:0")
+    cases = [case(800 + d, d, 2, 40) for d in (16, 40)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]   # default priority
+    ctxs = []
+    try:
+        for (h0, hks, c), s in zip(cases, streams):
+            p = parament.Parament("fp64")
+            ctxs.append(p)
+            p.set_hamiltonian(h0, *hks)
+            own = p.stream
+            assert own
+            p.set_stream(s.cuda_stream)
+            assert p.stream == s.cuda_stream
+            p.own = own
+        bufs = []
+        for p, (h0, hks, c), s in zip(ctxs, cases, streams):
+            d = h0.shape[0]
+            with torch.cuda.stream(s):
+                carr = torch.tensor(c, dtype=torch.complex128, device=dev)
+                out = torch.empty(d * d, dtype=torch.complex128, device=dev)
+            p.equiprop_device(0.02, carr.data_ptr(), c.shape[1], c.shape[0], out.data_ptr())
+            bufs.append((carr, out))
+        for s in streams:
+            s.synchronize()
+        for (carr, out), (h0, hks, c) in zip(bufs, cases):
+            d = h0.shape[0]
+            u = out.cpu().numpy().reshape(d, d)
+            assert rel_err(u, oracle_for("fp64", h0, hks, c, 0.02)) < TOL["fp64"]
+        for p, (h0, hks, c) in zip(ctxs, cases):
+            p.set_stream(p.own)
+            assert p.stream == p.own
+            assert rel_err(p.equiprop(0.02, *c), oracle_for("fp64", h0, hks, c, 0.02)) < TOL["fp64"]
+    finally:
+        torch.cuda.synchronize()
+        for p in ctxs:
+            p.destroy()
