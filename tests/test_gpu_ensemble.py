@@ -37,6 +37,31 @@ def test_ensemble_matches_restatement(quad):
             assert rel_err(u, ref) < 1e-5, (i, DIMS[i], pts[i])
 
 
+@pytest.mark.parametrize("dims,amps,pts,quad", [
+    ([7], [2], [50], "midpoint"),                                   # one instance
+    ([20, 24, 30, 18], [1, 2, 1, 3], [64, 33, 100, 5], "none"),     # one class only
+    ([90], [1], [41], "simpson"),                                   # one batched-GEMM instance
+    ([70, 128, 66], [2, 1, 3], [30, 20, 65], "midpoint"),           # batched-GEMM instances only
+])
+def test_ensemble_shapes(dims, amps, pts, quad):
+    """Ensembles of one instance, of one class, and of batched-GEMM instances only, as the graph
+    and as the loop, then again through the same object after new Hamiltonians (the plan and the
+    tuning are rebuilt)."""
+    from parament_amd.ensemble import Ensemble
+    dt = [0.02] * len(dims)
+    with Ensemble(dims, amps, pts, dt, quad) as e:
+        for seed in (31, 32):
+            hs, cs = make_ensemble(seed, dims, amps, pts, dt)
+            e.set_hamiltonians(hs)
+            for mode in ("graph", "loop"):
+                e.set_execution(mode)
+                us = e.equiprop(cs)
+                for i, (h, c, u) in enumerate(zip(hs, cs, us)):
+                    h0, hk, c32 = as_c64_inputs(h[0], h[1:], c)
+                    ref = eo.equiprop(h0, hk, c32, dt[i], mode=quad, precision="fp32")
+                    assert rel_err(u, ref) < 1e-5, (seed, mode, i, dims[i])
+
+
 def test_graph_replays_and_loop_agree():
     from parament_amd.ensemble import Ensemble
     dt = [0.02] * len(DIMS)
