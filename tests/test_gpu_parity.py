@@ -193,7 +193,7 @@ def test_full_size_dim8_4096_steps(prec):
 # ---- mixed precision (complex128 interface; bf16-split matrix cores + FP64 first-order terms) ----
 
 @pytest.mark.parametrize("mode,magnus", MODES)
-@pytest.mark.parametrize("d", [2, 8, 16, 31, 32, 33, 64])
+@pytest.mark.parametrize("d", [2, 8, 16, 31, 32, 33, 50, 63, 64])
 def test_mixed_parity(mode, magnus, d):
     """Same truncation as the FP32 table, within 1e-5 relative of the FP64 restatement."""
     h0, hks, c = case(800 + d, d, 2, 65)
