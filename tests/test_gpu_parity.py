@@ -109,7 +109,7 @@ MODES = [("none", False), ("midpoint", False), ("simpson", False), ("simpson", T
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("mode,magnus", MODES)
-@pytest.mark.parametrize("d", [2, 3, 8, 16, 17, 32, 33, 64])
+@pytest.mark.parametrize("d", [2, 3, 8, 16, 17, 24, 32, 33, 47, 63, 64])
 def test_parity_fused(prec, mode, magnus, d):
     h0, hks, c = case(100 + d, d, 2, 65)
     u, m = run_gpu(prec, h0, hks, c, 0.02, mode, magnus)
