@@ -119,7 +119,7 @@ def test_parity_fused(prec, mode, magnus, d):
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("mode,magnus", MODES)
-@pytest.mark.parametrize("d", [2, 8, 33, 72, 100])
+@pytest.mark.parametrize("d", [2, 8, 33, 72, 100, 129, 160])
 def test_parity_stepwise(prec, mode, magnus, d):
     h0, hks, c = case(200 + d, d, 2, 33)
     u, _ = run_gpu(prec, h0, hks, c, 0.02, mode, magnus, path="stepwise")
