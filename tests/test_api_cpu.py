@@ -266,3 +266,15 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=300)
     assert r.returncode == 2
     assert "measures one GPU" in r.stderr and not r.stdout.strip()
+
+
+def test_coefficient_limit_resolves_on_host():
+    """At most 256 operator coefficients per slice: A controls, with Magnus A + A + A(A-1)/2
+    (20 controls fit, 22 do not); checked before any device work."""
+    import parament_amd as parament
+    eye = np.eye(2)
+    with parament.Parament() as p:
+        with pytest.raises(ValueError, match="Invalid value"):
+            p.set_hamiltonian(eye, *([eye] * 257))
+        with pytest.raises(ValueError, match="Invalid value"):
+            p.set_hamiltonian(eye, *([eye] * 22), use_magnus=True, quadrature_mode="simpson")

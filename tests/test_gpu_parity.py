@@ -151,6 +151,16 @@ def test_manual_cycles(prec, cycles):
     assert rel_err(u, oracle_for(prec, h0, hks, c, 0.05, "midpoint", cycles=cycles)) < TOL[prec]
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("amps,magnus", [(256, False), (20, True)])
+def test_maximum_coefficients(prec, amps, magnus):
+    """The largest coefficient counts a slice holds: 256 controls, or 20 controls with their
+    20 + 190 Magnus commutators."""
+    h0, hks, c = case(900 + amps, 4, amps, 9)
+    u, _ = run_gpu(prec, h0, hks, c, 1e-3, "simpson", magnus)
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 1e-3, "simpson", magnus)) < TOL[prec]
+
+
 def test_fewer_amplitudes_than_hamiltonians():
     h0, hks, c = case(500, 8, 3, 21)
     u, _ = run_gpu("fp64", h0, hks, c[:1], 0.05, "simpson")
