@@ -43,6 +43,18 @@ def test_gradient_matches_block_expm_oracle(mode, d):
     assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
 
 
+@pytest.mark.parametrize("d", [8, 40])
+def test_gradient_many_controls(d):
+    """43 controls: the traces run eight controls per pass over each slice's derivative, so
+    five full groups and a partial last one are checked against the oracle."""
+    h0, hk, c, t = _problem(1700 + d, d, 43, 11)
+    hk = [0.2 * m for m in hk]
+    f, grad = gpu_grad("fp64", h0, hk, c, 0.02, "none", t)
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, 0.02, "none", t)
+    assert abs(f - f_ref) < 1e-12
+    assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
+
+
 @pytest.mark.parametrize("d", [5, 32])
 @pytest.mark.parametrize("dt", [0.02, 0.3])
 def test_gradient_both_derivative_forms_match_oracle(d, dt):
