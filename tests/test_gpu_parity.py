@@ -343,6 +343,16 @@ def test_mixed_stepwise_modes(variant, tol, d, n, mode):
     assert rel_err(u, ref) < tol
 
 
+@pytest.mark.parametrize("variant,tol", [("mixed", 1e-5), ("mixed_fp32", 1e-5), ("mixed_bf16x2", 5e-3)])
+def test_mixed_stepwise_magnus(variant, tol):
+    """The three product modes on batched GEMMs with Magnus slices (commutator operators built on
+    the GEMMs, their coefficients formed per slice)."""
+    h0, hks, c = case(1600, 70, 2, 33)
+    u, m = run_gpu("fp64", h0, hks, c, 0.01, "simpson", True, mixed=variant)
+    ref = eo.equiprop(h0, hks, c, 0.01, mode="simpson", use_magnus=True, cycles=m)
+    assert rel_err(u, ref) < tol
+
+
 def test_caller_streams_and_concurrent_contexts():
     """Two contexts (a fused and a step-wise dim) on two caller streams at once: the stream a
     context reports is the one it was given, each result equals the oracle once its own stream
