@@ -1494,14 +1494,15 @@ hipError_t ens_enqueue_loop(Ensemble* e) {
 }
 
 float ens_time(Ensemble* e, const std::function<hipError_t()>& f, int reps) {
+    // a candidate whose launches or timing fail is never the fastest
     if (f() != hipSuccess) return 1e30f;   // warm-up
-    hipEventRecord(e->t0, e->stream);
+    if (hipEventRecord(e->t0, e->stream) != hipSuccess) return 1e30f;
     for (int k = 0; k < reps; ++k)
         if (f() != hipSuccess) return 1e30f;
-    hipEventRecord(e->t1, e->stream);
-    if (hipEventSynchronize(e->t1) != hipSuccess) return 1e30f;
     float ms = 0;
-    hipEventElapsedTime(&ms, e->t0, e->t1);
+    if (hipEventRecord(e->t1, e->stream) != hipSuccess || hipEventSynchronize(e->t1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e->t0, e->t1) != hipSuccess)
+        return 1e30f;
     return ms / reps;
 }
 
