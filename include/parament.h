@@ -57,6 +57,8 @@ typedef enum Parament_ErrorCode {
     PARAMENT_STATUS_SELECT_SMALLER_DT = 70,
     PARAMENT_STATUS_NO_HAMILTONIAN = 80,
     PARAMENT_STATUS_INVALID_QUADRATURE_SELECTION = 90,
+    /* addition: the call needs more device memory than a fixed workspace holds */
+    PARAMENT_STATUS_WORKSPACE_TOO_SMALL = 100,
     PARAMENT_FAIL = 1000
 } Parament_ErrorCode;
 
@@ -118,19 +120,22 @@ LIBSPEC double OneNorm_fp64(cuDoubleComplex *mat, unsigned int dim);
 
 /* ======================== Additions (no reference counterpart) ======================== */
 
-/* Arithmetic of a complex128 context's equiprop (for callers that want the fast
- * mixed-precision path behind the fp64 interface):
- *   PARAMENT_PRECISION_NATIVE: every step in FP64 (default; the reference algorithm).
- *   PARAMENT_PRECISION_MIXED:  FP64 for Hnorm, M (fp32 table plus an accumulated-truncation
- *       bound), the Bessel coefficients, the slice Hamiltonians, the first-order Chebyshev
- *       terms and the upper levels of the product tree; the higher-order Chebyshev remainder
- *       and the lowest three tree levels run as bf16 three-way split products with FP32
- *       accumulation on the matrix cores (FP32-accurate products).
+/* Arithmetic of a context's equiprop:
+ *   PARAMENT_PRECISION_NATIVE: every step in the context's precision (the reference
+ *       algorithm): FP64 for complex128, FP32 for complex64.  Default of complex128 contexts.
+ *   PARAMENT_PRECISION_MIXED:  FP64 for Hnorm, the Bessel coefficients, the slice
+ *       Hamiltonians, the first-order Chebyshev terms and the upper levels of the product
+ *       tree; the higher-order Chebyshev remainder and the lowest three tree levels run as
+ *       bf16 three-way split products with FP32 accumulation on the matrix cores
+ *       (FP32-accurate products).  Default of complex64 contexts, which keep the FP32
+ *       iteration table's M and, for dims <= 16 on the automatic kernel path, the FP32
+ *       wavefront-resident kernels.  A complex128 context in this mode takes M from the FP32
+ *       table raised until the accumulated truncation error stays below 2e-6.
  *   PARAMENT_PRECISION_MIXED_FP32:   the same with FP32 matrix-core products.
  *   PARAMENT_PRECISION_MIXED_BF16X2: the same with bf16 hi+lo (two-way split) products
  *       (about 16 significant bits; for precision studies).
  * MIXED runs the fused on-chip kernel for dim <= 64 and batched GEMMs above; the other two
- * always run the batched GEMMs. */
+ * always run the batched GEMMs.  getPrecisionMode returns the current mode (-1 for NULL). */
 typedef enum Parament_PrecisionMode {
     PARAMENT_PRECISION_NATIVE = 0,
     PARAMENT_PRECISION_MIXED = 1,
@@ -138,7 +143,10 @@ typedef enum Parament_PrecisionMode {
     PARAMENT_PRECISION_MIXED_BF16X2 = 3
 } Parament_PrecisionMode;
 
+LIBSPEC Parament_ErrorCode Parament_setPrecisionMode(Parament_Context_f32 *handle, Parament_PrecisionMode mode);
 LIBSPEC Parament_ErrorCode Parament_setPrecisionMode_fp64(Parament_Context_f64 *handle, Parament_PrecisionMode mode);
+LIBSPEC int Parament_getPrecisionMode(Parament_Context_f32 *handle);
+LIBSPEC int Parament_getPrecisionMode_fp64(Parament_Context_f64 *handle);
 
 /* Chebyshev order M used by the last successful equiprop (-1 before any). */
 LIBSPEC int Parament_getIterationCycles(Parament_Context_f32 *handle);

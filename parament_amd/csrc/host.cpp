@@ -86,19 +86,23 @@ void bessel(int m, double x, double* re, double* im) {
 // p(y) = a_0 + 2 sum_{k>=1} a_k T_k(y) that the Clenshaw recurrence evaluates, returns m_k with
 // p = sum_k m_k Y^k for Y = tau X = x y (x = tau Hnorm), in FP64; mr[0] holds m_0 - 1.
 void power_coefficients(int m, double x, const double* are, const double* aim, double* mr, double* mi) {
-    std::vector<double> tprev(m + 1, 0.0), tcur(m + 1, 0.0), tnext(m + 1, 0.0);
-    std::vector<double> cr(m + 1, 0.0), ci(m + 1, 0.0);
+    // monomial coefficients of T_{k-1}, T_k, T_{k+1} (m < kMaxCheb)
+    double t0[pa::kMaxCheb + 1] = {}, t1[pa::kMaxCheb + 1] = {}, t2[pa::kMaxCheb + 1] = {};
+    double cr[pa::kMaxCheb + 1] = {}, ci[pa::kMaxCheb + 1] = {};
+    double *tprev = t0, *tcur = t1, *tnext = t2;
     tprev[0] = 1.0;   // T_0
     tcur[1] = 1.0;    // T_1
     cr[0] += are[0]; ci[0] += aim[0];
     for (int k = 1; k <= m; ++k) {
         for (int i = 0; i <= k; ++i) { cr[i] += 2.0 * are[k] * tcur[i]; ci[i] += 2.0 * aim[k] * tcur[i]; }
         if (k == m) break;
-        std::fill(tnext.begin(), tnext.end(), 0.0);   // T_{k+1} = 2 y T_k - T_{k-1}
+        for (int i = 0; i <= k + 1; ++i) tnext[i] = 0.0;   // T_{k+1} = 2 y T_k - T_{k-1}
         for (int i = 0; i <= k; ++i) tnext[i + 1] += 2.0 * tcur[i];
         for (int i = 0; i <= k - 1; ++i) tnext[i] -= tprev[i];
-        tprev.swap(tcur);
-        tcur.swap(tnext);
+        double* t = tprev;
+        tprev = tcur;
+        tcur = tnext;
+        tnext = t;
     }
     double xp = 1.0;
     for (int k = 0; k <= m; ++k, xp *= x) { mr[k] = cr[k] / xp; mi[k] = ci[k] / xp; }
@@ -156,7 +160,9 @@ template <class R> struct Context {
     int mmax = 11;
     bool manual = false;
     int last_m = -1;
-    int precision = PARAMENT_PRECISION_NATIVE;
+    // complex128 contexts compute in FP64 unless a mixed mode is selected; complex64 contexts
+    // default to the mixed pipeline (FP64 first-order terms and upper tree levels)
+    int precision = sizeof(R) == 4 ? PARAMENT_PRECISION_MIXED : PARAMENT_PRECISION_NATIVE;
     int path = 0;
     // Hamiltonian
     bool has_h = false;
@@ -174,12 +180,16 @@ template <class R> struct Context {
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     C* d_ops = nullptr;
+    // complex64 contexts: the operators (Magnus commutators included) in FP64, for the mixed pipeline
+    double2* d_ops64 = nullptr;
     // working memory (grow-only, like parament.cpp:418)
     C* d_c = nullptr; size_t cap_c = 0;
     void* d_leaves = nullptr; size_t cap_leaves = 0;
     void* d_nodes = nullptr; size_t cap_nodes = 0;
     C* d_gen[3] = {nullptr, nullptr, nullptr}; size_t cap_gen = 0;
     void* d_root = nullptr; size_t cap_root = 0;  // d x d result of the last equiprop
+    void* d_c64 = nullptr; size_t cap_c64 = 0;    // complex64 contexts: controls widened to FP64
+    void* d_out64 = nullptr; size_t cap_out64 = 0;  // complex64 contexts: FP64 result before rounding
     // gradient: leaves E, tree levels V, costates W, per-slice traces, trace g; tree layout policy
     void* d_gE = nullptr; size_t cap_gE = 0;
     void* d_gV = nullptr; size_t cap_gV = 0;
@@ -198,6 +208,8 @@ template <class R> struct Context {
 template <class R> void free_hamiltonian(Context<R>* h) {
     if (h->d_ops) hipFree(h->d_ops);
     h->d_ops = nullptr;
+    if (h->d_ops64) hipFree(h->d_ops64);
+    h->d_ops64 = nullptr;
     h->has_h = false;
 }
 
@@ -208,6 +220,8 @@ template <class R> void free_working(Context<R>* h) {
     for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
     h->cap_gen = 0;
     hipFree(h->d_root); h->d_root = nullptr; h->cap_root = 0;
+    hipFree(h->d_c64); h->d_c64 = nullptr; h->cap_c64 = 0;
+    hipFree(h->d_out64); h->d_out64 = nullptr; h->cap_out64 = 0;
     for (void** p : {&h->d_gE, &h->d_gV, &h->d_gW, &h->d_gdg, &h->d_gaux}) { hipFree(*p); *p = nullptr; }
     h->cap_gE = h->cap_gV = h->cap_gW = h->cap_gdg = h->cap_gaux = 0;
 }
@@ -221,6 +235,25 @@ bool grow(void** p, size_t* cap, size_t bytes) {
     *cap = bytes;
     return true;
 }
+
+// Device working memory of one equiprop call.  By default every buffer is one of the context's
+// grow-only allocations; with `fixed` set, buffers are consecutive 256-byte aligned pieces of
+// that caller-provided region, and a request beyond its end fails instead of allocating.
+struct Workspace {
+    char* fixed = nullptr;
+    size_t cap = 0, used = 0;
+    // `bytes` of device memory: the context slot (grown when needed) or the next fixed piece
+    void* get(void** slot, size_t* slot_cap, size_t bytes) {
+        if (!fixed) return grow(slot, slot_cap, bytes) ? *slot : nullptr;
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (bytes > cap || at > cap - bytes) return nullptr;
+        used = at + bytes;
+        return fixed + at;
+    }
+    Parament_ErrorCode failure() const {
+        return fixed ? PARAMENT_STATUS_WORKSPACE_TOO_SMALL : PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+};
 
 // Batched complex GEMM on the matrix cores (gemm.hip): C = alpha A B + beta Cin + diag I
 // (+ A + B with add_ab); pointers with their element type (true = complex128), strides in
@@ -326,15 +359,13 @@ template <class R> Parament_ErrorCode destroy(Context<R>* h) {
 
 // Magnus commutators on the device: [H0, H_k] then [H_a, H_b] (a < b, lexicographic),
 // each as A B - B A with the generic GEMM (parament.cpp:289-359).
-template <class R> hipError_t magnus_commutators(Context<R>* h) {
+template <class R> hipError_t magnus_commutators(cplx<R>* ops, unsigned d, unsigned A, hipStream_t st) {
     typedef cplx<R> C;
-    const unsigned d = h->dim, A = h->amps;
     const size_t dd = (size_t)d * d;
-    C* ops = h->d_ops;
     auto comm = [&](const C* X, const C* Y, C* out) -> hipError_t {
-        hipError_t e = zgemm_native<R>(X, 0, Y, 0, nullptr, 0, out, 0, c2(1), c2(0), c2(0), d, 1, h->stream);
+        hipError_t e = zgemm_native<R>(X, 0, Y, 0, nullptr, 0, out, 0, c2(1), c2(0), c2(0), d, 1, st);
         if (e != hipSuccess) return e;
-        return zgemm_native<R>(Y, 0, X, 0, out, 0, out, 0, c2(-1), c2(1), c2(0), d, 1, h->stream);
+        return zgemm_native<R>(Y, 0, X, 0, out, 0, out, 0, c2(-1), c2(1), c2(0), d, 1, st);
     };
     for (unsigned k = 0; k < A; ++k) {
         hipError_t e = comm(ops, ops + (1 + k) * dd, ops + (1 + A + k) * dd);
@@ -385,9 +416,29 @@ Parament_ErrorCode set_hamiltonian(Context<R>* h, const typename ApiCplx<R>::typ
     h->ncoef = ncoef;
     h->hnorm = one_norm(H0, dim);
     for (unsigned k = 0; k < amps; ++k) h->hnorm += one_norm(H1 + k * dd, dim);
-    if (use_magnus && magnus_commutators(h) != hipSuccess) {
+    if (use_magnus && magnus_commutators<R>(h->d_ops, dim, amps, h->stream) != hipSuccess) {
         free_hamiltonian(h);
         return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+    }
+    if constexpr (sizeof(R) == 4) {
+        // FP64 copy for the mixed pipeline: the complex64 inputs widen exactly; the Magnus
+        // commutators are formed again in FP64
+        std::vector<double2> wide((1 + (size_t)amps) * dd);
+        for (size_t i = 0; i < dd; ++i) wide[i] = c2(H0[i].x, H0[i].y);
+        for (size_t i = 0; i < amps * dd; ++i) wide[dd + i] = c2(H1[i].x, H1[i].y);
+        if (hipMalloc(&h->d_ops64, nops * dd * sizeof(double2)) != hipSuccess) {
+            hipGetLastError();
+            h->d_ops64 = nullptr;
+            free_hamiltonian(h);
+            return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
+        }
+        if (hipMemcpyAsync(h->d_ops64, wide.data(), wide.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream) !=
+                hipSuccess ||
+            (use_magnus && magnus_commutators<double>(h->d_ops64, dim, amps, h->stream) != hipSuccess) ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            free_hamiltonian(h);
+            return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
+        }
     }
     if (pa::launch_transpose<R>(h->d_ops + dd, ncoef, (int)dim, h->d_ops + nops * dd, h->stream) != hipSuccess) {
         free_hamiltonian(h);
@@ -421,10 +472,19 @@ template <class R> void mark(Context<R>* h, int i) {
     hipEventRecord(h->ev[i], h->stream);
 }
 
-// Mixed-precision step-wise path (complex128 interface, any dim): the Chebyshev GEMMs and the
-// lowest tree levels run on complex64 data with the products of the selected mode (bf16 three-
-// way split, FP32, or bf16 hi+lo), every GEMM epilogue in FP64; the first-order part
-// (a_0 - 1) I + a_1 s X_j is formed in FP64, and the tree levels above the lowest three in FP64.
+// Mixed-precision pipeline.  FP64 for Hnorm, the Bessel coefficients, the slice Hamiltonians,
+// the first-order part (a_0 - 1) I + a_1 s X_j and the tree levels above the lowest three; the
+// Chebyshev remainder and the lowest tree levels in FP32 arithmetic whose products are bf16
+// three-way split products (FP32 accurate), FP32 products or bf16 hi+lo products.  Inputs are
+// FP64 operators and controls: a complex64 context passes widened copies of its own.
+struct MixedSrc {
+    const double2* ops;
+    const double2* carr;
+    int N, M;
+    double tau;
+    unsigned pts, amps;
+};
+
 int gemm_mode_of(int precision) {
     switch (precision) {
         case PARAMENT_PRECISION_MIXED_FP32: return pa::GEMM_F32;
@@ -435,25 +495,51 @@ int gemm_mode_of(int precision) {
 
 constexpr int kLowTreeLevels = 3;   // tree levels run in the low-precision products
 
-Parament_ErrorCode run_mixed_stepwise(Context<double>* h, const double2* carr_dev, int N, double tau, int M,
-                                      unsigned pts, unsigned amps, double2* out_dev) {
-    const int d = (int)h->dim;
+// The three step-wise working arrays (X, b_k, b_{k+1}) of `bytes` each: the context's, grown
+// together, or three pieces of a fixed workspace.
+template <class R> bool gen_buffers(Context<R>* h, Workspace& ws, size_t bytes, void* out[3]) {
+    if (ws.fixed) {
+        for (int i = 0; i < 3; ++i)
+            if (!(out[i] = ws.get(nullptr, nullptr, bytes))) return false;
+        return true;
+    }
+    if (h->cap_gen < bytes) {
+        for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
+        h->cap_gen = 0;
+        for (auto& p : h->d_gen)
+            if (hipMalloc(&p, bytes) != hipSuccess) {
+                hipGetLastError();
+                for (auto& q : h->d_gen) { hipFree(q); q = nullptr; }
+                return false;
+            }
+        h->cap_gen = bytes;
+    }
+    for (int i = 0; i < 3; ++i) out[i] = h->d_gen[i];
+    return true;
+}
+
+// Step-wise form (any dim): the Clenshaw GEMMs batched over all slices on complex64 data with the
+// products of the context's mode, every GEMM epilogue in FP64.
+template <class R>
+Parament_ErrorCode run_mixed_stepwise(Context<R>* h, const MixedSrc& m, double2* out_dev, Workspace& ws) {
+    const int d = (int)h->dim, N = m.N, M = m.M;
     const size_t dd = (size_t)d * d;
     const int mode = gemm_mode_of(h->precision);
     hipStream_t st = h->stream;
+    void* gen[3];
+    if (!gen_buffers(h, ws, (size_t)N * dd * sizeof(float2), gen)) return ws.failure();
     double are[pa::kMaxCheb], aim[pa::kMaxCheb];
-    bessel(M, tau * h->hnorm, are, aim);
+    bessel(M, m.tau * h->hnorm, are, aim);
     pa::ChebArgs<double> ca;
-    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = d; ca.N = N; ca.M = M; ca.s = 2.0 / h->hnorm;
-    ca.ctl.c = carr_dev; ca.ctl.pts = (int)pts; ca.ctl.amps_in = (int)amps; ca.ctl.amps_set = (int)h->amps;
-    ca.ctl.mode = h->quad; ca.ctl.magnus = h->magnus ? 1 : 0; ca.ctl.tau = tau;
+    ca.ops = m.ops; ca.ncoef = h->ncoef; ca.d = d; ca.N = N; ca.M = M; ca.s = 2.0 / h->hnorm;
+    ca.ctl.c = m.carr; ca.ctl.pts = (int)m.pts; ca.ctl.amps_in = (int)m.amps; ca.ctl.amps_set = (int)h->amps;
+    ca.ctl.mode = h->quad; ca.ctl.magnus = h->magnus ? 1 : 0; ca.ctl.tau = m.tau;
     for (int k = 0; k <= M; ++k) { ca.are[k] = are[k]; ca.aim[k] = aim[k]; }
     ca.are[0] = are[0] - 1.0;
     ca.out = nullptr;
-    // complex64 working arrays inside the (complex128-sized) step-wise buffers
-    float2* X = reinterpret_cast<float2*>(h->d_gen[0]);
-    float2* b1 = reinterpret_cast<float2*>(h->d_gen[1]);
-    float2* b2 = reinterpret_cast<float2*>(h->d_gen[2]);
+    float2* X = reinterpret_cast<float2*>(gen[0]);
+    float2* b1 = reinterpret_cast<float2*>(gen[1]);
+    float2* b2 = reinterpret_cast<float2*>(gen[2]);
     const float2 aM = make_float2((float)are[M], (float)aim[M]);
     const float2 saM = make_float2((float)(ca.s * are[M]), (float)(ca.s * aim[M]));
     mark(h, 0);
@@ -492,9 +578,9 @@ Parament_ErrorCode run_mixed_stepwise(Context<double>* h, const double2* carr_de
     }
     // FP64 levels above: the first one reads the complex64 nodes
     const size_t n64 = (size_t)(r + 1) / 2;
-    if (!grow(&h->d_nodes, &h->cap_nodes, 2 * std::max<size_t>(n64, 1) * dd * sizeof(double2)))
-        return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
-    double2* cur = reinterpret_cast<double2*>(h->d_nodes);
+    void* nodes = ws.get(&h->d_nodes, &h->cap_nodes, 2 * std::max<size_t>(n64, 1) * dd * sizeof(double2));
+    if (!nodes) return ws.failure();
+    double2* cur = reinterpret_cast<double2*>(nodes);
     double2* nxt = cur + std::max<size_t>(n64, 1) * dd;
     if (r == 1) {
         if (pa::launch_widen(rd, d, 1, cur, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
@@ -523,83 +609,128 @@ Parament_ErrorCode run_mixed_stepwise(Context<double>* h, const double2* carr_de
     return PARAMENT_STATUS_SUCCESS;
 }
 
+// Fused form (dims <= 64): one kernel forms each slice on chip and folds the lowest tree levels;
+// the FP64 tree takes its complex64 nodes.
+template <class R>
+Parament_ErrorCode run_mixed_fused(Context<R>* h, const MixedSrc& m, double2* out_dev, Workspace& ws) {
+    const unsigned d = h->dim;
+    const size_t dd = (size_t)d * d;
+    const int N = m.N, M = m.M;
+    pa::MixedArgs ma;
+    ma.ops = m.ops;
+    ma.ncoef = h->ncoef; ma.d = (int)d;
+    ma.ctl.c = m.carr; ma.ctl.pts = (int)m.pts; ma.ctl.amps_in = (int)m.amps;
+    ma.ctl.amps_set = (int)h->amps; ma.ctl.mode = h->quad; ma.ctl.magnus = h->magnus ? 1 : 0; ma.ctl.tau = m.tau;
+    // The low-precision recurrence multiplies by s in FP32: take s = fl32(2 / Hnorm) and expand
+    // with Hnorm' = 2 / s (within 6e-8 of Hnorm) so that every term uses the same, exact s.
+    const float sf = (float)(2.0 / h->hnorm);
+    const double hn2 = 2.0 / (double)sf;
+    double bre[pa::kMaxCheb], bim[pa::kMaxCheb];
+    bessel(M, m.tau * hn2, bre, bim);
+    ma.N = N; ma.M = M; ma.s = (double)sf;
+    for (int k = 0; k <= M; ++k) { ma.are[k] = bre[k]; ma.aim[k] = bim[k]; }
+    ma.are[0] = bre[0] - 1.0;
+    // Small tau Hnorm: evaluate the same degree-M polynomial in powers of Y = tau X, which
+    // needs (M + 1) / 2 products instead of M - 1 (the power coefficients are then close to
+    // (-i)^k / k! and the form is well conditioned).
+    const double xarg = m.tau * hn2;
+    ma.ps = (xarg <= 1.0) ? 1 : 0;
+    if (ma.ps) {
+        power_coefficients(M, xarg, bre, bim, ma.are, ma.aim);
+        ma.ps_levels = power_split_levels(M, xarg);
+    }
+    // the slice kernel folds the lowest F tree levels; the FP64 tree takes its level-F nodes
+    const int F = pa::mixed_fused_levels(N);
+    const int n_nodes = (N + (1 << F) - 1) >> F;
+    const size_t nodes = (size_t)pa::tree_scratch_nodes<double>(n_nodes);
+    void* leaves = ws.get(&h->d_leaves, &h->cap_leaves, (size_t)n_nodes * dd * sizeof(float2));
+    void* scratch_nodes = leaves ? ws.get(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(double2)) : nullptr;
+    if (!scratch_nodes) return ws.failure();
+    ma.out = reinterpret_cast<float2*>(leaves);
+    mark(h, 0);
+    if (pa::launch_cheb_mixed(ma, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    mark(h, 1);
+    double2* nd = reinterpret_cast<double2*>(scratch_nodes);
+    if (pa::launch_tree<double>(leaves, true, n_nodes, (int)d, nd, nd + nodes * dd, out_dev, h->stream) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
+    mark(h, 2);
+    return PARAMENT_STATUS_SUCCESS;
+}
+
+// Whether a call runs the mixed pipeline: a mixed mode on a complex128 context; on a complex64
+// context every mode but NATIVE, except that the default mode keeps the wavefront-resident FP32
+// kernels for dims <= 16 on the automatic path.
+template <class R> bool uses_mixed(const Context<R>* h) {
+    if (h->precision == PARAMENT_PRECISION_NATIVE) return false;
+    if (sizeof(R) == 4 && h->precision == PARAMENT_PRECISION_MIXED && h->dim <= 16 && h->path == 0) return false;
+    return true;
+}
+
 // Runs the whole pipeline on the context's stream; carr_dev / out_dev are device pointers.
 template <class R>
-Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps, cplx<R>* out_dev) {
+Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps, cplx<R>* out_dev,
+                       Workspace& ws) {
     typedef cplx<R> C;
     int N = 0;
     double tau = 0;
     slices(h, pts, dt, &N, &tau);
     if (N <= 0) return PARAMENT_STATUS_INVALID_VALUE;
 
-    const bool mixed_req = (sizeof(R) == 8) && h->precision != PARAMENT_PRECISION_NATIVE;
+    const bool mixed = uses_mixed(h);
     int M = h->mmax;
     if (!h->manual) {
-        M = (sizeof(R) == 4 || mixed_req) ? select_fp32(h->hnorm, tau) : select_fp64(h->hnorm, tau);
+        M = (sizeof(R) == 4 || mixed) ? select_fp32(h->hnorm, tau) : select_fp64(h->hnorm, tau);
         if (M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
-        if (mixed_req) M = mixed_order(M, tau * h->hnorm, N);
+        // a complex128 context in a mixed mode targets 1e-5 against FP64: the truncation error,
+        // the same in every slice, must stay below that over the whole product; a complex64
+        // context keeps the FP32 table's order
+        if (sizeof(R) == 8 && mixed) M = mixed_order(M, tau * h->hnorm, N);
     }
     if (M >= pa::kMaxCheb) return PARAMENT_STATUS_INVALID_VALUE;
     h->last_m = M;
+
+    const unsigned d = h->dim;
+    const size_t dd = (size_t)d * d;
+    hipStream_t st = h->stream;
+
+    if (mixed) {
+        MixedSrc m;
+        m.N = N; m.M = M; m.tau = tau; m.pts = pts; m.amps = amps;
+        double2* out64 = nullptr;
+        if constexpr (sizeof(R) == 8) {
+            m.ops = h->d_ops;
+            m.carr = carr_dev;
+            out64 = out_dev;
+        } else {
+            const size_t n = (size_t)amps * pts;
+            void* c64 = ws.get(&h->d_c64, &h->cap_c64, std::max<size_t>(n, 1) * sizeof(double2));
+            void* o64 = c64 ? ws.get(&h->d_out64, &h->cap_out64, dd * sizeof(double2)) : nullptr;
+            if (!o64) return ws.failure();
+            if (pa::launch_widen(carr_dev, 1, (int)n, reinterpret_cast<double2*>(c64), st) != hipSuccess)
+                return PARAMENT_STATUS_CUBLAS_FAILED;
+            m.ops = h->d_ops64;
+            m.carr = reinterpret_cast<const double2*>(c64);
+            out64 = reinterpret_cast<double2*>(o64);
+        }
+        const bool fused = h->precision == PARAMENT_PRECISION_MIXED && d <= 64 && h->path == 0 && pa::mixed_supported((int)d);
+        const Parament_ErrorCode err = fused ? run_mixed_fused(h, m, out64, ws) : run_mixed_stepwise(h, m, out64, ws);
+        if (err != PARAMENT_STATUS_SUCCESS) return err;
+        if constexpr (sizeof(R) == 4) {
+            if (pa::launch_narrow(out64, (long)dd, out_dev, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        }
+        return PARAMENT_STATUS_SUCCESS;
+    }
+
     double are[pa::kMaxCheb], aim[pa::kMaxCheb];
     bessel(M, tau * h->hnorm, are, aim);
     // Slices are carried as E_j = U_j - I: the last Clenshaw step adds (a_0 - 1) I, formed in FP64.
     const double a0m1 = are[0] - 1.0;
-
-    const unsigned d = h->dim;
-    const size_t dd = (size_t)d * d;
     const bool fused = (d <= 64) && h->path == 0;
-    const bool mixed = mixed_req && h->precision == PARAMENT_PRECISION_MIXED && fused && pa::mixed_supported((int)d);
-
-    pa::Controls<R> ctl;
-    ctl.c = carr_dev; ctl.pts = (int)pts; ctl.amps_in = (int)amps; ctl.amps_set = (int)h->amps;
-    ctl.mode = h->quad; ctl.magnus = h->magnus ? 1 : 0; ctl.tau = (R)tau;
-
-    if (mixed) {
-        pa::MixedArgs ma;
-        ma.ops = reinterpret_cast<const double2*>(h->d_ops);
-        ma.ncoef = h->ncoef; ma.d = (int)d;
-        ma.ctl.c = reinterpret_cast<const double2*>(carr_dev); ma.ctl.pts = (int)pts; ma.ctl.amps_in = (int)amps;
-        ma.ctl.amps_set = (int)h->amps; ma.ctl.mode = h->quad; ma.ctl.magnus = h->magnus ? 1 : 0; ma.ctl.tau = tau;
-        // The low-precision recurrence multiplies by s in FP32: take s = fl32(2 / Hnorm) and expand
-        // with Hnorm' = 2 / s (within 6e-8 of Hnorm) so that every term uses the same, exact s.
-        const float sf = (float)(2.0 / h->hnorm);
-        const double hn2 = 2.0 / (double)sf;
-        double bre[pa::kMaxCheb], bim[pa::kMaxCheb];
-        bessel(M, tau * hn2, bre, bim);
-        ma.N = N; ma.M = M; ma.s = (double)sf;
-        for (int k = 0; k <= M; ++k) { ma.are[k] = bre[k]; ma.aim[k] = bim[k]; }
-        ma.are[0] = bre[0] - 1.0;
-        // Small tau Hnorm: evaluate the same degree-M polynomial in powers of Y = tau X, which
-        // needs (M + 1) / 2 products instead of M - 1 (the power coefficients are then close to
-        // (-i)^k / k! and the form is well conditioned).
-        const double xarg = tau * hn2;
-        ma.ps = (xarg <= 1.0) ? 1 : 0;
-        if (ma.ps) {
-            power_coefficients(M, xarg, bre, bim, ma.are, ma.aim);
-            ma.ps_levels = power_split_levels(M, xarg);
-        }
-        // the slice kernel folds the lowest F tree levels; the FP64 tree takes its level-F nodes
-        const int F = pa::mixed_fused_levels(N);
-        const int n_nodes = (N + (1 << F) - 1) >> F;
-        const size_t nodes = (size_t)pa::tree_scratch_nodes<double>(n_nodes);
-        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)n_nodes * dd * sizeof(float2)) ||
-            !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(double2)))
-            return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
-        ma.out = reinterpret_cast<float2*>(h->d_leaves);
-        mark(h, 0);
-        if (pa::launch_cheb_mixed(ma, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
-        mark(h, 1);
-        double2* nd = reinterpret_cast<double2*>(h->d_nodes);
-        if (pa::launch_tree<double>(h->d_leaves, true, n_nodes, (int)d, nd, nd + nodes * dd,
-                                    reinterpret_cast<double2*>(out_dev), h->stream) != hipSuccess)
-            return PARAMENT_STATUS_CUBLAS_FAILED;
-        mark(h, 2);
-        return PARAMENT_STATUS_SUCCESS;
-    }
 
     pa::ChebArgs<R> ca;
-    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = (int)d; ca.ctl = ctl; ca.N = N; ca.M = M;
+    ca.ops = h->d_ops; ca.ncoef = h->ncoef; ca.d = (int)d; ca.N = N; ca.M = M;
+    ca.ctl.c = carr_dev; ca.ctl.pts = (int)pts; ca.ctl.amps_in = (int)amps; ca.ctl.amps_set = (int)h->amps;
+    ca.ctl.mode = h->quad; ca.ctl.magnus = h->magnus ? 1 : 0; ca.ctl.tau = (R)tau;
     ca.s = (R)(2.0 / h->hnorm);  // the reference casts s and a_k to the working precision (parament.cpp:583-584, 406)
     for (int k = 0; k <= M; ++k) { ca.are[k] = (R)are[k]; ca.aim[k] = (R)aim[k]; }
     ca.are[0] = (R)a0m1;
@@ -610,56 +741,40 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         const int F = small ? pa::small_fold_levels(N) : 0;
         const int n_nodes = (N + (1 << F) - 1) >> F;
         const size_t nodes = (size_t)pa::tree_scratch_nodes<R>(n_nodes);
-        if (!grow(&h->d_leaves, &h->cap_leaves, (size_t)n_nodes * dd * sizeof(C)) ||
-            !grow(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(C)))
-            return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
-        ca.out = reinterpret_cast<C*>(h->d_leaves);
+        void* leaves = ws.get(&h->d_leaves, &h->cap_leaves, (size_t)n_nodes * dd * sizeof(C));
+        void* scratch_nodes = leaves ? ws.get(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(C)) : nullptr;
+        if (!scratch_nodes) return ws.failure();
+        ca.out = reinterpret_cast<C*>(leaves);
         mark(h, 0);
-        if ((small ? pa::launch_small_fold<R>(ca, h->stream) : pa::launch_cheb_fused<R>(ca, h->stream)) != hipSuccess)
+        if ((small ? pa::launch_small_fold<R>(ca, st) : pa::launch_cheb_fused<R>(ca, st)) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         mark(h, 1);
-        C* nd = reinterpret_cast<C*>(h->d_nodes);
-        if (pa::launch_tree<R>(h->d_leaves, false, n_nodes, (int)d, nd, nd + nodes * dd, out_dev, h->stream) !=
-            hipSuccess)
+        C* nd = reinterpret_cast<C*>(scratch_nodes);
+        if (pa::launch_tree<R>(leaves, false, n_nodes, (int)d, nd, nd + nodes * dd, out_dev, st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         mark(h, 2);
         return PARAMENT_STATUS_SUCCESS;
     }
 
     // Step-wise path: X, then M-1 batched GEMM launches, then ceil(log2 N) tree levels.
-    const size_t gen_bytes = (size_t)N * dd * sizeof(C);
-    if (h->cap_gen < gen_bytes) {
-        for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
-        h->cap_gen = 0;
-        for (auto& p : h->d_gen)
-            if (hipMalloc(&p, gen_bytes) != hipSuccess) {
-                hipGetLastError();
-                for (auto& q : h->d_gen) { hipFree(q); q = nullptr; }
-                return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
-            }
-        h->cap_gen = gen_bytes;
-    }
-    hipStream_t st = h->stream;
-    if constexpr (sizeof(R) == 8) {
-        if (mixed_req)
-            return run_mixed_stepwise(h, reinterpret_cast<const double2*>(carr_dev), N, tau, M, pts, amps,
-                                      reinterpret_cast<double2*>(out_dev));
-    }
-    C* X = h->d_gen[0];
+    void* gen[3];
+    if (!gen_buffers(h, ws, (size_t)N * dd * sizeof(C), gen)) return ws.failure();
+    C* X = reinterpret_cast<C*>(gen[0]);
+    C* w1 = reinterpret_cast<C*>(gen[1]);
+    C* w2 = reinterpret_cast<C*>(gen[2]);
     mark(h, 0);
     if (pa::launch_assemble<R>(ca, X, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
     // the slices as E = U - I (a_0 - 1 on the diagonal of the last step)
     double2 ak[pa::kMaxCheb];
     for (int k = 0; k <= M; ++k) ak[k] = c2(k == 0 ? a0m1 : (double)ca.are[k], (double)ca.aim[k]);
     C* leaves = nullptr;
-    if (clenshaw_batched<R>(X, (int)d, N, M, (double)ca.s, ak, h->d_gen[1], h->d_gen[2], nullptr, &leaves, st) !=
-        hipSuccess)
+    if (clenshaw_batched<R>(X, (int)d, N, M, (double)ca.s, ak, w1, w2, nullptr, &leaves, st) != hipSuccess)
         return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 1);
     // E_j = U_j - I for the N slices; reduce level by level (parament.cpp:675-714) on deviations
     // from identity: E' = A + B + A B.
     C* rd = leaves;
-    C* wr = leaves == h->d_gen[1] ? h->d_gen[2] : h->d_gen[1];
+    C* wr = leaves == w1 ? w2 : w1;
     int r = N;
     while (r > 1) {
         const int half = r / 2;
@@ -702,7 +817,8 @@ Parament_ErrorCode equiprop(Context<R>* h, const typename ApiCplx<R>::type* carr
         return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
     if (hipMemcpyAsync(h->d_c, carr, cbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
-    err = run(h, h->d_c, dt, pts, amps, reinterpret_cast<C*>(h->d_root));
+    Workspace ws;
+    err = run(h, h->d_c, dt, pts, amps, reinterpret_cast<C*>(h->d_root), ws);
     if (err == PARAMENT_STATUS_SUCCESS) {
         if (hipMemcpyAsync(out, h->d_root, dd * sizeof(C), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
             hipStreamSynchronize(h->stream) != hipSuccess)
@@ -721,7 +837,8 @@ Parament_ErrorCode equiprop_device(Context<R>* h, const typename ApiCplx<R>::typ
     if (!carr_dev || !out_dev) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
     Parament_ErrorCode err = ensure_common(h, amps);
     if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
-    err = run(h, reinterpret_cast<const cplx<R>*>(carr_dev), dt, pts, amps, reinterpret_cast<cplx<R>*>(out_dev));
+    Workspace ws;
+    err = run(h, reinterpret_cast<const cplx<R>*>(carr_dev), dt, pts, amps, reinterpret_cast<cplx<R>*>(out_dev), ws);
     return h->last_error = err;
 }
 
@@ -978,9 +1095,9 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
                             const cplx<R>* target_dev, R* fid_dev, R* grad_dev) {
     typedef cplx<R> C;
     if (!h->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
-    // dims <= 32: the fused kernels (on-chip slices and tree levels); larger dims: batched GEMMs
-    const bool stepwise = !pa::grad_supported((int)h->dim);
-    if (!stepwise && h->ncoef > pa::kGradCoeffs) return PARAMENT_STATUS_INVALID_VALUE;
+    // dims <= 32 with at most kGradCoeffs coefficient operators: the fused kernels (on-chip slices
+    // and tree levels); larger dims or more operators: batched GEMMs
+    const bool stepwise = !pa::grad_supported((int)h->dim) || h->ncoef > pa::kGradCoeffs;
     if (amps == 0 || amps > h->amps) return PARAMENT_STATUS_INVALID_VALUE;
     if (hipSetDevice(h->device) != hipSuccess) return PARAMENT_FAIL;
     int N = 0;
@@ -1135,6 +1252,14 @@ Parament_ErrorCode gradient_device(Context<R>* h, const typename ApiCplx<R>::typ
     if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
     return h->last_error = gradient(h, reinterpret_cast<const cplx<R>*>(carr_dev), dt, pts, amps,
                                     reinterpret_cast<const cplx<R>*>(target_dev), fid_dev, grad_dev);
+}
+
+template <class R> Parament_ErrorCode set_precision(Context<R>* h, int mode) {
+    if (mode != PARAMENT_PRECISION_NATIVE && mode != PARAMENT_PRECISION_MIXED && mode != PARAMENT_PRECISION_MIXED_FP32 &&
+        mode != PARAMENT_PRECISION_MIXED_BF16X2)
+        return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    h->precision = mode;
+    return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
 
 template <class R> Parament_ErrorCode set_checkpointing(Context<R>* h, unsigned seg, unsigned chunk) {
@@ -1787,6 +1912,7 @@ const char* Parament_errorMessage(Parament_ErrorCode errorCode) {
         case PARAMENT_STATUS_SELECT_SMALLER_DT: return "Timestep too large";
         case PARAMENT_STATUS_INVALID_QUADRATURE_SELECTION: return "Invalid quadrature selection.";
         case PARAMENT_STATUS_NO_HAMILTONIAN: return "No hamiltonian set";
+        case PARAMENT_STATUS_WORKSPACE_TOO_SMALL: return "The problem needs more device memory than the fixed workspace holds.";
         default: return "Unknown error code";
     }
 }
@@ -1826,14 +1952,14 @@ double OneNorm(cuComplex* mat, unsigned int dim) { return mat ? one_norm(mat, di
 double OneNorm_fp64(cuDoubleComplex* mat, unsigned int dim) { return mat ? one_norm(mat, dim) : 0.0; }
 
 // ---- additions ----
-Parament_ErrorCode Parament_setPrecisionMode_fp64(Parament_Context_f64* handle, Parament_PrecisionMode mode) {
-    if (!handle) return PARAMENT_STATUS_INVALID_VALUE;
-    if (mode != PARAMENT_PRECISION_NATIVE && mode != PARAMENT_PRECISION_MIXED && mode != PARAMENT_PRECISION_MIXED_FP32 &&
-        mode != PARAMENT_PRECISION_MIXED_BF16X2)
-        return CTX64(handle)->last_error = PARAMENT_STATUS_INVALID_VALUE;
-    CTX64(handle)->precision = (int)mode;
-    return CTX64(handle)->last_error = PARAMENT_STATUS_SUCCESS;
+Parament_ErrorCode Parament_setPrecisionMode(Parament_Context_f32* handle, Parament_PrecisionMode mode) {
+    return handle ? set_precision(CTX32(handle), (int)mode) : PARAMENT_STATUS_INVALID_VALUE;
 }
+Parament_ErrorCode Parament_setPrecisionMode_fp64(Parament_Context_f64* handle, Parament_PrecisionMode mode) {
+    return handle ? set_precision(CTX64(handle), (int)mode) : PARAMENT_STATUS_INVALID_VALUE;
+}
+int Parament_getPrecisionMode(Parament_Context_f32* handle) { return handle ? CTX32(handle)->precision : -1; }
+int Parament_getPrecisionMode_fp64(Parament_Context_f64* handle) { return handle ? CTX64(handle)->precision : -1; }
 int Parament_getIterationCycles(Parament_Context_f32* handle) { return handle ? CTX32(handle)->last_m : -1; }
 int Parament_getIterationCycles_fp64(Parament_Context_f64* handle) { return handle ? CTX64(handle)->last_m : -1; }
 

@@ -420,6 +420,17 @@ hipError_t launch_widen(const float2* in, int d, int n, double2* out, hipStream_
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void narrow_kernel(const double2* __restrict__ in, long total, float2* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < total) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
+hipError_t launch_narrow(const double2* in, long n, float2* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, out);
+    return hipGetLastError();
+}
+
 // E_j = (a_0 - 1) I + a_1 s X_j + R_j in FP64 (X_j recomputed from the operators), rounded once
 // to complex64: the first-order part of the mixed-precision slice propagator.
 __global__ __launch_bounds__(256) void first_order_kernel(ChebArgs<double> a, const float2* __restrict__ R,

@@ -73,6 +73,8 @@ template <class R> hipError_t launch_fill(const cplx<R>* X, cplx<R> alpha, cplx<
 hipError_t launch_assemble_c64(const ChebArgs<double>& a, float2* X, hipStream_t st);
 // n d x d complex64 matrices -> complex128.
 hipError_t launch_widen(const float2* in, int d, int n, double2* out, hipStream_t st);
+// n complex128 values -> complex64 (rounded to nearest).
+hipError_t launch_narrow(const double2* in, long n, float2* out, hipStream_t st);
 // E_j = (a_0 - 1) I + a_1 s X_j + R_j in FP64 (are[0] = a_0 - 1), stored as complex64.
 hipError_t launch_first_order(const ChebArgs<double>& a, const float2* R, float2* E, hipStream_t st);
 

@@ -6,18 +6,21 @@ from . import parament
 
 
 def expm(m):
-    """Compute the exponential of the matrix `m` via one equiprop slice.
+    """exp(m) of a square matrix of small norm, evaluated on the GPU as one equiprop slice.
 
-    H0 = 1j*m and H1 = m with a zero control give exp(-i * 1j*m) = exp(m).
-    Only works for matrices with small norms.
+    One slice of length 1 under the drift 1j*m propagates exp(-1j * 1j*m) = exp(m).  The matrix
+    is loaded as a zero-amplitude control as well, so that the norm bound (and with it the
+    Chebyshev order) is the one the reference's helper uses.  Complex64 result.
     """
-    dt = 1.0
-    with parament.Parament() as gpu_runner:
-        gpu_runner.set_hamiltonian(1j * m, m, use_magnus=False, quadrature_mode='none')
-        expm_gpu = gpu_runner.equiprop(dt, np.zeros(1))
-    return expm_gpu
+    m = np.asarray(m)
+    ctx = parament.Parament('fp32')
+    try:
+        ctx.set_hamiltonian(1j * m, m, quadrature_mode='none')
+        return ctx.equiprop(1.0, np.zeros(1))
+    finally:
+        ctx.destroy()
 
 
 def device_info():
-    """Print information about the available devices to stdout."""
+    """Print the visible HIP devices (name, architecture, compute units, memory) to stdout."""
     paramentlib.lib.device_info()

@@ -15,7 +15,7 @@ import numpy as np
 from . import paramentlib
 from .constants import (PARAMENT_QUADRATURE_MIDPOINT, PARAMENT_QUADRATURE_NONE, PARAMENT_QUADRATURE_SIMPSON,
                         PARAMENT_STATUS_SUCCESS)
-from .parament import _EXCEPTIONS, _dense
+from .parament import _dense, status_exception
 
 __all__ = ["Ensemble"]
 
@@ -42,10 +42,8 @@ class Ensemble:
                                                       self.pts, self.dt, _QUAD[quadrature_mode]))
 
     def _check(self, code):
-        if code == PARAMENT_STATUS_SUCCESS:
-            return
-        exc = _EXCEPTIONS.get(code, RuntimeError)
-        raise exc(f"Error code {code}: {self._lib.Parament_errorMessage(code).decode()}")
+        if code != PARAMENT_STATUS_SUCCESS:
+            raise status_exception(self._lib, code)
 
     def set_hamiltonians(self, hamiltonians):
         """``hamiltonians[i]`` = (H0, H1, ..., H_amps[i]) for instance i."""

@@ -93,7 +93,10 @@ for sfx, cptr, real in (("", c_cuComplex_p, np.float32), ("_fp64", c_cuDoubleCom
                                                                         ctypes.c_uint]
     getattr(lib, "Parament_getGradientWorkspace" + sfx).argtypes = [c_ParamentContext_p]
     getattr(lib, "Parament_getGradientWorkspace" + sfx).restype = ctypes.c_size_t
-lib.Parament_setPrecisionMode_fp64.argtypes = [c_ParamentContext_p, ctypes.c_int]
+for sfx in ("", "_fp64"):
+    getattr(lib, "Parament_setPrecisionMode" + sfx).argtypes = [c_ParamentContext_p, ctypes.c_int]
+    getattr(lib, "Parament_getPrecisionMode" + sfx).argtypes = [c_ParamentContext_p]
+    getattr(lib, "Parament_getPrecisionMode" + sfx).restype = ctypes.c_int
 _uint_p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 lib.Parament_ensembleCreate.argtypes = [ctypes.POINTER(c_ParamentContext_p), ctypes.c_uint, _uint_p, _uint_p, _uint_p,
                                         np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"), ctypes.c_int]

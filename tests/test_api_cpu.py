@@ -88,6 +88,11 @@ def test_one_norm_agrees_with_oracle():
         assert abs(paramentlib.lib.OneNorm_fp64(np.ascontiguousarray(m.ravel()), d) - eo.one_norm(m)) < 1e-12
         m32 = m.astype(np.complex64)
         assert abs(paramentlib.lib.OneNorm(np.ascontiguousarray(m32.ravel()), d) - eo.one_norm(m32)) < 1e-5
+    # max row sum of the row-major input, not column sums: 5 here, 7 for the transpose
+    m = np.array([[1, 2j], [0, 3 + 4j]])
+    for a, want in ((m, 5.0), (m.T, 7.0)):
+        assert paramentlib.lib.OneNorm_fp64(np.ascontiguousarray(a.ravel()), 2) == want
+        assert paramentlib.lib.OneNorm(np.ascontiguousarray(a.astype(np.complex64).ravel()), 2) == want
 
 
 def test_error_messages_match_reference_strings():
@@ -160,14 +165,20 @@ def test_precision_argument_checks():
     import parament_amd as parament
     with pytest.raises(ValueError):
         parament.Parament('fp16')
+    # complex64 contexts default to the mixed pipeline, complex128 contexts to FP64 throughout
     with parament.Parament() as context:
-        with pytest.raises(ValueError):
-            context.set_precision_mode('mixed')
-    with parament.Parament('fp64') as context:
-        context.set_precision_mode('mixed')
+        assert context.precision_mode == 'mixed'
         context.set_precision_mode('native')
+        assert context.precision_mode == 'native'
+    with parament.Parament('fp64') as context:
+        assert context.precision_mode == 'native'
+        for mode in ('mixed', 'mixed_fp32', 'mixed_bf16x2', 'native'):
+            context.set_precision_mode(mode)
+            assert context.precision_mode == mode
         with pytest.raises(ValueError):
             context.set_precision_mode('bf16')
+    # the C entry point rejects codes outside the enum with INVALID_VALUE
+    assert parament.paramentlib.lib.Parament_setPrecisionMode(None, 1) == 50
 
 
 # ---- additions: argument checks of the ensemble and gradient entry points (host only) ----

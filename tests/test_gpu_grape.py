@@ -232,6 +232,45 @@ def test_gradient_matches_central_differences_dim32_4096_steps():
         assert worst < 1e-3 * scale
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gradient_dim32_4096_steps_window_vs_block_expm(prec):
+    """The GRAPE workload at full size against the exact-derivative oracle (block-matrix expm,
+    the full 4096-slice prefix and suffix products) on a contiguous window of 512 steps in the
+    middle of the sequence, every control: FP64 to 1e-10 and complex64 to 1e-3 of the largest
+    component (the Chebyshev truncation at tau Hnorm = 0.03 is far below both)."""
+    from parament_amd import workloads
+    h0, hk, c, dt, quad, target = workloads.gue32_grape()
+    f, grad = gpu_grad(prec, h0, hk, c, dt, quad, target)
+    win = range(1792, 2304)
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, dt, quad, target, slices=win)
+    g, r = grad[:, win.start:win.stop], grad_ref[:, win.start:win.stop]
+    err = np.abs(g - r).max() / np.abs(r).max()
+    print(f"{prec}: window max error / max |grad| = {err:.3e}, F {f:.9e} vs {f_ref:.9e}")
+    if prec == "fp64":
+        assert abs(f - f_ref) < 1e-12
+        assert err < 1e-10
+    else:
+        assert abs(f - f_ref) < 1e-4 * f_ref
+        assert err < 1e-3
+
+
+@pytest.mark.parametrize("amps,magnus", [(10, True), (70, False)])
+def test_gradient_many_coefficient_operators_small_dim(amps, magnus):
+    """More coefficient operators than the fused gradient kernels hold (64) at a small dim: 10
+    Magnus controls carry 10 + 10 + 45 = 65 operators, 70 plain controls 70; the batched-GEMM
+    gradient takes them, against the exact-derivative oracle (Magnus slices included)."""
+    h0, hk, c, t = _problem(4800 + amps, 8, amps, 9)
+    hk = [m / amps for m in hk]
+    mode = "simpson" if magnus else "none"
+    import parament_amd as parament
+    with parament.Parament("fp64") as p:
+        p.set_hamiltonian(h0, *hk, use_magnus=magnus, quadrature_mode=mode)
+        f, grad = p.gate_fidelity_gradient(0.05, *c, target=t)
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, 0.05, mode, t, use_magnus=magnus)
+    assert abs(f - f_ref) < 1e-12
+    assert np.abs(grad - grad_ref).max() < 1e-10 * max(1.0, np.abs(grad_ref).max())
+
+
 def test_complex64_gradient_at_dim32_4096_steps():
     """complex64 gradient of the same workload against the FP64 adjoint gradient (which the
     tests above pin to the block-expm oracle and to finite differences)."""

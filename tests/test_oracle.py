@@ -10,6 +10,39 @@ from oracle.expm_stepwise import expm_stepwise
 from tests.helpers import gue, random_hamiltonians, rel_err
 
 
+# ---------------- OneNorm and Hnorm (mathhelper.cpp:82-114, parament.cpp:280-284) ----------------
+
+def test_one_norm_is_the_max_row_sum():
+    """A non-Hermitian case whose max row sum (5), max column sum (7), Frobenius norm (5.48) and
+    spectral norm (5.40) all differ: the reference sums the moduli along each row of the
+    row-major input (mathhelper.cpp:88-96)."""
+    m = np.array([[1, 2j], [0, 3 + 4j]])
+    assert eo.one_norm(m) == 5.0
+    assert eo.one_norm(m.T) == 7.0   # the transpose swaps rows and columns
+    assert eo.one_norm(m.astype(np.complex64)) == 5.0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_one_norm_equals_infinity_norm(seed):
+    """Against numpy's induced infinity norm on random non-Hermitian complex matrices of odd and
+    even dims; complex64 input takes each modulus in single precision (within 6e-8 of it)."""
+    rng = np.random.default_rng(seed)
+    d = 3 + 5 * seed
+    m = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)) * (1 + seed)
+    assert eo.one_norm(m) == pytest.approx(np.linalg.norm(m, np.inf), rel=1e-15)
+    assert abs(eo.one_norm(m) - np.linalg.norm(m, 1)) > 1e-6   # not the column sums
+    m32 = m.astype(np.complex64)
+    assert eo.one_norm(m32) == pytest.approx(np.linalg.norm(m32.astype(np.complex128), np.inf), rel=1e-7)
+
+
+def test_hamiltonian_norm_sums_the_operators():
+    """Hnorm = ||H0|| + sum_k ||H_k||, each a max row sum: 5 + 7 + 2."""
+    m = np.array([[1, 2j], [0, 3 + 4j]])
+    assert eo.hamiltonian_norm(m, [m.T, 2 * np.eye(2)]) == 14.0
+    # the reference docstring case: |diag(1, -1)| + |sigma_x| = 2
+    assert eo.hamiltonian_norm(np.diag([1.0, -1.0]), [np.array([[0, 1], [1, 0]])]) == 2.0
+
+
 # ---------------- iteration tables (parament.cpp:723-766) ----------------
 
 @pytest.mark.parametrize("table,eps", [(eo.FP32_TABLE, np.finfo(np.float32).eps),
@@ -339,3 +372,61 @@ def test_grape_oracle_matches_central_differences(mode):
             cm[k, p] -= hstep
             fd = (fidelity(h[0], h[1:], cp, 0.2, mode, t) - fidelity(h[0], h[1:], cm, 0.2, mode, t)) / (2 * hstep)
             assert abs(fd - grad[k, p]) < 1e-8, (k, p, fd, grad[k, p])
+
+
+def test_grape_oracle_magnus_matches_central_differences():
+    """Magnus slices: the exact-derivative oracle against central differences of the stepwise
+    exponential of the Magnus exponents, for every sample of three controls (pair terms
+    included) and an odd and an even sample count."""
+    from oracle.grape_oracle import fidelity, fidelity_gradient
+    from parament_amd.workloads import haar_unitary
+    rng = np.random.default_rng(23)
+    h = [gue(rng, 3) for _ in range(4)]
+    t = haar_unitary(rng, 3)
+    for pts in (9, 8):
+        c = rng.uniform(-1, 1, (3, pts))
+        f, grad = fidelity_gradient(h[0], h[1:], c, 0.2, "simpson", t, use_magnus=True)
+        assert abs(f - fidelity(h[0], h[1:], c, 0.2, "simpson", t, use_magnus=True)) < 1e-13
+        hstep = 1e-6
+        for k in range(3):
+            for p in range(pts):
+                cp, cm = c.copy(), c.copy()
+                cp[k, p] += hstep
+                cm[k, p] -= hstep
+                fd = (fidelity(h[0], h[1:], cp, 0.2, "simpson", t, True)
+                      - fidelity(h[0], h[1:], cm, 0.2, "simpson", t, True)) / (2 * hstep)
+                assert abs(fd - grad[k, p]) < 1e-8, (pts, k, p, fd, grad[k, p])
+        if pts % 2 == 0:
+            assert np.all(grad[:, -1] == 0)   # the sample Simpson drops
+
+
+def test_grape_oracle_window_is_the_restricted_sum():
+    """The slice window sums exactly the chosen slices' contributions: two windows that split
+    the slices add up to the full gradient."""
+    from oracle.grape_oracle import fidelity_gradient
+    from parament_amd.workloads import haar_unitary
+    rng = np.random.default_rng(29)
+    h = [gue(rng, 4) for _ in range(3)]
+    c = rng.uniform(-1, 1, (2, 12))
+    t = haar_unitary(rng, 4)
+    for mode, magnus, n in (("none", False, 12), ("simpson", False, 5), ("simpson", True, 5)):
+        _, full = fidelity_gradient(h[0], h[1:], c, 0.1, mode, t, use_magnus=magnus)
+        _, a = fidelity_gradient(h[0], h[1:], c, 0.1, mode, t, use_magnus=magnus, slices=range(0, 3))
+        _, b = fidelity_gradient(h[0], h[1:], c, 0.1, mode, t, use_magnus=magnus, slices=range(3, n))
+        assert np.allclose(a + b, full, atol=1e-15)
+
+
+@pytest.mark.parametrize("mode,magnus,pts", [("none", False, 37), ("midpoint", False, 38), ("simpson", False, 41),
+                                             ("simpson", False, 40), ("simpson", True, 41)])
+def test_parallel_expm_product_equals_sequential(mode, magnus, pts):
+    """The block-parallel truth oracle: each block forms its own slices from its sample window
+    (shared boundary samples included), so the product equals the sequential one up to the
+    association of the matrix products."""
+    from oracle.expm_stepwise import expm_stepwise_parallel
+    rng = np.random.default_rng(31)
+    h = [gue(rng, 5) for _ in range(3)]
+    c = rng.uniform(-1, 1, (2, pts))
+    seq = expm_stepwise(h[0], h[1:], c, 0.05, mode, magnus)
+    for workers in (1, 3, 7):
+        par = expm_stepwise_parallel(h[0], h[1:], c, 0.05, mode, magnus, workers=workers)
+        assert rel_err(par, seq) < 1e-14
