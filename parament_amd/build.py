@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libparament.so")
-SOURCES = ["host.cpp", "kernels.hip", "mixed.hip", "grad.hip", "gemm.hip", "ensemble.hip"]
+SOURCES = ["host.cpp", "kernels.hip", "mixed.hip", "mixed_h2.hip", "grad.hip", "gemm.hip", "ensemble.hip"]
 ARCH = os.environ.get("PARAMENT_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 VALU instructions are disabled for all device code.
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -41,7 +41,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(7, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

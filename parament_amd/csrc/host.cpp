@@ -5,6 +5,7 @@
 // messages, iteration tables, Bessel coefficients and slice counts.  The device work is
 // done by the kernels in kernels.hip / mixed.hip on one HIP stream per context.
 #include <math.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -125,6 +126,32 @@ int power_split_levels(int m, double x) {
     return bits;
 }
 
+// The same for the fp16 two-way split form: a product on the leading parts only has relative error
+// ~2^-11; bits 2j = 0 select it, 1 the full product.
+int power_split_levels_h2(int m, double x) {
+    int bits = 0;
+    double w = x * x / 2.0;
+    for (int j = 1; j < (m - 1) / 2; ++j) {
+        w *= x * x / ((2.0 * j + 1.0) * (2.0 * j + 2.0));
+        bits |= (w * 0x1p-11 <= 1e-8 ? 0 : 1) << (2 * j);
+    }
+    return bits;
+}
+
+// Split scheme of the fused mixed kernel's power form: 3 (default) or 4 = scaled fp16 two-way
+// split with three or four products per FP32-accurate product (mixed_h2.hip), 0 = bf16 three-way
+// split with six (mixed.hip).  PARAMENT_MIXED_SPLIT = f16x3 | f16x4 | bf16x3 selects one, for
+// comparisons.
+int mixed_split_scheme() {
+    static const int scheme = [] {
+        const char* v = getenv("PARAMENT_MIXED_SPLIT");
+        if (v && !strcmp(v, "bf16x3")) return 0;
+        if (v && !strcmp(v, "f16x4")) return 4;
+        return 3;
+    }();
+    return scheme;
+}
+
 // Chebyshev order of the mixed-precision path.  The FP32 table bounds the per-slice tail
 // 2 sum_{k>M} |J_k(x)| at ~0.05 eps32; that truncation error is the same for every slice and
 // adds up over the product, so the mixed path (whose target is 1e-5 against FP64) also
@@ -151,6 +178,18 @@ struct GradKey {
     int quad, ncoef, M, device;
     GradKey() { memset(this, 0, sizeof(*this)); }
     bool operator==(const GradKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+// Everything an equiprop call passes to its kernels: the arguments, the settings that choose M,
+// the kernels and the coefficients, and the addresses of the operators and of every working
+// buffer.  Zero-filled before use so that byte comparison is exact.
+struct PropKey {
+    const void *carr, *out, *ops, *ops64, *leaves, *nodes, *gen0, *gen1, *gen2, *c64, *out64, *stream;
+    double dt, hnorm;
+    unsigned pts, amps, amps_set, dim;
+    int quad, magnus, ncoef, precision, path, manual, mmax, device;
+    PropKey() { memset(this, 0, sizeof(*this)); }
+    bool operator==(const PropKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
 template <class R> struct Context {
@@ -202,6 +241,9 @@ template <class R> struct Context {
     // argument that reaches a kernel (GradKey), replayed while they keep agreeing
     GradKey grad_key, grad_prev;
     hipGraphExec_t grad_exec = nullptr;
+    // the same for equiprop: repeated identical calls replay a captured graph of the launches
+    PropKey prop_key, prop_prev;
+    hipGraphExec_t prop_exec = nullptr;
     hipStream_t cap_stream = nullptr;
 };
 
@@ -350,6 +392,7 @@ template <class R> Parament_ErrorCode destroy(Context<R>* h) {
         for (auto& e : h->ev)
             if (e) hipEventDestroy(e);
         if (h->grad_exec) hipGraphExecDestroy(h->grad_exec);
+        if (h->prop_exec) hipGraphExecDestroy(h->prop_exec);
         if (h->cap_stream) hipStreamDestroy(h->cap_stream);
         hipStreamDestroy(h->own_stream);
     }
@@ -647,8 +690,11 @@ Parament_ErrorCode run_mixed_fused(Context<R>* h, const MixedSrc& m, double2* ou
     void* scratch_nodes = leaves ? ws.get(&h->d_nodes, &h->cap_nodes, 2 * nodes * dd * sizeof(double2)) : nullptr;
     if (!scratch_nodes) return ws.failure();
     ma.out = reinterpret_cast<float2*>(leaves);
+    const int split = mixed_split_scheme();
+    if (ma.ps && split) ma.ps_levels = power_split_levels_h2(M, xarg);
     mark(h, 0);
-    if (pa::launch_cheb_mixed(ma, h->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    if ((ma.ps && split ? pa::launch_cheb_h2(ma, split, h->stream) : pa::launch_cheb_mixed(ma, h->stream)) != hipSuccess)
+        return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 1);
     double2* nd = reinterpret_cast<double2*>(scratch_nodes);
     if (pa::launch_tree<double>(leaves, true, n_nodes, (int)d, nd, nd + nodes * dd, out_dev, h->stream) != hipSuccess)
@@ -795,6 +841,58 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     return PARAMENT_STATUS_SUCCESS;
 }
 
+// run() on the context's own buffers.  A call whose key (PropKey) equals the previous call's is
+// captured as a HIP graph on a capture stream and replayed by later calls with that key: the
+// launches and their arguments are then issued with one graph launch (small problems are bound by
+// that issue cost).  The key holds every buffer address, so a call that grew a buffer in between
+// never replays a graph with stale addresses.  Timing mode always launches directly.
+template <class R>
+Parament_ErrorCode run_replayable(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps,
+                                  cplx<R>* out_dev) {
+    Workspace ws;
+    if (h->timing) return run(h, carr_dev, dt, pts, amps, out_dev, ws);
+    PropKey key;
+    key.carr = carr_dev; key.out = out_dev; key.ops = h->d_ops; key.ops64 = h->d_ops64;
+    key.leaves = h->d_leaves; key.nodes = h->d_nodes; key.gen0 = h->d_gen[0]; key.gen1 = h->d_gen[1];
+    key.gen2 = h->d_gen[2]; key.c64 = h->d_c64; key.out64 = h->d_out64; key.stream = h->stream;
+    key.dt = dt; key.hnorm = h->hnorm; key.pts = pts; key.amps = amps; key.amps_set = h->amps; key.dim = h->dim;
+    key.quad = h->quad; key.magnus = h->magnus ? 1 : 0; key.ncoef = h->ncoef; key.precision = h->precision;
+    key.path = h->path; key.manual = h->manual ? 1 : 0; key.mmax = h->mmax; key.device = h->device;
+    if (h->prop_exec && key == h->prop_key)
+        return hipGraphLaunch(h->prop_exec, h->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+    if (!(key == h->prop_prev)) {   // first call with this key: launch directly
+        h->prop_prev = key;
+        return run(h, carr_dev, dt, pts, amps, out_dev, ws);
+    }
+    if (!h->cap_stream && hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        hipGetLastError();
+        h->cap_stream = nullptr;
+        return run(h, carr_dev, dt, pts, amps, out_dev, ws);
+    }
+    if (h->prop_exec) { hipGraphExecDestroy(h->prop_exec); h->prop_exec = nullptr; }
+    hipGraph_t g = nullptr;
+    hipStream_t own = h->stream;
+    h->stream = h->cap_stream;
+    Parament_ErrorCode err = PARAMENT_STATUS_CUBLAS_FAILED;
+    if (hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        err = run(h, carr_dev, dt, pts, amps, out_dev, ws);
+        const hipError_t r = hipStreamEndCapture(h->cap_stream, &g);
+        if (err == PARAMENT_STATUS_SUCCESS && r == hipSuccess && g &&
+            hipGraphInstantiate(&h->prop_exec, g, nullptr, nullptr, 0) != hipSuccess)
+            h->prop_exec = nullptr;
+        if (g) hipGraphDestroy(g);
+    }
+    hipGetLastError();
+    h->stream = own;
+    if (err != PARAMENT_STATUS_SUCCESS || !h->prop_exec) {   // capture unavailable: launch directly
+        h->prop_exec = nullptr;
+        Workspace direct;
+        return run(h, carr_dev, dt, pts, amps, out_dev, direct);
+    }
+    h->prop_key = key;
+    return hipGraphLaunch(h->prop_exec, h->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
+}
+
 template <class R> Parament_ErrorCode ensure_common(Context<R>* h, unsigned amps) {
     if (!h->has_h) return PARAMENT_STATUS_NO_HAMILTONIAN;
     if (amps > h->amps || amps == 0) return PARAMENT_STATUS_INVALID_VALUE;
@@ -817,8 +915,7 @@ Parament_ErrorCode equiprop(Context<R>* h, const typename ApiCplx<R>::type* carr
         return h->last_error = PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
     if (hipMemcpyAsync(h->d_c, carr, cbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         return h->last_error = PARAMENT_STATUS_CUBLAS_FAILED;
-    Workspace ws;
-    err = run(h, h->d_c, dt, pts, amps, reinterpret_cast<C*>(h->d_root), ws);
+    err = run_replayable(h, h->d_c, dt, pts, amps, reinterpret_cast<C*>(h->d_root));
     if (err == PARAMENT_STATUS_SUCCESS) {
         if (hipMemcpyAsync(out, h->d_root, dd * sizeof(C), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
             hipStreamSynchronize(h->stream) != hipSuccess)
@@ -837,8 +934,7 @@ Parament_ErrorCode equiprop_device(Context<R>* h, const typename ApiCplx<R>::typ
     if (!carr_dev || !out_dev) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
     Parament_ErrorCode err = ensure_common(h, amps);
     if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
-    Workspace ws;
-    err = run(h, reinterpret_cast<const cplx<R>*>(carr_dev), dt, pts, amps, reinterpret_cast<cplx<R>*>(out_dev), ws);
+    err = run_replayable(h, reinterpret_cast<const cplx<R>*>(carr_dev), dt, pts, amps, reinterpret_cast<cplx<R>*>(out_dev));
     return h->last_error = err;
 }
 

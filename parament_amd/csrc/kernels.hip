@@ -306,6 +306,67 @@ __global__ __launch_bounds__(256) void tree_split_kernel(const SrcT* __restrict_
     }
 }
 
+// Dims <= 16: block reductions of up to 64 nodes per work-group.  The block's nodes sit in LDS and
+// are reduced level by level, each product's d x d outputs spread over the threads (vector units:
+// an 8 x 8 complex product is 512 multiply-adds, too small for a matrix-core tile), with the
+// reference's pairing (later node on the left) and zero nodes past the end, which return their
+// partner bit for bit (the odd-carry rule).  Every thread reads its operands before the barrier
+// that precedes the in-place write of the level's results.  Two launches reduce up to 4096 nodes.
+constexpr int kSmallTreeMaxPer = 8;   // outputs per thread and level (B * d * d <= 16 * 256)
+
+__host__ __device__ inline int small_tree_block(int d) {
+    int b = 64;
+    while (b > 1 && b * d * d > 2 * kSmallTreeMaxPer * 256) b >>= 1;
+    return b;
+}
+
+template <class R, class SrcT>
+__global__ __launch_bounds__(256) void small_tree_kernel(const SrcT* __restrict__ src, int n_in, int d, int B,
+                                                         cplx<R>* __restrict__ dst, int final_level) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char small_tree_lds[];
+    cplx<R>* nd = reinterpret_cast<cplx<R>*>(small_tree_lds);
+    const int dd = d * d;
+    const int first = blockIdx.x * B;
+    const int live = min(B, n_in - first);
+    for (int e = threadIdx.x; e < B * dd; e += 256) {
+        cplx<R> v = mk<R>(0, 0);
+        if (e < live * dd) {
+            const SrcT x = src[(size_t)first * dd + e];
+            v = mk<R>(R(x.x), R(x.y));
+        }
+        nd[e] = v;
+    }
+    __syncthreads();
+    for (int cnt = B; cnt > 1; cnt >>= 1) {
+        const int total = (cnt >> 1) * dd;
+        cplx<R> res[kSmallTreeMaxPer];
+#pragma unroll
+        for (int t = 0; t < kSmallTreeMaxPer; ++t) {
+            const int e = threadIdx.x + 256 * t;
+            if (e < total) {
+                const int p = e / dd, ik = e - p * dd, i = ik / d, k = ik - i * d;
+                const cplx<R>* A = nd + (size_t)(2 * p + 1) * dd;   // later node
+                const cplx<R>* Bm = nd + (size_t)(2 * p) * dd;      // earlier node
+                cplx<R> acc = cadd(A[ik], Bm[ik]);
+                for (int m = 0; m < d; ++m) acc = cadd(acc, cmul(A[i * d + m], Bm[m * d + k]));
+                res[t] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kSmallTreeMaxPer; ++t) {
+            const int e = threadIdx.x + 256 * t;
+            if (e < total) nd[e] = res[t];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < dd; e += 256) {
+        cplx<R> v = nd[e];
+        if (final_level && e / d == e % d) v.x += R(1);
+        dst[(size_t)blockIdx.x * dd + e] = v;
+    }
+}
+
 template <class R> constexpr int tree_logb() { return 3; }
 // levels with at most this many nodes go through tree_split_kernel
 constexpr int kTreeSplitNodes = 1024;
@@ -338,6 +399,33 @@ template <class R>
 hipError_t launch_tree(const void* leaves, bool leaves_are_c64, int N, int d, cplx<R>* scratch0, cplx<R>* scratch1,
                        cplx<R>* root, hipStream_t st) {
     if (d <= 0 || d > 64 || N <= 0) return hipErrorInvalidValue;
+    if (d <= 16) {
+        // block reductions in LDS: ceil(n / B) nodes per launch
+        const int B = small_tree_block(d);
+        cplx<R>* bufs[2] = {scratch0, scratch1};
+        int which = 0, n = N;
+        bool first = true;
+        while (true) {
+            const int n_out = (n + B - 1) / B;
+            const bool fin = n_out == 1;
+            int b = B;   // a single block needs only the next power of two above its node count
+            if (fin) while (b > 1 && (b >> 1) >= n) b >>= 1;
+            cplx<R>* out = fin ? root : bufs[which];
+            const size_t lds = (size_t)b * d * d * sizeof(cplx<R>);
+            if (first && leaves_are_c64)
+                hipLaunchKernelGGL((small_tree_kernel<R, float2>), dim3(n_out), dim3(256), lds, st,
+                                   reinterpret_cast<const float2*>(leaves), n, d, b, out, fin ? 1 : 0);
+            else
+                hipLaunchKernelGGL((small_tree_kernel<R, cplx<R>>), dim3(n_out), dim3(256), lds, st,
+                                   first ? reinterpret_cast<const cplx<R>*>(leaves) : bufs[which ^ 1], n, d, b, out,
+                                   fin ? 1 : 0);
+            if (fin) break;
+            first = false;
+            n = n_out;
+            which ^= 1;
+        }
+        return hipGetLastError();
+    }
     constexpr int B = 1 << tree_logb<R>();
     // a level of n nodes: B-fold blocks on one work-group each, or one pairwise level split over
     // work-groups once few nodes remain; returns the node count of the level it writes

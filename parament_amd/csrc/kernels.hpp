@@ -46,6 +46,9 @@ struct MixedArgs {
 // Fused Chebyshev slice kernel, dims <= 64 (padded to 16 / 32 / 64).
 template <class R> hipError_t launch_cheb_fused(const ChebArgs<R>& a, hipStream_t st);
 hipError_t launch_cheb_mixed(const MixedArgs& a, hipStream_t st);
+// The power form of the mixed kernel on scaled fp16 two-way split products (mixed_h2.hip): full =
+// 3 or 4 fp16 products per FP32-accurate product; a.ps must be set.
+hipError_t launch_cheb_h2(const MixedArgs& a, int full, hipStream_t st);
 
 // Small dims (d <= 16, ncoef <= 64, operators within 48 KB): the slices of a.j0 .. a.j0 + N - 1
 // with the lowest small_fold_levels(N) tree levels folded in; a.out receives the level-F nodes

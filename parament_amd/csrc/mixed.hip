@@ -557,8 +557,8 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
         __syncthreads();
     };
 
-    // out = A + B + A B for the later node A and the earlier node B (split product).
-    auto combine = [&](const Tile& A, const Tile& B, Tile& out) __attribute__((always_inline)) {
+    // A := A + B + A B for the later node A and the earlier node B (split product, in place).
+    auto combine = [&](Tile& A, const Tile& B) __attribute__((always_inline)) {
         write_split<D, false>(A, Xp, row0, col0);
         write_split<D, true>(B, Bp, row0, col0);
         __syncthreads();
@@ -568,8 +568,8 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
         for (int n = 0; n < TN; ++n)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                out.re[n][v] = (A.re[n][v] + B.re[n][v]) + (acc.re[0][n][v] + acc.re[1][n][v]);
-                out.im[n][v] = (A.im[n][v] + B.im[n][v]) + (acc.im[0][n][v] + acc.im[1][n][v]);
+                A.re[n][v] = (A.re[n][v] + B.re[n][v]) + (acc.re[0][n][v] + acc.re[1][n][v]);
+                A.im[n][v] = (A.im[n][v] + B.im[n][v]) + (acc.im[0][n][v] + acc.im[1][n][v]);
             }
         __syncthreads();
     };
@@ -582,53 +582,47 @@ __global__ __launch_bounds__(64 * MShape<D>::G) void cheb_mixed_kernel(MixedArgs
             const int nb = (w + (int)gridDim.x) * NB;
             return nb < a.N ? nb : -1;
         };
-        // Binary counter over the block's slices p = 0 .. 2^F - 1 (unrolled, so every stack index is
-        // a constant): slice p completes the level-l nodes for the trailing one bits of p, each
-        // the product of the pending earlier sibling stk[l] and the newer node (later on the left).
-        Tile stk[F > 0 ? F : 1];
-        Tile node;
-#pragma unroll
+        // The block's slices run one after another (a loop: the slice code is emitted once).  After
+        // slice p the carry chain over the levels l (unrolled: every pending node stays in fixed
+        // registers) combines with the pending earlier sibling pend[l] while bit l of p is set and
+        // otherwise parks the node in pend[l].  Slices past N are zero nodes, which a combine
+        // returns unchanged (the reference's odd carry).
+        Tile pend[F > 0 ? F : 1];
+#pragma unroll 1
         for (int p = 0; p < NB; ++p) {
             const int j = w * NB + p;
+            Tile e;
             if (j < a.N) {   // uniform
-                Tile e;
                 slice(j, next_of(j), e);
-                int merged = 0;   // levels completed by slice p: the trailing one bits of p
+            } else {
 #pragma unroll
-                for (int l = 0; l < F; ++l) {
-                    const int mask = (2 << l) - 1;
-                    if ((p & mask) == mask) { Tile m; combine(e, stk[l], m); e = m; merged = l + 1; }
-                }
-                if (merged == F) {
-                    node = e;
-                } else {
+                for (int n = 0; n < TN; ++n)
 #pragma unroll
-                    for (int l = 0; l < F; ++l)
-                        if (l == merged) stk[l] = e;
-                }
+                    for (int v = 0; v < 4; ++v) { e.re[n][v] = 0.f; e.im[n][v] = 0.f; }
             }
-        }
-        // a ragged block: the pending nodes stk[l] for the set bits l of the live count fold from
-        // the lowest level up (each earlier than everything folded before it)
-        const int live = min(a.N - w * NB, NB);
-        if (live < NB) {
-            bool have = false;
+            bool carry = true;
 #pragma unroll
             for (int l = 0; l < F; ++l) {
-                if ((live >> l) & 1) {
-                    if (!have) { node = stk[l]; have = true; }
-                    else { Tile m; combine(node, stk[l], m); node = m; }
+                if (carry) {
+                    if ((p >> l) & 1) {
+                        combine(e, pend[l]);
+                    } else {
+                        pend[l] = e;
+                        carry = false;
+                    }
                 }
             }
-        }
-        float2* out = a.out + (size_t)w * dd;
+            if (carry) {   // p = 2^F - 1: e is the block's node
+                float2* out = a.out + (size_t)w * dd;
 #pragma unroll
-        for (int n = 0; n < TN; ++n) {
-            const int col = mcol(col0, n);
+                for (int n = 0; n < TN; ++n) {
+                    const int col = mcol(col0, n);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = mrow(row0, v);
-                if (row < d && col < d) out[row * d + col] = make_float2(node.re[n][v], node.im[n][v]);
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = mrow(row0, v);
+                        if (row < d && col < d) out[row * d + col] = make_float2(e.re[n][v], e.im[n][v]);
+                    }
+                }
             }
         }
     }

@@ -465,3 +465,48 @@ def test_caller_streams_and_concurrent_contexts():
         torch.cuda.synchronize()
         for p in ctxs:
             p.destroy()
+
+
+@pytest.mark.parametrize("prec,d,mode,quad", [("fp32", 8, None, "none"), ("fp64", 8, None, "none"),
+                                             ("fp64", 64, "mixed", "simpson"), ("fp32", 40, None, "simpson"),
+                                             ("fp64", 72, None, "midpoint"), ("fp32n", 16, None, "none")])
+def test_repeated_calls_replay_identically(prec, d, mode, quad):
+    """Repeated identical calls are captured as a graph from the third call on and replayed: every
+    replay equals the direct call bit for bit (with other work and synchronisations between the
+    replays); a replay reads the controls' current contents; a larger call in between (which grows
+    the working buffers) does not leave a replay pointing at the old ones."""
+    import torch
+    import parament_amd as parament
+    h0, hks, c = case(2100 + d, d, 2, 257)
+    ctype = torch.complex64 if prec.startswith("fp32") else torch.complex128
+    with parament.Parament("fp32" if prec == "fp32n" else prec) as p:
+        if prec == "fp32n":
+            p.set_precision_mode("native")
+        if mode:
+            p.set_precision_mode(mode)
+        p.set_hamiltonian(h0, *hks, quadrature_mode=quad)
+        direct = p.equiprop(0.01, *c)
+        m = p.iteration_cycles
+        tol = 1e-5 if mode == "mixed" else TOL[prec]   # the mixed mode's bar, at its own M
+        assert rel_err(direct, oracle_for(prec, h0, hks, c, 0.01, quad, cycles=m)) < tol
+        for rep in range(5):
+            u = p.equiprop(0.01, *c)
+            assert np.array_equal(u, direct), rep
+            torch.cuda.synchronize()
+            _ = (torch.ones(1000, device="cuda:0") * rep).sum().item()   # unrelated device work in between
+        # device pointers: in-place changes of the controls between replays are seen
+        carr = torch.tensor(c, dtype=ctype, device="cuda:0")
+        out = torch.empty(d * d, dtype=ctype, device="cuda:0")
+        for rep in range(4):
+            cc = c * (1.0 + 0.1 * rep)
+            carr.copy_(torch.tensor(cc, dtype=ctype))
+            p.equiprop_device(0.01, carr.data_ptr(), c.shape[1], c.shape[0], out.data_ptr())
+            p.synchronize()
+            u = out.cpu().numpy().reshape(d, d)
+            assert rel_err(u, oracle_for(prec, h0, hks, cc, 0.01, quad, cycles=m)) < tol, rep
+        # a larger call grows the buffers; the original call afterwards still matches the direct result
+        big = np.tile(c, (1, 4))
+        ub = p.equiprop(0.01, *big)
+        assert rel_err(ub, oracle_for(prec, h0, hks, big, 0.01, quad, cycles=p.iteration_cycles)) < tol
+        for rep in range(3):
+            assert np.array_equal(p.equiprop(0.01, *c), direct), rep
