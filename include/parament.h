@@ -160,6 +160,18 @@ LIBSPEC Parament_ErrorCode Parament_equipropDevice(Parament_Context_f32 *handle,
     unsigned int pts, unsigned int amps, cuComplex *out_dev);
 LIBSPEC Parament_ErrorCode Parament_equipropDevice_fp64(Parament_Context_f64 *handle, const cuDoubleComplex *carr_dev,
     double dt, unsigned int pts, unsigned int amps, cuDoubleComplex *out_dev);
+/* Allocation-free variant of Parament_equipropDevice_fp64.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, arithmetic and result as
+ * Parament_equipropDevice_fp64 (every precision mode and kernel path), but the call's working
+ * memory comes from a fixed 256 MiB device workspace that the library holds on each GPU it runs
+ * on until the process ends, and it allocates nothing.  A problem whose working set exceeds the
+ * workspace returns PARAMENT_STATUS_WORKSPACE_TOO_SMALL (e.g. dim 64 at 65,536 steps fits in
+ * the mixed mode, about 3,000 slices in FP64).  All calls on one GPU share the workspace: calls
+ * that overlap on one GPU (other streams, other contexts, other threads) race.  Asynchronous on
+ * the context's stream, like equipropDevice; setHamiltonian must have run (it allocates). */
+LIBSPEC Parament_ErrorCode Parament_equipropDeviceNoAlloc_fp64(Parament_Context_f64 *handle,
+    const cuDoubleComplex *carr_dev, double dt, unsigned int pts, unsigned int amps, cuDoubleComplex *out_dev);
 LIBSPEC Parament_ErrorCode Parament_synchronize(Parament_Context_f32 *handle);
 LIBSPEC Parament_ErrorCode Parament_synchronize_fp64(Parament_Context_f64 *handle);
 

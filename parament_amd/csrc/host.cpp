@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -938,6 +939,56 @@ Parament_ErrorCode equiprop_device(Context<R>* h, const typename ApiCplx<R>::typ
     return h->last_error = err;
 }
 
+
+// ----- allocation-free equiprop (no counterpart in the reference) -----
+// For callers that cannot allocate device memory (code under HIP graph capture, tight serving
+// loops, harnesses that forbid hipMalloc): the working memory of a call is carved from one fixed
+// device array instead of the context's grow-only buffers.  Each GPU the library runs on holds
+// its own copy of the array until the process ends; all calls on one GPU share it, so calls that
+// overlap on one GPU race.  No kernel reads workspace memory it has not written in the same call,
+// so nothing needs clearing between calls.
+constexpr size_t kFixedWorkspaceBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kFixedWorkspaceBytes];
+
+// Device address of `scratch` on device `dev`, which must be current: looked up on the first
+// call on each device and kept (null when the lookup fails).
+char* fixed_workspace(int dev) {
+    static std::mutex mu;
+    static char* addr[kMaxDevices] = {};
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            hipGetLastError();
+            return nullptr;
+        }
+        addr[dev] = static_cast<char*>(p);
+    }
+    return addr[dev];
+}
+
+template <class R>
+Parament_ErrorCode equiprop_device_no_alloc(Context<R>* h, const cplx<R>* carr_dev, double dt, unsigned pts, unsigned amps,
+                                            cplx<R>* out_dev) {
+    if (!h) return PARAMENT_STATUS_INVALID_VALUE;
+    if (!h->has_h) return h->last_error = PARAMENT_STATUS_NO_HAMILTONIAN;
+    if (!carr_dev || !out_dev) return h->last_error = PARAMENT_STATUS_INVALID_VALUE;
+    Parament_ErrorCode err = ensure_common(h, amps);   // makes the context's device current
+    if (err != PARAMENT_STATUS_SUCCESS) return h->last_error = err;
+    char* base = fixed_workspace(h->device);
+    if (!base) return h->last_error = PARAMENT_FAIL;
+    Workspace ws;
+    ws.fixed = base;
+    ws.cap = kFixedWorkspaceBytes;
+    // kernel timing creates its events on first use: none are recorded by this entry point
+    const bool timing = h->timing;
+    h->timing = false;
+    err = run(h, carr_dev, dt, pts, amps, out_dev, ws);
+    h->timing = timing;
+    return h->last_error = err;
+}
 
 // ----- adjoint gate-fidelity gradient (grad.hip) -----
 // Layout of one gradient call over the pairwise tree.  Level l has n[l] = ceil(N / 2^l) nodes,
@@ -2066,6 +2117,12 @@ Parament_ErrorCode Parament_equipropDevice(Parament_Context_f32* handle, const c
 Parament_ErrorCode Parament_equipropDevice_fp64(Parament_Context_f64* handle, const cuDoubleComplex* carr_dev, double dt,
                                                 unsigned int pts, unsigned int amps, cuDoubleComplex* out_dev) {
     return equiprop_device<double>(CTX64(handle), carr_dev, dt, pts, amps, out_dev);
+}
+Parament_ErrorCode Parament_equipropDeviceNoAlloc_fp64(Parament_Context_f64* handle, const cuDoubleComplex* carr_dev,
+                                                       double dt, unsigned int pts, unsigned int amps,
+                                                       cuDoubleComplex* out_dev) {
+    return equiprop_device_no_alloc<double>(CTX64(handle), reinterpret_cast<const double2*>(carr_dev), dt, pts, amps,
+                                            reinterpret_cast<double2*>(out_dev));
 }
 Parament_ErrorCode Parament_synchronize(Parament_Context_f32* handle) {
     if (!handle) return PARAMENT_STATUS_INVALID_VALUE;

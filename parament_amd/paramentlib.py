@@ -114,6 +114,8 @@ lib.Parament_ensembleGateFidelityGradient.argtypes = [c_ParamentContext_p, c_cuC
                                                       np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
 lib.Parament_ensembleGateFidelityGradientDevice.argtypes = [c_ParamentContext_p, ctypes.c_void_p, ctypes.c_void_p,
                                                             ctypes.c_void_p, ctypes.c_void_p]
+lib.Parament_equipropDeviceNoAlloc_fp64.argtypes = [c_ParamentContext_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_uint,
+                                                    ctypes.c_uint, ctypes.c_void_p]
 lib.Parament_errorMessage.argtypes = [ctypes.c_int]
 lib.Parament_errorMessage.restype = ctypes.c_char_p
 lib.Parament_selectIterationCycles_fp32.argtypes = [ctypes.c_double, ctypes.c_double]
