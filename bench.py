@@ -39,12 +39,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Dense peaks of one MI355X (AMD's published figures).  The mixed paths compute FP32-accurate
-# products as six bf16 products each (three-way split, pairs p + q <= 2), so their peak is the
-# bf16 dense peak over six.
-PEAK_TFLOPS = {"fp64": 78.6, "fp32_native": 157.3, "mixed": 2500.0 / 6.0, "fp32": 2500.0 / 6.0}
-PEAK_CONVENTION = ("mixed and fp32 (complex64 default): bf16 dense 2500 TFLOP/s / 6 bf16 products per FP32-accurate "
-                   "product; fp64: 78.6; fp32_native: 157.3")
+# Dense peaks of one MI355X (AMD's published figures).  The mixed paths compute each FP32-accurate
+# real product as several 16-bit matrix-core products: three with the library's default scaled
+# fp16 two-way split (h0 h0' + h0 h1' + h1 h0'), six with the bf16 three-way split
+# (PARAMENT_MIXED_SPLIT=bf16x3), four with PARAMENT_MIXED_SPLIT=f16x4; their peak is the 16-bit
+# dense peak (2.5 PFLOP/s) over that count.
+SPLIT = os.environ.get("PARAMENT_MIXED_SPLIT", "f16x3")
+SPLIT_PRODUCTS = {"f16x3": 3, "f16x4": 4, "bf16x3": 6}.get(SPLIT, 3)
+PEAK_TFLOPS = {"fp64": 78.6, "fp32_native": 157.3, "mixed": 2500.0 / SPLIT_PRODUCTS, "fp32": 2500.0 / SPLIT_PRODUCTS}
+PEAK_CONVENTION = (f"mixed and fp32 (complex64 default): 16-bit dense 2500 TFLOP/s / {SPLIT_PRODUCTS} matrix-core "
+                   f"products per FP32-accurate product ({SPLIT} split); fp64: 78.6; fp32_native: 157.3")
 MIXED_PATHS = ("mixed", "fp32")
 
 # HBM bytes per launch of the mixed slice kernel on this workload, from rocprofv3 FETCH_SIZE and
@@ -462,7 +466,7 @@ def main():
         "metric": "steps/s at dim 64 @<=1e-5 vs FP64 (mixed vs plain FP64); GRAPE grads/s",
         "value": p["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": p["ms_per_call"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "complex128", "mixed": "bf16x3(fp32 acc)+fp64", "fp32": "bf16x3(fp32 acc)+fp64",
+        "dtype": {"fp64": "complex128", "mixed": f"{SPLIT} split(fp32 acc)+fp64", "fp32": f"{SPLIT} split(fp32 acc)+fp64",
                   "fp32_native": "complex64"}[best],
         "data": "synthetic",
         "config": {"workload": "heisenberg6: dim 64, 3 controls, 65536 steps, dt 5e-3, simpson (32767 slices)",
@@ -475,7 +479,8 @@ def main():
                      "kernel": "Chebyshev slice kernel (folds %d tree levels)" % lv, "flops_per_launch": fl,
                      "flops_convention": "algorithmic: 8 d^3 M per slice (M Chebyshev GEMMs) + 8 d^2 A assembly "
                                          "+ 8 d^3 per folded tree product; 8 flops per complex MAC",
-                     "kernel_ms": p["slice_kernel_ms"], "peak_convention": PEAK_CONVENTION},
+                     "kernel_ms": p["slice_kernel_ms"], "peak_convention": PEAK_CONVENTION,
+                     "frac_vs_bf16_dense": p["slice_kernel_alg_tflops"] / 2500.0 if best in MIXED_PATHS else None},
         "speedup_mixed_vs_fp64": paths["mixed"]["steps_per_s"] / paths["fp64"]["steps_per_s"],
         "paths": paths,
     }

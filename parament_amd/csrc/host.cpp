@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -1796,7 +1797,18 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
                 }
         }
     }
-    for (int i : e->large) {   // output tile of each large instance's batched GEMMs
+    // output tile of the large instances' batched GEMMs.  The best tile follows the dimension:
+    // instances are grouped by dim in bands of 16, the band's instance with the most slices is
+    // timed with each tile, and the whole band takes the fastest.
+    std::map<unsigned, int> rep_of_band;   // band -> instance timed
+    for (int i : e->large) {
+        const unsigned band = (e->in[i].d + 15) / 16;
+        const auto hit = rep_of_band.find(band);
+        if (hit == rep_of_band.end() || e->in[i].N > e->in[hit->second].N) rep_of_band[band] = i;
+    }
+    std::map<unsigned, int> tile_of_band;
+    for (const auto& br : rep_of_band) {
+        const int i = br.second;
         float best = 1e30f;
         int best_tile = 64;
         for (int tile : {64, 32, 16}) {   // 16: the whole-product kernel
@@ -1804,8 +1816,9 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
             const float t = ens_time(e, [&] { return ens_enqueue_large(e, i, e->stream, 512); }, 1);
             if (t < best) { best = t; best_tile = tile; }
         }
-        e->in[i].tile = best_tile;
+        tile_of_band[br.first] = best_tile;
     }
+    for (int i : e->large) e->in[i].tile = tile_of_band[(e->in[i].d + 15) / 16];
     if (!e->large.empty()) {
         float best = 1e30f;
         int best_nb = 1;

@@ -198,8 +198,10 @@ __global__ __launch_bounds__(64) void small_fold_kernel(ChebArgs<R> a, int n_nod
 }
 
 int small_fold_levels(int N) {
+    // fold while at least 2048 nodes remain (measured at dim 8: 4096 slices run fastest with one
+    // folded level, 27 us per call against 28 with none and 31 with two)
     int f = 0;
-    while (f < 5 && (N >> (f + 1)) >= 4096) ++f;
+    while (f < 5 && (N >> (f + 1)) >= 2048) ++f;
     return f;
 }
 
@@ -312,43 +314,57 @@ __global__ __launch_bounds__(256) void tree_split_kernel(const SrcT* __restrict_
 // reference's pairing (later node on the left) and zero nodes past the end, which return their
 // partner bit for bit (the odd-carry rule).  Every thread reads its operands before the barrier
 // that precedes the in-place write of the level's results.  Two launches reduce up to 4096 nodes.
-constexpr int kSmallTreeMaxPer = 8;   // outputs per thread and level (B * d * d <= 16 * 256)
+constexpr int kSmallTreeMaxPer = 8;   // outputs per thread and level (B * D * D <= 16 * 256)
 
-__host__ __device__ inline int small_tree_block(int d) {
+// nodes per work-group for nodes padded to D x D
+__host__ __device__ inline int small_tree_block(int D) {
     int b = 64;
-    while (b > 1 && b * d * d > 2 * kSmallTreeMaxPer * 256) b >>= 1;
+    while (b > 1 && b * D * D > 2 * kSmallTreeMaxPer * 256) b >>= 1;
     return b;
 }
 
-template <class R, class SrcT>
+// Nodes are held padded to D x D with zeros (D = 4, 8, 16): a zero row or column of E = U - I
+// stays zero through A + B + A B, so the d x d block is what the unpadded products give.
+template <class R, int D, class SrcT>
 __global__ __launch_bounds__(256) void small_tree_kernel(const SrcT* __restrict__ src, int n_in, int d, int B,
                                                          cplx<R>* __restrict__ dst, int final_level) {
     extern __shared__ __attribute__((aligned(16))) unsigned char small_tree_lds[];
     cplx<R>* nd = reinterpret_cast<cplx<R>*>(small_tree_lds);
+    constexpr int DD = D * D;
     const int dd = d * d;
     const int first = blockIdx.x * B;
     const int live = min(B, n_in - first);
-    for (int e = threadIdx.x; e < B * dd; e += 256) {
-        cplx<R> v = mk<R>(0, 0);
-        if (e < live * dd) {
-            const SrcT x = src[(size_t)first * dd + e];
-            v = mk<R>(R(x.x), R(x.y));
-        }
-        nd[e] = v;
+    // all of the block's loads in flight before the first LDS store (B * D * D <= 16 * 256)
+    SrcT ld[2 * kSmallTreeMaxPer];
+#pragma unroll
+    for (int t = 0; t < 2 * kSmallTreeMaxPer; ++t) {
+        const int e = threadIdx.x + 256 * t;
+        const int node = e / DD, ik = e % DD, i = ik / D, k = ik % D;
+        ld[t].x = 0;
+        ld[t].y = 0;
+        if (e < B * DD && node < live && i < d && k < d) ld[t] = src[(size_t)(first + node) * dd + i * d + k];
+    }
+#pragma unroll
+    for (int t = 0; t < 2 * kSmallTreeMaxPer; ++t) {
+        const int e = threadIdx.x + 256 * t;
+        if (e < B * DD) nd[e] = mk<R>(R(ld[t].x), R(ld[t].y));
     }
     __syncthreads();
     for (int cnt = B; cnt > 1; cnt >>= 1) {
-        const int total = (cnt >> 1) * dd;
+        const int total = (cnt >> 1) * DD;
         cplx<R> res[kSmallTreeMaxPer];
+        // with nodes padded to D x D, total is a multiple of 64 unless it is below 64, so the
+        // guard never splits a wave that has work past the first
 #pragma unroll
         for (int t = 0; t < kSmallTreeMaxPer; ++t) {
             const int e = threadIdx.x + 256 * t;
             if (e < total) {
-                const int p = e / dd, ik = e - p * dd, i = ik / d, k = ik - i * d;
-                const cplx<R>* A = nd + (size_t)(2 * p + 1) * dd;   // later node
-                const cplx<R>* Bm = nd + (size_t)(2 * p) * dd;      // earlier node
+                const int p = e / DD, ik = e % DD, i = ik / D, k = ik % D;
+                const cplx<R>* A = nd + (size_t)(2 * p + 1) * DD;   // later node
+                const cplx<R>* Bm = nd + (size_t)(2 * p) * DD;      // earlier node
                 cplx<R> acc = cadd(A[ik], Bm[ik]);
-                for (int m = 0; m < d; ++m) acc = cadd(acc, cmul(A[i * d + m], Bm[m * d + k]));
+#pragma unroll
+                for (int m = 0; m < D; ++m) acc = cadd(acc, cmul(A[i * D + m], Bm[m * D + k]));
                 res[t] = acc;
             }
         }
@@ -361,10 +377,23 @@ __global__ __launch_bounds__(256) void small_tree_kernel(const SrcT* __restrict_
         __syncthreads();
     }
     for (int e = threadIdx.x; e < dd; e += 256) {
-        cplx<R> v = nd[e];
-        if (final_level && e / d == e % d) v.x += R(1);
+        const int i = e / d, k = e % d;
+        cplx<R> v = nd[i * D + k];
+        if (final_level && i == k) v.x += R(1);
         dst[(size_t)blockIdx.x * dd + e] = v;
     }
+}
+
+template <class R, class SrcT>
+static void launch_small_tree(int D, const SrcT* src, int n, int d, int b, cplx<R>* out, bool fin, hipStream_t st) {
+    const int n_out = (n + b - 1) / b;
+    const size_t lds = (size_t)b * D * D * sizeof(cplx<R>);
+    if (D == 4)
+        hipLaunchKernelGGL((small_tree_kernel<R, 4, SrcT>), dim3(n_out), dim3(256), lds, st, src, n, d, b, out, fin ? 1 : 0);
+    else if (D == 8)
+        hipLaunchKernelGGL((small_tree_kernel<R, 8, SrcT>), dim3(n_out), dim3(256), lds, st, src, n, d, b, out, fin ? 1 : 0);
+    else
+        hipLaunchKernelGGL((small_tree_kernel<R, 16, SrcT>), dim3(n_out), dim3(256), lds, st, src, n, d, b, out, fin ? 1 : 0);
 }
 
 template <class R> constexpr int tree_logb() { return 3; }
@@ -400,8 +429,9 @@ hipError_t launch_tree(const void* leaves, bool leaves_are_c64, int N, int d, cp
                        cplx<R>* root, hipStream_t st) {
     if (d <= 0 || d > 64 || N <= 0) return hipErrorInvalidValue;
     if (d <= 16) {
-        // block reductions in LDS: ceil(n / B) nodes per launch
-        const int B = small_tree_block(d);
+        // block reductions in LDS: ceil(n / B) nodes per launch, nodes padded to D x D
+        const int D = d <= 4 ? 4 : (d <= 8 ? 8 : 16);
+        const int B = small_tree_block(D);
         cplx<R>* bufs[2] = {scratch0, scratch1};
         int which = 0, n = N;
         bool first = true;
@@ -411,14 +441,11 @@ hipError_t launch_tree(const void* leaves, bool leaves_are_c64, int N, int d, cp
             int b = B;   // a single block needs only the next power of two above its node count
             if (fin) while (b > 1 && (b >> 1) >= n) b >>= 1;
             cplx<R>* out = fin ? root : bufs[which];
-            const size_t lds = (size_t)b * d * d * sizeof(cplx<R>);
             if (first && leaves_are_c64)
-                hipLaunchKernelGGL((small_tree_kernel<R, float2>), dim3(n_out), dim3(256), lds, st,
-                                   reinterpret_cast<const float2*>(leaves), n, d, b, out, fin ? 1 : 0);
+                launch_small_tree<R>(D, reinterpret_cast<const float2*>(leaves), n, d, b, out, fin, st);
             else
-                hipLaunchKernelGGL((small_tree_kernel<R, cplx<R>>), dim3(n_out), dim3(256), lds, st,
-                                   first ? reinterpret_cast<const cplx<R>*>(leaves) : bufs[which ^ 1], n, d, b, out,
-                                   fin ? 1 : 0);
+                launch_small_tree<R>(D, first ? reinterpret_cast<const cplx<R>*>(leaves) : bufs[which ^ 1], n, d, b, out,
+                                     fin, st);
             if (fin) break;
             first = false;
             n = n_out;

@@ -53,12 +53,12 @@ __device__ __forceinline__ void split2(float x0, float x1, f16x2& h0, f16x2& h1,
     h1[0] = (_Float16)(s0 - (float)h0[0]);
     h1[1] = (_Float16)(s1 - (float)h0[1]);
 }
-// The same from FP64 values; also returns what the parts leave out, x - (h0 + h1) / S.  The
-// scaled value is rounded to FP32 first (y, with its rounding residual e) and y is split with the
-// hardware FP32 -> fp16 conversions; the empty asm statement keeps the compiler from merging the
-// two roundings into a double -> half conversion, which has no instruction and is emulated.
-__device__ __forceinline__ void split2d(double x0, double x1, f16x2& h0, f16x2& h1, float& r0, float& r1) {
-    const double s0 = x0 * (double)kScale, s1 = x1 * (double)kScale;
+// The same from FP64 values already scaled by S (s = S x); also returns what the parts leave out,
+// x - (h0 + h1) / S.  The scaled value is rounded to FP32 first (y, with its rounding residual e)
+// and y is split with the hardware FP32 -> fp16 conversions; the empty asm statement keeps the
+// compiler from merging the two roundings into a double -> half conversion, which has no
+// instruction and is emulated.
+__device__ __forceinline__ void split2d(double s0, double s1, f16x2& h0, f16x2& h1, float& r0, float& r1) {
     float y0 = (float)s0, y1 = (float)s1;
     const float e0 = (float)(s0 - (double)y0), e1 = (float)(s1 - (double)y1);
     asm volatile("" : "+v"(y0), "+v"(y1));
@@ -202,7 +202,10 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
     constexpr int NB = 1 << F;
     typedef HTile<TN> Tile;
     __shared__ __attribute__((aligned(16))) _Float16 Xp[4 * PL];
-    __shared__ __attribute__((aligned(16))) _Float16 Bp[4 * PL];
+    // B operands, two sets: consecutive Horner steps read one and write the other, so a step needs
+    // one barrier (the one after its write)
+    __shared__ __attribute__((aligned(16))) _Float16 Bpp[2][4 * PL];
+    _Float16* const Bp = Bpp[0];
     // Y_j - (h0 + h1) / S: the FP64 residual of the split, for the first-order term
     __shared__ __attribute__((aligned(16))) float2 Rx[D * (D + 2)];
     __shared__ double2 coef[kMaxCoeffs];
@@ -223,9 +226,12 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
             in[n][v] = EXACT || (hrow(row0, v) < d && hcol(col0, n) < d);
         }
 
+    // the assembly forms S Y = (tau S) H0 + sum_t (tau S c'_t) O_t: coef[] holds tau S c'_t
+    const double ts = a.ctl.tau * (double)kScale;
+    auto scaled = [&](double2 c) { return mk<double>(c.x * ts, c.y * ts); };
     auto slice = [&](int j, int jn, Tile& e) __attribute__((always_inline)) {
         if (!coef_ready)
-            for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = slice_coefficient(a.ctl, t, j);
+            for (int t = tid; t < a.ncoef; t += 64 * G) coef[t] = scaled(slice_coefficient(a.ctl, t, j));
         __syncthreads();
         // Y_j = tau (H0 + sum_t c'_t O_t) in FP64 as 2 x 2 blocks (lane (bi, bk) = (l / 8, l % 8) of
         // a 16 x 16 region): both Y (Xp) and Z = Y^T (Bp) take 4-byte stores per row pair
@@ -242,8 +248,32 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
                     const int ii = i + (e4 >> 1), kk = k + (e4 & 1);
                     v[e4] = EXACT || (ii < d && kk < d);
                     x[e4] = v[e4] ? a.ops[ii * d + kk] : mk<double>(0, 0);
+                    x[e4].x *= ts;
+                    x[e4].y *= ts;
                 }
-                for (int t0 = 0; t0 < a.ncoef; t0 += 2) {
+                if (a.ncoef <= 3) {
+                    // every operator element of the iteration requested at once (one load latency)
+                    double2 o[3][4];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+#pragma unroll
+                        for (int e4 = 0; e4 < 4; ++e4) {
+                            const size_t at = (size_t)(q + 1) * dd + (i + (e4 >> 1)) * d + k + (e4 & 1);
+                            o[q][e4] = (v[e4] && q < a.ncoef) ? a.ops[at] : mk<double>(0, 0);
+                        }
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const double2 cc = q < a.ncoef ? coef[q] : mk<double>(0, 0);
+#pragma unroll
+                        for (int e4 = 0; e4 < 4; ++e4) {
+                            x[e4].x = __builtin_fma(cc.x, o[q][e4].x, x[e4].x);
+                            x[e4].x = __builtin_fma(-cc.y, o[q][e4].y, x[e4].x);
+                            x[e4].y = __builtin_fma(cc.x, o[q][e4].y, x[e4].y);
+                            x[e4].y = __builtin_fma(cc.y, o[q][e4].x, x[e4].y);
+                        }
+                    }
+                }
+                for (int t0 = 0; a.ncoef > 3 && t0 < a.ncoef; t0 += 2) {
                     double2 o[2][4];
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
@@ -254,11 +284,17 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
                         }
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (t0 + q < a.ncoef)
+                        if (t0 + q < a.ncoef) {
+                            const double2 cc = coef[t0 + q];
 #pragma unroll
-                            for (int e4 = 0; e4 < 4; ++e4) x[e4] = cadd(x[e4], cmul(coef[t0 + q], o[q][e4]));
+                            for (int e4 = 0; e4 < 4; ++e4) {   // x += cc o, four FMAs
+                                x[e4].x = __builtin_fma(cc.x, o[q][e4].x, x[e4].x);
+                                x[e4].x = __builtin_fma(-cc.y, o[q][e4].y, x[e4].x);
+                                x[e4].y = __builtin_fma(cc.x, o[q][e4].y, x[e4].y);
+                                x[e4].y = __builtin_fma(cc.y, o[q][e4].x, x[e4].y);
+                            }
+                        }
                 }
-                const double tau = a.ctl.tau;
                 // parts of the row pairs: p[c][h][part] (c: re / im, h: row i + h), columns k, k + 1
                 f16x2 p[2][2][2];
                 float rr[2][2][2];
@@ -267,8 +303,7 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const double2 &xa = x[2 * h], &xb = x[2 * h + 1];
-                        split2d(tau * (c ? xa.y : xa.x), tau * (c ? xb.y : xb.x), p[c][h][0], p[c][h][1], rr[c][h][0],
-                                rr[c][h][1]);
+                        split2d(c ? xa.y : xa.x, c ? xb.y : xb.x, p[c][h][0], p[c][h][1], rr[c][h][0], rr[c][h][1]);
                     }
 #pragma unroll
                 for (int c = 0; c < 2; ++c)
@@ -308,7 +343,8 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         wave_cgemm_h2<D, FULL>(Bp, Xp, row0, col0, acc);   // S^2 Z Z = S^2 (Y^2)^T
         __syncthreads();
         write_h2<D, true>(acc, Xp, row0, col0, kScale * kInvScale2);   // A := Y^2 for the rest of the slice
-        // Q_jj = m_{2jj} I + m_{2jj+1} Y (+ the product, raw)
+        // Q_jj = m_{2jj} I + m_{2jj+1} Y (+ the product, raw): FP32, except Q_1 (the largest terms,
+        // whose constant diagonal m_2 recurs in every slice), formed in FP64 and rounded once
         auto bracket = [&](int jj, bool with_acc) {
             const double mr = a.are[2 * jj], mi = a.aim[2 * jj];
             const double nr = a.are[2 * jj + 1], ni = a.aim[2 * jj + 1];
@@ -331,24 +367,25 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         };
         // whether the product Y^2 Q_{jj+1} (jj >= 1) runs on the leading parts only
         auto lead_only = [&](int jj) { return jj >= 1 && ((a.ps_levels >> (2 * jj)) & 3) == 0; };
-        auto write_q = [&](int jj) {
-            if (lead_only(jj - 1)) write_h2<D, true, 1>(q, Bp, row0, col0);
-            else write_h2<D, true, 2>(q, Bp, row0, col0);
+        auto write_q = [&](int jj, _Float16* dstp) {
+            if (lead_only(jj - 1)) write_h2<D, true, 1>(q, dstp, row0, col0);
+            else write_h2<D, true, 2>(q, dstp, row0, col0);
         };
+        int cur = 1;   // B set holding the current Q (set 0 held Z, read by the product above)
         bracket(J, false);
-        write_q(J);
+        write_q(J, Bpp[cur]);
         __syncthreads();
         for (int jj = J - 1; jj >= 1; --jj) {
-            if (lead_only(jj)) wave_cgemm_h2<D, 1>(Xp, Bp, row0, col0, acc);
-            else wave_cgemm_h2<D, FULL>(Xp, Bp, row0, col0, acc);
+            if (lead_only(jj)) wave_cgemm_h2<D, 1>(Xp, Bpp[cur], row0, col0, acc);
+            else wave_cgemm_h2<D, FULL>(Xp, Bpp[cur], row0, col0, acc);
             bracket(jj, true);
-            __syncthreads();
-            write_q(jj);
+            cur ^= 1;
+            write_q(jj, Bpp[cur]);   // every wave finished reading this set before the last barrier
             __syncthreads();
         }
-        if (pre && tid < a.ncoef) coef[tid] = coef_finish_h(a.ctl, raw);
+        if (pre && tid < a.ncoef) coef[tid] = scaled(coef_finish_h(a.ctl, raw));
         coef_ready = pre;
-        wave_cgemm_h2<D, FULL>(Xp, Bp, row0, col0, acc);   // S^2 R = S^2 Y^2 Q_1
+        wave_cgemm_h2<D, FULL>(Xp, Bpp[cur], row0, col0, acc);   // S^2 R = S^2 Y^2 Q_1
         const double m0m1 = a.are[0];
         const double2 m1 = mk<double>(a.are[1], a.aim[1]);
 #pragma unroll
