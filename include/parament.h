@@ -126,8 +126,12 @@ LIBSPEC double OneNorm_fp64(cuDoubleComplex *mat, unsigned int dim);
  *   PARAMENT_PRECISION_MIXED:  FP64 for Hnorm, the Bessel coefficients, the slice
  *       Hamiltonians, the first-order Chebyshev terms and the upper levels of the product
  *       tree; the higher-order Chebyshev remainder and the lowest three tree levels run as
- *       bf16 three-way split products with FP32 accumulation on the matrix cores
- *       (FP32-accurate products).  Default of complex64 contexts, which keep the FP32
+ *       split products with FP32 accumulation on the matrix cores (FP32-accurate products):
+ *       scaled fp16 two-way splits (three fp16 products per product) in the fused kernel's
+ *       power form (dim <= 64, tau Hnorm <= 1), bf16 three-way splits (six bf16 products)
+ *       otherwise.  The environment variable PARAMENT_MIXED_SPLIT=bf16x3 (or f16x4, all
+ *       four fp16 products) selects the other form of the fused kernel, for precision
+ *       studies; it is read once per process.  Default of complex64 contexts, which keep the FP32
  *       iteration table's M and, for dims <= 16 on the automatic kernel path, the FP32
  *       wavefront-resident kernels.  A complex128 context in this mode takes M from the FP32
  *       table raised until the accumulated truncation error stays below 2e-6.
@@ -247,11 +251,11 @@ LIBSPEC size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64 *handle);
  * Each instance computes what Parament_equiprop computes for it (FP32 iteration-count table,
  * same quadrature and product order).  setExecution(0) (default) runs the whole ensemble as
  * one captured HIP graph: the instances of dim <= 64 in multi-instance kernels per dim class
- * (16, 32, 64), the larger ones on batched GEMMs, on concurrent branches; the wave split, tree
- * block size and grid of each class and the number of branches are tuned on first use (or by
- * ensembleTune).  setExecution(1) runs one instance after the other (the per-instance loop).
- * getTuning fills config[10]: (T, log2 tree block, grid multiple) for classes 16, 32, 64, then
- * the number of branches. */
+ * (16, 32, 64), the larger ones on batched GEMMs, on concurrent branches (up to four for the
+ * larger ones); the wave split, tree block size and grid of each class and the GEMM tile per band
+ * of 16 dims are tuned on first use (or by ensembleTune).  setExecution(1) runs one instance
+ * after the other (the per-instance loop).  getTuning fills config[10]: (T, log2 tree block,
+ * grid multiple) for classes 16, 32, 64, then the number of branches. */
 typedef struct Parament_Ensemble Parament_Ensemble;
 LIBSPEC Parament_ErrorCode Parament_ensembleCreate(Parament_Ensemble **ensemble_p, unsigned int count,
     const unsigned int *dims, const unsigned int *amps, const unsigned int *pts, const double *dt,

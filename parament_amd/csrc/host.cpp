@@ -1434,8 +1434,9 @@ template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, dou
 // the step-wise path on batched GEMMs.  The classes and the large instances go on separate
 // branches (streams forked from and joined to the ensemble's stream), the whole solve is
 // captured once as a hipGraph and replayed.  The tuner picks, per class, the wave split, the
-// tree block size and the persistent grid, and the number of branches for large instances, by
-// timing each candidate on the ensemble itself.
+// tree block size and the persistent grid, and per band of 16 dims the output tile of the large
+// instances' GEMMs, by timing each candidate on the ensemble itself; large instances run on up to
+// four branches.
 constexpr int kEnsSide = 8;        // side streams: 3 classes + up to 5 large-instance branches
 constexpr int kEnsClasses = 3;     // dim classes 16, 32, 64
 
@@ -1766,9 +1767,9 @@ hipError_t ens_enqueue_loop(Ensemble* e) {
     return r;
 }
 
-float ens_time(Ensemble* e, const std::function<hipError_t()>& f, int reps) {
+float ens_time(Ensemble* e, const std::function<hipError_t()>& f, int reps, bool warm = true) {
     // a candidate whose launches or timing fail is never the fastest
-    if (f() != hipSuccess) return 1e30f;   // warm-up
+    if (warm && f() != hipSuccess) return 1e30f;   // warm-up
     if (hipEventRecord(e->t0, e->stream) != hipSuccess) return 1e30f;
     for (int k = 0; k < reps; ++k)
         if (f() != hipSuccess) return 1e30f;
@@ -1786,15 +1787,24 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
     for (int c = 0; c < kEnsClasses; ++c) {
         if (!e->cls[c].n_slice) continue;
         const int D = 16 << c;
+        // wave split and tree block size on the default grid first, then the persistent grid size
+        // for the best of those (the grid size has not changed the choice of the other two)
         float best = 1e30f;
+        auto trial = [&](const EnsCfg& cf) {
+            const float t = ens_time(e, [&] { return ens_enqueue_class(e, c, cf, e->stream); }, 1);
+            if (t < best) { best = t; e->cfg[c] = cf; }
+        };
         for (int T = 1; T <= 2; ++T) {
             if (!pa::ens_shape_valid(D, T)) continue;
-            for (int logb = 2; logb <= 3; ++logb)
-                for (int gm : {0, 2, 8}) {
-                    EnsCfg cf; cf.T = T; cf.logb = logb; cf.grid_mult = gm;
-                    const float t = ens_time(e, [&] { return ens_enqueue_class(e, c, cf, e->stream); }, 2);
-                    if (t < best) { best = t; e->cfg[c] = cf; }
-                }
+            for (int logb = 2; logb <= 3; ++logb) {
+                EnsCfg cf; cf.T = T; cf.logb = logb; cf.grid_mult = 0;
+                trial(cf);
+            }
+        }
+        const EnsCfg base = e->cfg[c];
+        for (int gm : {2, 8}) {
+            EnsCfg cf = base; cf.grid_mult = gm;
+            trial(cf);
         }
     }
     // output tile of the large instances' batched GEMMs.  The best tile follows the dimension:
@@ -1819,20 +1829,10 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
         tile_of_band[br.first] = best_tile;
     }
     for (int i : e->large) e->in[i].tile = tile_of_band[(e->in[i].d + 15) / 16];
-    if (!e->large.empty()) {
-        float best = 1e30f;
-        int best_nb = 1;
-        int saved[kEnsClasses];   // time the large-instance branches alone
-        for (int c = 0; c < kEnsClasses; ++c) { saved[c] = e->cls[c].n_slice; e->cls[c].n_slice = 0; }
-        for (int nb : {1, 2, 4}) {
-            if (nb > 1 && (size_t)nb > e->large.size()) continue;
-            e->branches = nb;
-            const float t = ens_time(e, [&] { return ens_enqueue(e); }, 1);
-            if (t < best) { best = t; best_nb = nb; }
-        }
-        for (int c = 0; c < kEnsClasses; ++c) e->cls[c].n_slice = saved[c];
-        e->branches = best_nb;
-    }
+    // large-instance branches: four (or one per large instance when fewer).  Timing the branch
+    // counts costs whole solves of the large instances, and four has been fastest wherever it
+    // was timed (512-system ensemble: 1, 2, 4 branches)
+    e->branches = std::max(1, std::min(4, (int)e->large.size()));
     if (hipStreamSynchronize(e->stream) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
     e->tuned = true;
     return PARAMENT_STATUS_SUCCESS;
