@@ -248,8 +248,8 @@ LIBSPEC size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64 *handle);
  *   H:   (1 + amps[i]) row-major dims[i] x dims[i] matrices (H0, then H_1 ..),
  *   carr: amps[i] x pts[i] samples (carr layout of Parament_equiprop),
  *   out: dims[i] x dims[i] propagators.
- * Each instance computes what Parament_equiprop computes for it (FP32 iteration-count table,
- * same quadrature and product order).  setExecution(0) (default) runs the whole ensemble as
+ * Each instance computes what Parament_equiprop computes for it in PARAMENT_PRECISION_NATIVE
+ * mode (FP32 throughout, FP32 iteration-count table, same quadrature and product order).  setExecution(0) (default) runs the whole ensemble as
  * one captured HIP graph: the instances of dim <= 64 in multi-instance kernels per dim class
  * (16, 32, 64), the larger ones on batched GEMMs, on concurrent branches (up to four for the
  * larger ones); the wave split, tree block size and grid of each class and the GEMM tile per band

@@ -243,27 +243,26 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
                 const int i = 16 * (rg / RT) + 2 * (lane >> 3), k = 16 * (rg % RT) + 2 * (lane & 7);
                 bool v[4];
                 double2 x[4];
-#pragma unroll
-                for (int e4 = 0; e4 < 4; ++e4) {
-                    const int ii = i + (e4 >> 1), kk = k + (e4 & 1);
-                    v[e4] = EXACT || (ii < d && kk < d);
-                    x[e4] = v[e4] ? a.ops[ii * d + kk] : mk<double>(0, 0);
-                    x[e4].x *= ts;
-                    x[e4].y *= ts;
-                }
                 if (a.ncoef <= 3) {
-                    // every operator element of the iteration requested at once (one load latency)
-                    double2 o[3][4];
+                    // every operator element of the iteration requested before any arithmetic (one
+                    // load latency): the loads are unconditional, an operator slot past ncoef reads
+                    // H0's element again and is weighted by zero, and a padded element reads the
+                    // operators' element 0 and is set to zero afterwards
+                    double2 h[4], o[3][4];
 #pragma unroll
-                    for (int q = 0; q < 3; ++q)
+                    for (int e4 = 0; e4 < 4; ++e4) {
+                        const int ii = i + (e4 >> 1), kk = k + (e4 & 1);
+                        v[e4] = EXACT || (ii < d && kk < d);
+                        const int at = v[e4] ? ii * d + kk : 0;
+                        h[e4] = a.ops[at];
 #pragma unroll
-                        for (int e4 = 0; e4 < 4; ++e4) {
-                            const size_t at = (size_t)(q + 1) * dd + (i + (e4 >> 1)) * d + k + (e4 & 1);
-                            o[q][e4] = (v[e4] && q < a.ncoef) ? a.ops[at] : mk<double>(0, 0);
-                        }
+                        for (int q = 0; q < 3; ++q) o[q][e4] = a.ops[(q < a.ncoef ? (size_t)(q + 1) * dd : 0) + at];
+                    }
+#pragma unroll
+                    for (int e4 = 0; e4 < 4; ++e4) x[e4] = mk<double>(h[e4].x * ts, h[e4].y * ts);
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
-                        const double2 cc = q < a.ncoef ? coef[q] : mk<double>(0, 0);
+                        const double2 cc = (q < a.ncoef) ? coef[q] : mk<double>(0, 0);
 #pragma unroll
                         for (int e4 = 0; e4 < 4; ++e4) {
                             x[e4].x = __builtin_fma(cc.x, o[q][e4].x, x[e4].x);
@@ -271,6 +270,18 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
                             x[e4].y = __builtin_fma(cc.x, o[q][e4].y, x[e4].y);
                             x[e4].y = __builtin_fma(cc.y, o[q][e4].x, x[e4].y);
                         }
+                    }
+#pragma unroll
+                    for (int e4 = 0; e4 < 4; ++e4)
+                        if (!v[e4]) x[e4] = mk<double>(0, 0);
+                } else {
+#pragma unroll
+                    for (int e4 = 0; e4 < 4; ++e4) {
+                        const int ii = i + (e4 >> 1), kk = k + (e4 & 1);
+                        v[e4] = EXACT || (ii < d && kk < d);
+                        x[e4] = v[e4] ? a.ops[ii * d + kk] : mk<double>(0, 0);
+                        x[e4].x *= ts;
+                        x[e4].y *= ts;
                     }
                 }
                 for (int t0 = 0; a.ncoef > 3 && t0 < a.ncoef; t0 += 2) {
