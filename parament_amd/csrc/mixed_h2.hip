@@ -13,7 +13,9 @@
 // tree nodes |E| = |U - I| are at most 2 here (tau Hnorm <= 1, unitary slices), so S x stays below
 // 2^15 < 65504, and an entry whose low part falls below fp16's normal range loses at most
 // 2^-25 / S = 2^-39 in absolute terms.  The accumulated products carry S^2, removed exactly by the
-// power-of-two factor 2^-28.
+// power-of-two factor 2^-28.  The Horner brackets Q_j, the slice result E and the fold's tree
+// nodes are held multiplied by S (exact), so that their planes are written without a multiply;
+// the work-group's node is divided by S when it is stored.
 #include "kernels.hpp"
 
 namespace pa {
@@ -45,13 +47,22 @@ template <int D> struct HShape {
 };
 
 // The two fp16 parts of the scaled values f x0, f x1 (round to nearest even; f = S for plain
-// values, S / S^2 for raw product accumulators).
+// values, S / S^2 for raw product accumulators, 1 for values held scaled by S).
 __device__ __forceinline__ void split2(float x0, float x1, f16x2& h0, f16x2& h1, float f = kScale) {
     const float s0 = x0 * f, s1 = x1 * f;
     h0[0] = (_Float16)s0;
     h0[1] = (_Float16)s1;
     h1[0] = (_Float16)(s0 - (float)h0[0]);
     h1[1] = (_Float16)(s1 - (float)h0[1]);
+}
+// The same for values already scaled by S.  one holds 1.0 where the compiler cannot see it, so
+// that s - h0 still becomes one mixed-precision FMA (s * one - h0, rounded to fp16) instead of a
+// conversion, a subtraction and a conversion.
+__device__ __forceinline__ void split2s(float s0, float s1, float one, f16x2& h0, f16x2& h1) {
+    h0[0] = (_Float16)s0;
+    h0[1] = (_Float16)s1;
+    h1[0] = (_Float16)__builtin_fmaf(s0, one, -(float)h0[0]);
+    h1[1] = (_Float16)__builtin_fmaf(s1, one, -(float)h0[1]);
 }
 // The same from FP64 values already scaled by S (s = S x); also returns what the parts leave out,
 // x - (h0 + h1) / S.  The scaled value is rounded to FP32 first (y, with its rounding residual e)
@@ -82,7 +93,8 @@ __device__ __forceinline__ f16x8 hneg(f16x8 v) {
     return __builtin_bit_cast(f16x8, u);
 }
 
-// c = S^2 (A B) on the wave's 16 x 32 tile (consumers apply kInvScale2): A planes row-major, B planes transposed
+// c = S^2 (A B) on the wave's 16 x 32 tile (consumers remove S^2, or S for a result held scaled
+// by S): A planes row-major, B planes transposed
 // (Bt[col][k]); plane 2 * part + {0: re, 1: im}.  NP = 1: h0 h0' only (relative error ~2^-11);
 // 3: with the two cross terms (~2^-21); 4: all four products.
 template <int D, int NP>
@@ -129,8 +141,9 @@ __device__ __forceinline__ void wave_cgemm_h2(const _Float16* __restrict__ Ap, c
 
 // Tile (accumulator layout) -> fp16 planes of its NL leading parts.  TRANSPOSED: the natural
 // write, P[col][row] = b(row, col), four consecutive rows per lane in one 8-byte store;
-// otherwise row-major P[row][col], 2-byte stores.
-template <int D, bool TRANSPOSED, int NL = 2>
+// otherwise row-major P[row][col], 2-byte stores.  The tile is multiplied by f before the split,
+// unless f is an opaque 1.0 (a tile held scaled by S): then split2s.
+template <int D, bool TRANSPOSED, int NL = 2, bool SCALED = false>
 __device__ __forceinline__ void write_h2(const HTile<HShape<D>::TN>& b, _Float16* __restrict__ P, int row0, int col0,
                                          float f = kScale) {
     constexpr int PL = HShape<D>::PL, TN = HShape<D>::TN;
@@ -142,8 +155,13 @@ __device__ __forceinline__ void write_h2(const HTile<HShape<D>::TN>& b, _Float16
         for (int c = 0; c < 2; ++c) {
             const f32x4h& x = c ? b.im[n] : b.re[n];
             f16x2 lo0, lo1, hi0, hi1;
-            split2(x[0], x[1], lo0, lo1, f);
-            split2(x[2], x[3], hi0, hi1, f);
+            if (SCALED) {
+                split2s(x[0], x[1], f, lo0, lo1);
+                split2s(x[2], x[3], f, hi0, hi1);
+            } else {
+                split2(x[0], x[1], lo0, lo1, f);
+                split2(x[2], x[3], hi0, hi1, f);
+            }
 #pragma unroll
             for (int t = 0; t < NL; ++t) {
                 const f16x2& a = t ? lo1 : lo0;
@@ -226,6 +244,8 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
             in[n][v] = EXACT || (hrow(row0, v) < d && hcol(col0, n) < d);
         }
 
+    float one = 1.0f;   // opaque to the compiler (split2s)
+    asm volatile("" : "+v"(one));
     // the assembly forms S Y = (tau S) H0 + sum_t (tau S c'_t) O_t: coef[] holds tau S c'_t
     const double ts = a.ctl.tau * (double)kScale;
     auto scaled = [&](double2 c) { return mk<double>(c.x * ts, c.y * ts); };
@@ -354,11 +374,13 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         wave_cgemm_h2<D, FULL>(Bp, Xp, row0, col0, acc);   // S^2 Z Z = S^2 (Y^2)^T
         __syncthreads();
         write_h2<D, true>(acc, Xp, row0, col0, kScale * kInvScale2);   // A := Y^2 for the rest of the slice
-        // Q_jj = m_{2jj} I + m_{2jj+1} Y (+ the product, raw): FP32, except Q_1 (the largest terms,
-        // whose constant diagonal m_2 recurs in every slice), formed in FP64 and rounded once
+        // S Q_jj = S (m_{2jj} I + m_{2jj+1} Y) (+ the product, raw S^2 Y^2 Q_{jj+1}): FP32, except
+        // Q_1 (the largest terms, whose constant diagonal m_2 recurs in every slice), formed in FP64
+        // and rounded once.  Q is held scaled by S (exact), so its planes are written without a
+        // multiply.
         auto bracket = [&](int jj, bool with_acc) {
-            const double mr = a.are[2 * jj], mi = a.aim[2 * jj];
-            const double nr = a.are[2 * jj + 1], ni = a.aim[2 * jj + 1];
+            const double mr = a.are[2 * jj] * kScale, mi = a.aim[2 * jj] * kScale;
+            const double nr = a.are[2 * jj + 1] * kScale, ni = a.aim[2 * jj + 1] * kScale;
 #pragma unroll
             for (int n = 0; n < TN; ++n)
 #pragma unroll
@@ -367,20 +389,20 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
                     const float ci = with_acc ? acc.im[n][v] : 0.f;
                     if (jj == 1) {
                         const double yr = ye.re[n][v], yi = ye.im[n][v];
-                        q.re[n][v] = (float)((nr * yr - ni * yi) + (double)(cr * kInvScale2) + (dg[n][v] ? mr : 0.0));
-                        q.im[n][v] = (float)((nr * yi + ni * yr) + (double)(ci * kInvScale2) + (dg[n][v] ? mi : 0.0));
+                        q.re[n][v] = (float)((nr * yr - ni * yi) + (double)(cr * kInvScale) + (dg[n][v] ? mr : 0.0));
+                        q.im[n][v] = (float)((nr * yi + ni * yr) + (double)(ci * kInvScale) + (dg[n][v] ? mi : 0.0));
                     } else {
                         const float yr = ye.re[n][v], yi = ye.im[n][v];
-                        q.re[n][v] = __builtin_fmaf(cr, kInvScale2, ((float)nr * yr - (float)ni * yi) + (dg[n][v] ? (float)mr : 0.f));
-                        q.im[n][v] = __builtin_fmaf(ci, kInvScale2, ((float)nr * yi + (float)ni * yr) + (dg[n][v] ? (float)mi : 0.f));
+                        q.re[n][v] = __builtin_fmaf(cr, kInvScale, ((float)nr * yr - (float)ni * yi) + (dg[n][v] ? (float)mr : 0.f));
+                        q.im[n][v] = __builtin_fmaf(ci, kInvScale, ((float)nr * yi + (float)ni * yr) + (dg[n][v] ? (float)mi : 0.f));
                     }
                 }
         };
         // whether the product Y^2 Q_{jj+1} (jj >= 1) runs on the leading parts only
         auto lead_only = [&](int jj) { return jj >= 1 && ((a.ps_levels >> (2 * jj)) & 3) == 0; };
         auto write_q = [&](int jj, _Float16* dstp) {
-            if (lead_only(jj - 1)) write_h2<D, true, 1>(q, dstp, row0, col0);
-            else write_h2<D, true, 2>(q, dstp, row0, col0);
+            if (lead_only(jj - 1)) write_h2<D, true, 1, true>(q, dstp, row0, col0, one);
+            else write_h2<D, true, 2, true>(q, dstp, row0, col0, one);
         };
         int cur = 1;   // B set holding the current Q (set 0 held Z, read by the product above)
         bracket(J, false);
@@ -397,8 +419,10 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         if (pre && tid < a.ncoef) coef[tid] = scaled(coef_finish_h(a.ctl, raw));
         coef_ready = pre;
         wave_cgemm_h2<D, FULL>(Xp, Bpp[cur], row0, col0, acc);   // S^2 R = S^2 Y^2 Q_1
-        const double m0m1 = a.are[0];
-        const double2 m1 = mk<double>(a.are[1], a.aim[1]);
+        // E = (m_0 - 1) I + m_1 Y + Y^2 Q_1 in FP64, held scaled by S like every tree node of the
+        // fold (exact), so that the fold's plane writes need no multiply
+        const double m0m1 = a.are[0] * kScale;
+        const double2 m1 = mk<double>(a.are[1] * kScale, a.aim[1] * kScale);
 #pragma unroll
         for (int n = 0; n < TN; ++n) {
             const int col = hcol(col0, n);
@@ -410,8 +434,8 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
                     const float2 rx = Rx[row * SR + col];
                     const double2 y = mk<double>((double)ye.re[n][v] + (double)rx.x, (double)ye.im[n][v] + (double)rx.y);
                     const double2 t = cmul(m1, y);
-                    er = t.x + (double)(acc.re[n][v] * kInvScale2) + (dg[n][v] ? m0m1 : 0.0);
-                    ei = t.y + (double)(acc.im[n][v] * kInvScale2);
+                    er = t.x + (double)(acc.re[n][v] * kInvScale) + (dg[n][v] ? m0m1 : 0.0);
+                    ei = t.y + (double)(acc.im[n][v] * kInvScale);
                 }
                 e.re[n][v] = (float)er;
                 e.im[n][v] = (float)ei;
@@ -420,10 +444,10 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         __syncthreads();
     };
 
-    // A := A + B + A B for the later node A and the earlier node B (in place)
+    // A := A + B + A B for the later node A and the earlier node B (in place; both held scaled by S)
     auto combine = [&](Tile& A, const Tile& B) __attribute__((always_inline)) {
-        write_h2<D, false>(A, Xp, row0, col0);
-        write_h2<D, true>(B, Bp, row0, col0);
+        write_h2<D, false, 2, true>(A, Xp, row0, col0, one);
+        write_h2<D, true, 2, true>(B, Bp, row0, col0, one);
         __syncthreads();
         Tile acc;
         wave_cgemm_h2<D, FULL>(Xp, Bp, row0, col0, acc);
@@ -431,8 +455,8 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         for (int n = 0; n < TN; ++n)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                A.re[n][v] = __builtin_fmaf(acc.re[n][v], kInvScale2, A.re[n][v] + B.re[n][v]);
-                A.im[n][v] = __builtin_fmaf(acc.im[n][v], kInvScale2, A.im[n][v] + B.im[n][v]);
+                A.re[n][v] = __builtin_fmaf(acc.re[n][v], kInvScale, A.re[n][v] + B.re[n][v]);
+                A.im[n][v] = __builtin_fmaf(acc.im[n][v], kInvScale, A.im[n][v] + B.im[n][v]);
             }
         __syncthreads();
     };
@@ -475,7 +499,8 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
                     const int col = hcol(col0, n);
 #pragma unroll
                     for (int v = 0; v < 4; ++v)
-                        if (in[n][v]) out[hrow(row0, v) * d + col] = make_float2(e.re[n][v], e.im[n][v]);
+                        if (in[n][v])
+                            out[hrow(row0, v) * d + col] = make_float2(e.re[n][v] * kInvScale, e.im[n][v] * kInvScale);
                 }
             }
         }
