@@ -96,8 +96,9 @@ __device__ __forceinline__ f16x8 hneg(f16x8 v) {
 // c = S^2 (A B) on the wave's 16 x 32 tile (consumers remove S^2, or S for a result held scaled
 // by S): A planes row-major, B planes transposed
 // (Bt[col][k]); plane 2 * part + {0: re, 1: im}.  NP = 1: h0 h0' only (relative error ~2^-11);
-// 3: with the two cross terms (~2^-21); 4: all four products.
-template <int D, int NP>
+// 3: with the two cross terms (~2^-21); 4: all four products.  NEGP: A's planes 4 + part hold
+// its negated imaginary parts (otherwise each fragment is negated here).
+template <int D, int NP, bool NEGP = false>
 __device__ __forceinline__ void wave_cgemm_h2(const _Float16* __restrict__ Ap, const _Float16* __restrict__ Bp, int row0,
                                               int col0, HTile<HShape<D>::TN>& c) {
     constexpr int PL = HShape<D>::PL, TN = HShape<D>::TN;
@@ -126,7 +127,7 @@ __device__ __forceinline__ void wave_cgemm_h2(const _Float16* __restrict__ Ap, c
                 if (!use) continue;
                 const f16x8 Ar = *reinterpret_cast<const f16x8*>(Ap + (2 * p) * PL + ka);
                 const f16x8 Ai = *reinterpret_cast<const f16x8*>(Ap + (2 * p + 1) * PL + ka);
-                const f16x8 An = hneg(Ai);
+                const f16x8 An = NEGP ? *reinterpret_cast<const f16x8*>(Ap + (4 + p) * PL + ka) : hneg(Ai);
 #pragma unroll
                 for (int n = 0; n < TN; ++n) {
                     c.re[n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, Br[n], c.re[n], 0, 0, 0);
@@ -142,8 +143,9 @@ __device__ __forceinline__ void wave_cgemm_h2(const _Float16* __restrict__ Ap, c
 // Tile (accumulator layout) -> fp16 planes of its NL leading parts.  TRANSPOSED: the natural
 // write, P[col][row] = b(row, col), four consecutive rows per lane in one 8-byte store;
 // otherwise row-major P[row][col], 2-byte stores.  The tile is multiplied by f before the split,
-// unless f is an opaque 1.0 (a tile held scaled by S): then split2s.
-template <int D, bool TRANSPOSED, int NL = 2, bool SCALED = false>
+// unless f is an opaque 1.0 (a tile held scaled by S): then split2s.  NEGIM (natural write):
+// planes 4 + part also receive the negated imaginary parts.
+template <int D, bool TRANSPOSED, int NL = 2, bool SCALED = false, bool NEGIM = false>
 __device__ __forceinline__ void write_h2(const HTile<HShape<D>::TN>& b, _Float16* __restrict__ P, int row0, int col0,
                                          float f = kScale) {
     constexpr int PL = HShape<D>::PL, TN = HShape<D>::TN;
@@ -169,6 +171,11 @@ __device__ __forceinline__ void write_h2(const HTile<HShape<D>::TN>& b, _Float16
                 if (TRANSPOSED) {
                     const f16x4 v = {a[0], a[1], z[0], z[1]};
                     *reinterpret_cast<f16x4*>(P + (2 * t + c) * PL + HShape<D>::off(col, rbase)) = v;
+                    if (NEGIM && c == 1) {
+                        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+                        const u32x2 u = __builtin_bit_cast(u32x2, v) ^ 0x80008000u;
+                        *reinterpret_cast<u32x2*>(P + (4 + t) * PL + HShape<D>::off(col, rbase)) = u;
+                    }
                 } else {
                     P[(2 * t + c) * PL + HShape<D>::off(rbase, col)] = a[0];
                     P[(2 * t + c) * PL + HShape<D>::off(rbase + 1, col)] = a[1];
@@ -219,7 +226,8 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
     constexpr int TN = HShape<D>::TN;
     constexpr int NB = 1 << F;
     typedef HTile<TN> Tile;
-    __shared__ __attribute__((aligned(16))) _Float16 Xp[4 * PL];
+    // A operands: planes 0-3 as in wave_cgemm_h2; planes 4, 5: Y^2's negated imaginary parts
+    __shared__ __attribute__((aligned(16))) _Float16 Xp[6 * PL];
     // B operands, two sets: consecutive Horner steps read one and write the other, so a step needs
     // one barrier (the one after its write)
     __shared__ __attribute__((aligned(16))) _Float16 Bpp[2][4 * PL];
@@ -373,7 +381,7 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         }
         wave_cgemm_h2<D, FULL>(Bp, Xp, row0, col0, acc);   // S^2 Z Z = S^2 (Y^2)^T
         __syncthreads();
-        write_h2<D, true>(acc, Xp, row0, col0, kScale * kInvScale2);   // A := Y^2 for the rest of the slice
+        write_h2<D, true, 2, false, true>(acc, Xp, row0, col0, kScale * kInvScale2);   // A := Y^2 for the rest of the slice
         // S Q_jj = S (m_{2jj} I + m_{2jj+1} Y) (+ the product, raw S^2 Y^2 Q_{jj+1}): FP32, except
         // Q_1 (the largest terms, whose constant diagonal m_2 recurs in every slice), formed in FP64
         // and rounded once.  Q is held scaled by S (exact), so its planes are written without a
@@ -409,8 +417,8 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         write_q(J, Bpp[cur]);
         __syncthreads();
         for (int jj = J - 1; jj >= 1; --jj) {
-            if (lead_only(jj)) wave_cgemm_h2<D, 1>(Xp, Bpp[cur], row0, col0, acc);
-            else wave_cgemm_h2<D, FULL>(Xp, Bpp[cur], row0, col0, acc);
+            if (lead_only(jj)) wave_cgemm_h2<D, 1, true>(Xp, Bpp[cur], row0, col0, acc);
+            else wave_cgemm_h2<D, FULL, true>(Xp, Bpp[cur], row0, col0, acc);
             bracket(jj, true);
             cur ^= 1;
             write_q(jj, Bpp[cur]);   // every wave finished reading this set before the last barrier
@@ -418,7 +426,7 @@ __global__ __launch_bounds__(64 * HShape<D>::G) void cheb_h2_kernel(MixedArgs a)
         }
         if (pre && tid < a.ncoef) coef[tid] = scaled(coef_finish_h(a.ctl, raw));
         coef_ready = pre;
-        wave_cgemm_h2<D, FULL>(Xp, Bpp[cur], row0, col0, acc);   // S^2 R = S^2 Y^2 Q_1
+        wave_cgemm_h2<D, FULL, true>(Xp, Bpp[cur], row0, col0, acc);   // S^2 R = S^2 Y^2 Q_1
         // E = (m_0 - 1) I + m_1 Y + Y^2 Q_1 in FP64, held scaled by S like every tree node of the
         // fold (exact), so that the fold's plane writes need no multiply
         const double m0m1 = a.are[0] * kScale;
