@@ -1433,10 +1433,9 @@ template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, dou
 // kernels, one slice kernel and one tree-level kernel per class and level; larger instances run
 // the step-wise path on batched GEMMs.  The classes and the large instances go on separate
 // branches (streams forked from and joined to the ensemble's stream), the whole solve is
-// captured once as a hipGraph and replayed.  The tuner picks, per class, the wave split, the
-// tree block size and the persistent grid, and per band of 16 dims the output tile of the large
-// instances' GEMMs, by timing each candidate on the ensemble itself; large instances run on up to
-// four branches.
+// captured once as a hipGraph and replayed.  The tuner picks, per class, the tree block size and
+// the persistent grid, and per band of 16 dims the output tile of the large instances' GEMMs, by
+// timing each candidate on the ensemble itself; large instances run on up to four branches.
 constexpr int kEnsSide = 8;        // side streams: 3 classes + up to 5 large-instance branches
 constexpr int kEnsClasses = 3;     // dim classes 16, 32, 64
 
@@ -1787,19 +1786,19 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
     for (int c = 0; c < kEnsClasses; ++c) {
         if (!e->cls[c].n_slice) continue;
         const int D = 16 << c;
-        // wave split and tree block size on the default grid first, then the persistent grid size
-        // for the best of those (the grid size has not changed the choice of the other two)
+        // the tree block size on the default grid first, then the persistent grid size for the
+        // better of those (the grid size has not changed the block size chosen).  One wave split
+        // (T = 1): the split of a slice over two waves' tiles was slower in every class whenever
+        // it was timed (512-system ensemble, every run).
         float best = 1e30f;
         auto trial = [&](const EnsCfg& cf) {
             const float t = ens_time(e, [&] { return ens_enqueue_class(e, c, cf, e->stream); }, 1);
             if (t < best) { best = t; e->cfg[c] = cf; }
         };
-        for (int T = 1; T <= 2; ++T) {
-            if (!pa::ens_shape_valid(D, T)) continue;
-            for (int logb = 2; logb <= 3; ++logb) {
-                EnsCfg cf; cf.T = T; cf.logb = logb; cf.grid_mult = 0;
-                trial(cf);
-            }
+        if (!pa::ens_shape_valid(D, 1)) continue;
+        for (int logb = 2; logb <= 3; ++logb) {
+            EnsCfg cf; cf.T = 1; cf.logb = logb; cf.grid_mult = 0;
+            trial(cf);
         }
         const EnsCfg base = e->cfg[c];
         for (int gm : {2, 8}) {
