@@ -327,8 +327,8 @@ def ensemble_bench(calls):
 
 
 def host_cpu():
-    """CPU model name and the number of host threads the baselines use (BLAS threads; the box's
-    share of its host is 16 threads per GPU, set by OMP_NUM_THREADS)."""
+    """CPU model name and the number of host threads the baselines use (BLAS threads: the
+    machine's share of host threads per GPU, as OMP_NUM_THREADS sets it)."""
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:

@@ -131,7 +131,13 @@ LIBSPEC double OneNorm_fp64(cuDoubleComplex *mat, unsigned int dim);
  *       power form (dim <= 64, tau Hnorm <= 1), bf16 three-way splits (six bf16 products)
  *       otherwise.  The environment variable PARAMENT_MIXED_SPLIT=bf16x3 (or f16x4, all
  *       four fp16 products) selects the other form of the fused kernel, for precision
- *       studies; it is read once per process.  Default of complex64 contexts, which keep the FP32
+ *       studies; it is read once per process.  Its accuracy (within 1e-5 of FP64 at the
+ *       benchmark workloads) depends on the slices' actual norm ||tau X_j||, which the order M
+ *       and the power form do not see: as in the reference, Hnorm sums the operators' norms
+ *       without the control amplitudes.  Measured at the dim-64, 65,536-step workload with the
+ *       controls scaled so that max ||tau X_j||_2 = 0.22 / 0.71 / 1.38: 6.5e-6 / 1.2e-5 / 1.7e-4;
+ *       controls much larger than 1 in modulus call for NATIVE (FP64) or a smaller dt.
+ *       Default of complex64 contexts, which keep the FP32
  *       iteration table's M and, for dims <= 16 on the automatic kernel path, the FP32
  *       wavefront-resident kernels.  A complex128 context in this mode takes M from the FP32
  *       table raised until the accumulated truncation error stays below 2e-6.

@@ -9,10 +9,13 @@
 // exact in FP32), and a product is the sum h0 h0' + h0 h1' + h1 h0' (+ h1 h1' when FULL = 4) of
 // fp16 products accumulated in FP32: three (four) matrix-core products per FP32-accurate product
 // instead of the six of the bf16 three-way split, and two operand planes instead of three.  The
-// scale keeps every operand of the power form in fp16's normal range: |Y|, |Y^2|, |Q_j| and the
-// tree nodes |E| = |U - I| are at most 2 here (tau Hnorm <= 1, unitary slices), so S x stays below
-// 2^15 < 65504, and an entry whose low part falls below fp16's normal range loses at most
-// 2^-25 / S = 2^-39 in absolute terms.  The accumulated products carry S^2, removed exactly by the
+// scale keeps the operands in fp16's range: an entry is at most the spectral norm of its matrix,
+// |Y|, |Y^2| <= ||tau X||_2^2 and |Q_j| < 1 while ||tau X||_2 <= 1 (for controls of modulus <= 1,
+// ||tau X||_2 <= tau Hnorm <= 1 here), and the tree nodes |E| = |U - I| <= 2 for unitary slices,
+// so S x stays below 2^15 < 65504 (overflow needs an entry of 4).  An entry whose low part falls
+// below fp16's normal range loses at most 2^-25 / S = 2^-39 in absolute terms.  The 22-bit
+// products' rounding grows with ||tau X||_2 (controls well above 1 in modulus): see the precision
+// modes in parament.h.  The accumulated products carry S^2, removed exactly by the
 // power-of-two factor 2^-28.  The Horner brackets Q_j, the slice result E and the fold's tree
 // nodes are held multiplied by S (exact), so that their planes are written without a multiply;
 // the work-group's node is divided by S when it is stored.

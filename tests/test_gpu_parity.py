@@ -295,6 +295,22 @@ def test_full_size_heisenberg6_mixed_vs_oracle():
     assert np.linalg.norm(u.conj().T @ u - np.eye(64)) < 2 * 1e-5 * np.sqrt(64)
 
 
+def test_full_size_heisenberg6_mixed_larger_controls():
+    """The mixed path's error grows with the slices' actual norm, which M and the power form do not
+    see (Hnorm leaves out the control amplitudes, as in the reference): with the headline controls
+    scaled by 4 (max ||tau X_j||_2 = 0.71 against 0.22) it measured 1.2e-5 against FP64 at the
+    same M.  Held below 2e-5, finite, and above the headline's error (so the scaled controls did
+    reach the kernel)."""
+    h0, hk, c, dt, quad = _headline(False)
+    u4, m = run_gpu("fp64", h0, hk, 4.0 * c, dt, quad, mixed=True)
+    ref4, _ = run_gpu("fp64", h0, hk, 4.0 * c, dt, quad, cycles=m)
+    u1, _ = run_gpu("fp64", h0, hk, c, dt, quad, mixed=True)
+    ref1, _ = run_gpu("fp64", h0, hk, c, dt, quad, cycles=m)
+    e4, e1 = rel_err(u4, ref4), rel_err(u1, ref1)
+    assert np.isfinite(u4).all()
+    assert e1 < e4 < 2e-5, (e1, e4)
+
+
 def test_full_size_heisenberg6_complex64_vs_oracle():
     """complex64 context (the reference's fp32 interface, M = 7 from the FP32 table) against the
     oracle on the complex64-rounded inputs at M = 7.  The default arithmetic (FP64 first-order
