@@ -491,38 +491,77 @@ template hipError_t launch_tree<double>(const void*, bool, int, int, double2*, d
 // ---------------------------------------------------------------------------------------
 // Generic step-wise kernels (any dim): the reference's launch structure
 // (parament.cpp:486-718) with the diagonal add and the scaling fused into the GEMM.
+// Assembly X_j = H0 + sum_t c'_tj O_t, one thread per matrix element over kAsmSlices consecutive
+// slices: the element's operator values are read once and used for every slice of the group
+// (one read per slice had the operators streamed from the cache hierarchy once per slice,
+// (1 + ncoef) x 16 bytes per element of every slice).  Per slice the sum keeps its order.
+// Dynamic LDS: the group's coefficients, kAsmSlices x ncoef.
+constexpr int kAsmSlices = 8;
+
+template <class R>
+__device__ __forceinline__ void asm_group_coefficients(const ChebArgs<R>& a, int j0, int ns, cplx<R>* coef,
+                                                       bool local_index) {
+    for (int idx = threadIdx.x; idx < ns * a.ncoef; idx += blockDim.x) {
+        const int s = idx / a.ncoef, t = idx % a.ncoef;
+        coef[idx] = slice_coefficient(a.ctl, t, (local_index ? 0 : a.j0) + j0 + s);
+    }
+    __syncthreads();
+}
+
+// x[s] = X_{j0+s} at element e (s < kAsmSlices; entries past the group's ns slices are unused)
+template <class R>
+__device__ __forceinline__ void asm_group_element(const ChebArgs<R>& a, const cplx<R>* coef, int e, int dd,
+                                                  cplx<R> (&x)[kAsmSlices]) {
+    const cplx<R> h = a.ops[e];
+#pragma unroll
+    for (int s = 0; s < kAsmSlices; ++s) x[s] = h;
+    for (int t = 0; t < a.ncoef; ++t) {
+        const cplx<R> o = a.ops[(size_t)(t + 1) * dd + e];
+#pragma unroll
+        for (int s = 0; s < kAsmSlices; ++s) x[s] = cadd(x[s], cmul(coef[s * a.ncoef + t], o));
+    }
+}
+
 template <class R, class TO>
 __global__ __launch_bounds__(256) void assemble_kernel(ChebArgs<R> a, TO* __restrict__ X, int blocks_per) {
-    __shared__ cplx<R> coef[kMaxCoeffs];
-    const int j = blockIdx.x / blocks_per, b = blockIdx.x % blocks_per;
-    for (int t = threadIdx.x; t < a.ncoef; t += blockDim.x) coef[t] = slice_coefficient(a.ctl, t, a.j0 + j);
-    __syncthreads();
+    extern __shared__ __attribute__((aligned(16))) unsigned char asm_lds[];
+    cplx<R>* coef = reinterpret_cast<cplx<R>*>(asm_lds);
+    const int j0 = (blockIdx.x / blocks_per) * kAsmSlices, b = blockIdx.x % blocks_per;
+    const int ns = min(kAsmSlices, a.N - j0);
+    asm_group_coefficients<R>(a, j0, ns, coef, false);
     const int dd = a.d * a.d;
     const int e = b * 256 + threadIdx.x;
     if (e >= dd) return;
-    cplx<R> x = a.ops[e];
-    for (int t = 0; t < a.ncoef; ++t) x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + e]));
-    TO o;
-    o.x = x.x; o.y = x.y;
-    X[(size_t)j * dd + e] = o;
+    cplx<R> x[kAsmSlices];
+    asm_group_element<R>(a, coef, e, dd, x);
+#pragma unroll
+    for (int s = 0; s < kAsmSlices; ++s)
+        if (s < ns) {
+            TO o;
+            o.x = x[s].x; o.y = x[s].y;
+            X[(size_t)(j0 + s) * dd + e] = o;
+        }
+}
+
+template <class R, class TO>
+static hipError_t launch_assemble_to(const ChebArgs<R>& a, TO* X, hipStream_t st) {
+    if (a.ncoef > kMaxCoeffs || a.N <= 0) return hipErrorInvalidValue;
+    const int blocks_per = (a.d * a.d + 255) / 256;
+    const unsigned groups = (unsigned)((a.N + kAsmSlices - 1) / kAsmSlices);
+    const size_t lds = (size_t)kAsmSlices * a.ncoef * sizeof(cplx<R>);
+    hipLaunchKernelGGL((assemble_kernel<R, TO>), dim3(groups * blocks_per), dim3(256), lds, st, a, X, blocks_per);
+    return hipGetLastError();
 }
 
 template <class R>
 hipError_t launch_assemble(const ChebArgs<R>& a, cplx<R>* X, hipStream_t st) {
-    if (a.ncoef > kMaxCoeffs) return hipErrorInvalidValue;
-    const int blocks_per = (a.d * a.d + 255) / 256;
-    hipLaunchKernelGGL((assemble_kernel<R, cplx<R>>), dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, X, blocks_per);
-    return hipGetLastError();
+    return launch_assemble_to<R, cplx<R>>(a, X, st);
 }
 template hipError_t launch_assemble<float>(const ChebArgs<float>&, float2*, hipStream_t);
 template hipError_t launch_assemble<double>(const ChebArgs<double>&, double2*, hipStream_t);
 
 hipError_t launch_assemble_c64(const ChebArgs<double>& a, float2* X, hipStream_t st) {
-    if (a.ncoef > kMaxCoeffs) return hipErrorInvalidValue;
-    const int blocks_per = (a.d * a.d + 255) / 256;
-    hipLaunchKernelGGL((assemble_kernel<double, float2>), dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, X,
-                       blocks_per);
-    return hipGetLastError();
+    return launch_assemble_to<double, float2>(a, X, st);
 }
 
 __global__ __launch_bounds__(256) void widen_kernel(const float2* __restrict__ in, long total, double2* __restrict__ out) {
@@ -548,31 +587,41 @@ hipError_t launch_narrow(const double2* in, long n, float2* out, hipStream_t st)
     return hipGetLastError();
 }
 
-// E_j = (a_0 - 1) I + a_1 s X_j + R_j in FP64 (X_j recomputed from the operators), rounded once
-// to complex64: the first-order part of the mixed-precision slice propagator.
+// E_j = (a_0 - 1) I + a_1 s X_j + R_j in FP64 (X_j recomputed from the operators, kAsmSlices
+// slices per thread as in assemble_kernel), rounded once to complex64: the first-order part of
+// the mixed-precision slice propagator.
 __global__ __launch_bounds__(256) void first_order_kernel(ChebArgs<double> a, const float2* __restrict__ R,
                                                           float2* __restrict__ E, int blocks_per) {
-    __shared__ double2 coef[kMaxCoeffs];
-    const int j = blockIdx.x / blocks_per, b = blockIdx.x % blocks_per;
-    for (int t = threadIdx.x; t < a.ncoef; t += blockDim.x) coef[t] = slice_coefficient(a.ctl, t, j);
-    __syncthreads();
+    extern __shared__ __attribute__((aligned(16))) unsigned char fo_lds[];
+    double2* coef = reinterpret_cast<double2*>(fo_lds);
+    const int j0 = (blockIdx.x / blocks_per) * kAsmSlices, b = blockIdx.x % blocks_per;
+    const int ns = min(kAsmSlices, a.N - j0);
+    asm_group_coefficients<double>(a, j0, ns, coef, true);
     const int d = a.d, dd = d * d;
     const int e = b * 256 + threadIdx.x;
     if (e >= dd) return;
-    double2 x = a.ops[e];
-    for (int t = 0; t < a.ncoef; ++t) x = cadd(x, cmul(coef[t], a.ops[(size_t)(t + 1) * dd + e]));
+    double2 x[kAsmSlices];
+    asm_group_element<double>(a, coef, e, dd, x);
     const double2 a1s = mk<double>(a.are[1] * a.s, a.aim[1] * a.s);
-    const double2 f = cmul(a1s, x);
-    const float2 r = R[(size_t)j * dd + e];
-    const double er = f.x + (double)r.x + ((e / d == e % d) ? a.are[0] : 0.0);
-    const double ei = f.y + (double)r.y;
-    E[(size_t)j * dd + e] = make_float2((float)er, (float)ei);
+    const double dg = (e / d == e % d) ? a.are[0] : 0.0;
+#pragma unroll
+    for (int s = 0; s < kAsmSlices; ++s)
+        if (s < ns) {
+            const size_t o = (size_t)(j0 + s) * dd + e;
+            const double2 f = cmul(a1s, x[s]);
+            const float2 r = R[o];
+            const double er = f.x + (double)r.x + dg;
+            const double ei = f.y + (double)r.y;
+            E[o] = make_float2((float)er, (float)ei);
+        }
 }
 
 hipError_t launch_first_order(const ChebArgs<double>& a, const float2* R, float2* E, hipStream_t st) {
-    if (a.ncoef > kMaxCoeffs) return hipErrorInvalidValue;
+    if (a.ncoef > kMaxCoeffs || a.N <= 0) return hipErrorInvalidValue;
     const int blocks_per = (a.d * a.d + 255) / 256;
-    hipLaunchKernelGGL(first_order_kernel, dim3((unsigned)a.N * blocks_per), dim3(256), 0, st, a, R, E, blocks_per);
+    const unsigned groups = (unsigned)((a.N + kAsmSlices - 1) / kAsmSlices);
+    const size_t lds = (size_t)kAsmSlices * a.ncoef * sizeof(double2);
+    hipLaunchKernelGGL(first_order_kernel, dim3(groups * blocks_per), dim3(256), lds, st, a, R, E, blocks_per);
     return hipGetLastError();
 }
 

@@ -221,6 +221,21 @@ def test_mixed_parity(mode, magnus, d):
     assert rel_err(u, ref) < 1e-5
 
 
+@pytest.mark.parametrize("amps", [3, 4, 6])
+@pytest.mark.parametrize("d", [32, 48, 64])
+def test_mixed_parity_control_counts(d, amps):
+    """The fp16 split kernel assembles the slice Hamiltonian with every operator load of an
+    iteration issued at once for up to three operators (a fourth and later slot reads H0 again,
+    weighted by zero) and in a loop beyond; both against the FP64 restatement, with complex
+    controls of modulus up to 1 (power form), Simpson quadrature."""
+    rng = np.random.default_rng(5300 + 10 * d + amps)
+    h = [gue(rng, d) for _ in range(amps + 1)]
+    c = rng.uniform(-1, 1, (amps, 257)) * np.exp(1j * rng.uniform(0, 2 * np.pi, (amps, 257)))
+    u, m = run_gpu("fp64", h[0], h[1:], c, 0.01, "simpson", mixed=True)
+    ref = eo.equiprop(h[0], h[1:], c, 0.01, mode="simpson", cycles=m)
+    assert rel_err(u, ref) < 1e-5
+
+
 @pytest.mark.parametrize("d,n", [(d, n) for d in (8, 64) for n in (255, 256, 257, 259, 1024, 1025, 1026, 1027, 1029)]
                          + [(8, n) for n in (4096, 4097, 4098, 4099, 4101, 4103, 4105)] + [(64, 4099), (64, 4103)])
 def test_mixed_fused_tree_levels(d, n):
