@@ -1,5 +1,5 @@
 // Batched complex GEMM on the gfx950 matrix cores for the step-wise (any-dim) path:
-//     C_b = alpha A_b B_b + beta Cin_b + diag I  (+ A_b + B_b when add_ab)
+//     C_b = alpha A_b B_b + beta Cin_b + beta2 Cin2_b + diag I  (+ A_b + B_b when add_ab)
 // over a batch of d x d row-major complex matrices, with the arithmetic of the products set by
 // the mode: FP64 (16x16x4 f64 instruction), FP32 (16x16x4 f32), or bf16 split products
 // (16x16x32 bf16; every FP32 operand is split into 2 or 3 bf16 slices, s0 = bf16(x),
@@ -54,8 +54,33 @@ __device__ __forceinline__ void st(float2* p, long i, double2 v) { p[i] = make_f
 
 template <class TA, class TB, class TC, class TO>
 struct EpiIO {
-    const TA* A; const TB* B; const TC* Cin; TO* C;
+    const TA* A; const TB* B; const TC* Cin; const TC* Cin2; TO* C;
 };
+
+// Output element o of product b from the accumulated x: alpha x + beta Cin + beta2 Cin2
+// (+ A + B) + diag, in FP64, rounded once.  Each element reads its own Cin / Cin2 positions
+// before it is written, so C may be one of them.  A, B point at product b's operands.
+template <class TA, class TB, class TC, class TO>
+__device__ __forceinline__ void epi_store(const GemmArgs& g, const EpiIO<TA, TB, TC, TO>& io, const TA* A,
+                                          const TB* B, long b, long dd, long o, bool on_diag, double xr, double xi) {
+    double2 y = mk<double>(g.alpha_re * xr - g.alpha_im * xi, g.alpha_re * xi + g.alpha_im * xr);
+    if (io.Cin) {
+        const double2 c = ld(io.Cin, b * g.sc * dd + o);
+        y.x += g.beta_re * c.x - g.beta_im * c.y;
+        y.y += g.beta_re * c.y + g.beta_im * c.x;
+    }
+    if (io.Cin2) {
+        const double2 c = ld(io.Cin2, b * g.sc2 * dd + o);
+        y.x += g.beta2_re * c.x - g.beta2_im * c.y;
+        y.y += g.beta2_re * c.y + g.beta2_im * c.x;
+    }
+    if (g.add_ab) {
+        const double2 a = ld(A, o), bb = ld(B, o);
+        y.x += a.x + bb.x; y.y += a.y + bb.y;
+    }
+    if (on_diag) { y.x += g.diag_re; y.y += g.diag_im; }
+    st(io.C, b * g.so * dd + o, y);
+}
 
 // FP64 / FP32 matrix-core tiles: A chunk [64][32] (stride SA), B chunk [32][64] (stride SB).
 // The next K chunk is fetched into registers while the matrix cores work on the current one.
@@ -111,7 +136,6 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
         __syncthreads();
     }
     // epilogue in FP64
-    const long cb = b * g.so * dd, ib = b * g.sc * dd;
 #pragma unroll
     for (int m = 0; m < TW; ++m)
 #pragma unroll
@@ -120,20 +144,7 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
             for (int v = 0; v < 4; ++v) {
                 const int gi = ti * BT + acc_row<R>(wr, m, v), gj = tj * BT + acc_col(wc, n);
                 if (gi >= d || gj >= d) continue;
-                const long o = (long)gi * d + gj;
-                const double xr = acc.re[m][n][v], xi = acc.im[m][n][v];
-                double2 y = mk<double>(g.alpha_re * xr - g.alpha_im * xi, g.alpha_re * xi + g.alpha_im * xr);
-                if (io.Cin) {
-                    const double2 c = ld(io.Cin, ib + o);
-                    y.x += g.beta_re * c.x - g.beta_im * c.y;
-                    y.y += g.beta_re * c.y + g.beta_im * c.x;
-                }
-                if (g.add_ab) {
-                    const double2 a = ld(A, o), bb = ld(B, o);
-                    y.x += a.x + bb.x; y.y += a.y + bb.y;
-                }
-                if (gi == gj) { y.x += g.diag_re; y.y += g.diag_im; }
-                st(io.C, cb + o, y);
+                epi_store(g, io, A, B, b, dd, (long)gi * d + gj, gi == gj, acc.re[m][n][v], acc.im[m][n][v]);
             }
 }
 
@@ -189,27 +200,13 @@ __global__ __launch_bounds__(64 * NW) void zgemm_row_kernel(GemmArgs g, EpiIO<TA
         wave_cgemm<R, 1, NW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, 0, acc, min(KT, (d - k0 + 3) & ~3));
         __syncthreads();
     }
-    const long cb = b * g.so * dd, ib = b * g.sc * dd;
 #pragma unroll
     for (int n = 0; n < NW; ++n)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int gi = acc_row<R>(wr, 0, v), gj = acc_col(0, n);
             if (gi >= d || gj >= d) continue;
-            const long o = (long)gi * d + gj;
-            const double xr = acc.re[0][n][v], xi = acc.im[0][n][v];
-            double2 y = mk<double>(g.alpha_re * xr - g.alpha_im * xi, g.alpha_re * xi + g.alpha_im * xr);
-            if (io.Cin) {
-                const double2 c = ld(io.Cin, ib + o);
-                y.x += g.beta_re * c.x - g.beta_im * c.y;
-                y.y += g.beta_re * c.y + g.beta_im * c.x;
-            }
-            if (g.add_ab) {
-                const double2 a = ld(A, o), bb = ld(B, o);
-                y.x += a.x + bb.x; y.y += a.y + bb.y;
-            }
-            if (gi == gj) { y.x += g.diag_re; y.y += g.diag_im; }
-            st(io.C, cb + o, y);
+            epi_store(g, io, A, B, b, dd, (long)gi * d + gj, gi == gj, acc.re[0][n][v], acc.im[0][n][v]);
         }
 }
 
@@ -330,7 +327,6 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             }
         __syncthreads();
     }
-    const long cb = b * g.so * dd, ib = b * g.sc * dd;
 #pragma unroll
     for (int m = 0; m < TW; ++m)
 #pragma unroll
@@ -339,21 +335,9 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             for (int v = 0; v < 4; ++v) {
                 const int gi = ti * BT + wr + 16 * m + 4 * kq + v, gj = tj * BT + wc + 16 * n + r16;
                 if (gi >= d || gj >= d) continue;
-                const long o = (long)gi * d + gj;
-                const double xr = (double)accr[0][m][n][v] + (double)accr[1][m][n][v];
-                const double xi = (double)acci[0][m][n][v] + (double)acci[1][m][n][v];
-                double2 y = mk<double>(g.alpha_re * xr - g.alpha_im * xi, g.alpha_re * xi + g.alpha_im * xr);
-                if (io.Cin) {
-                    const double2 c = ld(io.Cin, ib + o);
-                    y.x += g.beta_re * c.x - g.beta_im * c.y;
-                    y.y += g.beta_re * c.y + g.beta_im * c.x;
-                }
-                if (g.add_ab) {
-                    const double2 a = ld(A, o), bb = ld(B, o);
-                    y.x += a.x + bb.x; y.y += a.y + bb.y;
-                }
-                if (gi == gj) { y.x += g.diag_re; y.y += g.diag_im; }
-                st(io.C, cb + o, y);
+                epi_store(g, io, A, B, b, dd, (long)gi * d + gj, gi == gj,
+                          (double)accr[0][m][n][v] + (double)accr[1][m][n][v],
+                          (double)acci[0][m][n][v] + (double)acci[1][m][n][v]);
             }
 }
 
@@ -395,7 +379,7 @@ template <class TA, class TB, class TC, class TO>
 hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
     EpiIO<TA, TB, TC, TO> io;
     io.A = static_cast<const TA*>(g.A); io.B = static_cast<const TB*>(g.B);
-    io.Cin = static_cast<const TC*>(g.Cin); io.C = static_cast<TO*>(g.C);
+    io.Cin = static_cast<const TC*>(g.Cin); io.Cin2 = static_cast<const TC*>(g.Cin2); io.C = static_cast<TO*>(g.C);
     // complex64 FP32 products at dims 33-128: the whole-product kernel when asked for (tile 16),
     // and by default where it measured faster than the 32/64 tiles (dims 33-48: 27-28%,
     // 97-112: 5-6%; at 49-96 and 113-128 it was level or slower)
@@ -410,7 +394,7 @@ hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
 }  // namespace
 
 // Operand / output element types: complex128 when the flag is set, complex64 otherwise; Cin
-// has the output's type.  FP64 products take complex64 operands (the transition from
+// and Cin2 have the output's type.  FP64 products take complex64 operands (the transition from
 // low-precision tree levels) and complex128 ones; the FP32 and split modes run on complex64.
 hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st) {
     if (g.d <= 0 || g.batch < 0) return hipErrorInvalidValue;

@@ -227,7 +227,7 @@ template <class R> struct Context {
     C* d_c = nullptr; size_t cap_c = 0;
     void* d_leaves = nullptr; size_t cap_leaves = 0;
     void* d_nodes = nullptr; size_t cap_nodes = 0;
-    C* d_gen[3] = {nullptr, nullptr, nullptr}; size_t cap_gen = 0;
+    C* d_gen[4] = {nullptr, nullptr, nullptr, nullptr}; size_t cap_gen = 0; int n_gen = 0;
     void* d_root = nullptr; size_t cap_root = 0;  // d x d result of the last equiprop
     void* d_c64 = nullptr; size_t cap_c64 = 0;    // complex64 contexts: controls widened to FP64
     void* d_out64 = nullptr; size_t cap_out64 = 0;  // complex64 contexts: FP64 result before rounding
@@ -262,7 +262,7 @@ template <class R> void free_working(Context<R>* h) {
     hipFree(h->d_leaves); h->d_leaves = nullptr; h->cap_leaves = 0;
     hipFree(h->d_nodes); h->d_nodes = nullptr; h->cap_nodes = 0;
     for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
-    h->cap_gen = 0;
+    h->cap_gen = 0; h->n_gen = 0;
     hipFree(h->d_root); h->d_root = nullptr; h->cap_root = 0;
     hipFree(h->d_c64); h->d_c64 = nullptr; h->cap_c64 = 0;
     hipFree(h->d_out64); h->d_out64 = nullptr; h->cap_out64 = 0;
@@ -299,16 +299,18 @@ struct Workspace {
     }
 };
 
-// Batched complex GEMM on the matrix cores (gemm.hip): C = alpha A B + beta Cin + diag I
-// (+ A + B with add_ab); pointers with their element type (true = complex128), strides in
+// Batched complex GEMM on the matrix cores (gemm.hip): C = alpha A B + beta Cin + beta2 Cin2 +
+// diag I (+ A + B with add_ab); pointers with their element type (true = complex128), strides in
 // matrices.
 struct GemmOp { const void* p; bool c128; long stride; };
 hipError_t zgemm(int mode, GemmOp A, GemmOp B, GemmOp Cin, void* C, bool c_c128, long so, double2 alpha, double2 beta,
-                 double2 diag, int d, int batch, hipStream_t st, bool add_ab = false, int tile = 0) {
+                 double2 diag, int d, int batch, hipStream_t st, bool add_ab = false, int tile = 0,
+                 GemmOp Cin2 = GemmOp{nullptr, false, 0}, double2 beta2 = double2{0.0, 0.0}) {
     pa::GemmArgs g;
     g.tile = tile;
     g.A = A.p; g.sa = A.stride; g.B = B.p; g.sb = B.stride; g.Cin = Cin.p; g.sc = Cin.stride; g.C = C; g.so = so;
     g.alpha_re = alpha.x; g.alpha_im = alpha.y; g.beta_re = beta.x; g.beta_im = beta.y;
+    g.Cin2 = Cin2.p; g.sc2 = Cin2.stride; g.beta2_re = beta2.x; g.beta2_im = beta2.y;
     g.diag_re = diag.x; g.diag_im = diag.y;
     g.d = d; g.batch = batch; g.add_ab = add_ab ? 1 : 0;
     g.a_c128 = A.c128; g.b_c128 = B.c128; g.out_c128 = c_c128;
@@ -358,6 +360,51 @@ hipError_t clenshaw_batched(const cplx<R>* X, int d, int cnt, int M, double s, c
     if (result) *result = b1;
     return r;
 }
+
+// Slices E_j = p(X_j) - I of a batch of cnt slices in the power form on batched GEMMs, for
+// tau Hnorm <= 1: p = sum_k m_k Y^k, Y = tau X (m from power_coefficients, m[0] = m_0 - 1), by
+// Horner's rule in S = Y^2 with J = (M - 1) / 2:
+//   B_{J-1} = m_{2J+1} S Y + m_{2J} S + m_{2J-1} Y + m_{2J-2} I,
+//   B_j     = S B_{j+1} + m_{2j+1} Y + m_{2j} I   (j = J-2 .. 0),   E = B_0,
+// (M + 1) / 2 GEMMs against the recurrence's M - 1, every Y term and diagonal in the FP64 GEMM
+// epilogue.  The polynomial is the one the recurrence evaluates; only the rounding differs.
+// With first_order false the last step leaves out m_1 Y + (m_0 - 1) I, which the caller adds in
+// FP64.  Buffers of cnt slices: S and w0, w1 as well when M >= 7; for M >= 5 the last step
+// overwrites X (which it reads in place).  *result: the buffer that holds E.
+template <class R>
+hipError_t power_batched(int mode, cplx<R>* X, int d, int cnt, int M, double tau, const double2* m, bool first_order,
+                         cplx<R>* S, cplx<R>* w0, cplx<R>* w1, cplx<R>** result, hipStream_t st, int tile = 0) {
+    const bool w = sizeof(R) == 8;
+    const int J = (M - 1) / 2;
+    if (J < 1 || (J >= 3 && !w1)) return hipErrorInvalidValue;
+    auto op = [&](const void* p) { return GemmOp{p, w, 1}; };
+    const GemmOp none{nullptr, w, 0};
+    hipError_t r = zgemm(mode, op(X), op(X), none, S, w, 1, c2(tau * tau), c2(0), c2(0), d, cnt, st, false, tile);
+    cplx<R>* cur = w0;
+    {
+        const int j = J - 1;
+        const bool lin = j > 0 || first_order;   // the step's own Y term and diagonal
+        if (r == hipSuccess)
+            r = zgemm(mode, op(S), op(X), op(S), cur, w, 1, cscale(tau, m[2 * J + 1]), m[2 * J], lin ? m[2 * j] : c2(0), d,
+                      cnt, st, false, tile, lin ? op(X) : none, lin ? cscale(tau, m[2 * j + 1]) : c2(0));
+    }
+    for (int j = J - 2; j >= 0 && r == hipSuccess; --j) {
+        const bool lin = j > 0 || first_order;
+        cplx<R>* o = j == 0 ? X : (cur == w0 ? w1 : w0);
+        r = zgemm(mode, op(S), op(cur), lin ? op(X) : none, o, w, 1, c2(1), lin ? cscale(tau, m[2 * j + 1]) : c2(0),
+                  lin ? m[2 * j] : c2(0), d, cnt, st, false, tile);
+        cur = o;
+    }
+    if (result) *result = cur;
+    return r;
+}
+
+// Whether the power form applies at x = tau Hnorm: x <= 1 (the power coefficients are then close
+// to (-i)^k / k! and the form is well conditioned) and x^M a normal double (power_coefficients
+// divides by x^k; a manual M far above the table's at a tiny x would underflow it).
+inline bool power_form_ok(int M, double x) { return x <= 1.0 && std::pow(x, M) >= 1e-300; }
+// The step-wise complex64 paths take it from M = 5 on (at M = 3 both forms take two GEMMs).
+inline bool stepwise_power(int M, double x) { return M >= 5 && power_form_ok(M, x); }
 
 template <class R> Parament_ErrorCode bind_device(Context<R>* h) {
     if (h->bound) {
@@ -542,37 +589,39 @@ constexpr int kLowTreeLevels = 3;   // tree levels run in the low-precision prod
 
 // The three step-wise working arrays (X, b_k, b_{k+1}) of `bytes` each: the context's, grown
 // together, or three pieces of a fixed workspace.
-template <class R> bool gen_buffers(Context<R>* h, Workspace& ws, size_t bytes, void* out[3]) {
+// n (3 or 4) step-wise working buffers of bytes each, grow-only.
+template <class R> bool gen_buffers(Context<R>* h, Workspace& ws, size_t bytes, void* out[], int n = 3) {
     if (ws.fixed) {
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < n; ++i)
             if (!(out[i] = ws.get(nullptr, nullptr, bytes))) return false;
         return true;
     }
-    if (h->cap_gen < bytes) {
+    if (h->cap_gen < bytes || h->n_gen < n) {
         for (auto& p : h->d_gen) { hipFree(p); p = nullptr; }
-        h->cap_gen = 0;
-        for (auto& p : h->d_gen)
-            if (hipMalloc(&p, bytes) != hipSuccess) {
+        h->cap_gen = 0; h->n_gen = 0;
+        for (int i = 0; i < n; ++i)
+            if (hipMalloc(&h->d_gen[i], bytes) != hipSuccess) {
                 hipGetLastError();
                 for (auto& q : h->d_gen) { hipFree(q); q = nullptr; }
                 return false;
             }
-        h->cap_gen = bytes;
+        h->cap_gen = bytes; h->n_gen = n;
     }
-    for (int i = 0; i < 3; ++i) out[i] = h->d_gen[i];
+    for (int i = 0; i < n; ++i) out[i] = h->d_gen[i];
     return true;
 }
 
-// Step-wise form (any dim): the Clenshaw GEMMs batched over all slices on complex64 data with the
-// products of the context's mode, every GEMM epilogue in FP64.
+// Step-wise form (any dim): the Clenshaw GEMMs (or the power form's) batched over all slices on
+// complex64 data with the products of the context's mode, every GEMM epilogue in FP64.
 template <class R>
 Parament_ErrorCode run_mixed_stepwise(Context<R>* h, const MixedSrc& m, double2* out_dev, Workspace& ws) {
     const int d = (int)h->dim, N = m.N, M = m.M;
     const size_t dd = (size_t)d * d;
     const int mode = gemm_mode_of(h->precision);
     hipStream_t st = h->stream;
-    void* gen[3];
-    if (!gen_buffers(h, ws, (size_t)N * dd * sizeof(float2), gen)) return ws.failure();
+    const bool power = stepwise_power(M, m.tau * h->hnorm);
+    void* gen[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (!gen_buffers(h, ws, (size_t)N * dd * sizeof(float2), gen, power && M >= 7 ? 4 : 3)) return ws.failure();
     double are[pa::kMaxCheb], aim[pa::kMaxCheb];
     bessel(M, m.tau * h->hnorm, are, aim);
     pa::ChebArgs<double> ca;
@@ -585,30 +634,49 @@ Parament_ErrorCode run_mixed_stepwise(Context<R>* h, const MixedSrc& m, double2*
     float2* X = reinterpret_cast<float2*>(gen[0]);
     float2* b1 = reinterpret_cast<float2*>(gen[1]);
     float2* b2 = reinterpret_cast<float2*>(gen[2]);
-    const float2 aM = make_float2((float)are[M], (float)aim[M]);
-    const float2 saM = make_float2((float)(ca.s * are[M]), (float)(ca.s * aim[M]));
+    float2* rd = b1;   // the slices E_j
+    float2* wr = X;    // the other half of the tree's ping-pong
     mark(h, 0);
-    if (pa::launch_assemble_c64(ca, X, st) != hipSuccess ||
-        pa::launch_fill<float>(nullptr, make_float2(0, 0), aM, d, N, b2, st) != hipSuccess ||
-        pa::launch_fill<float>(X, saM, make_float2((float)are[M - 1], (float)aim[M - 1]), d, N, b1, st) != hipSuccess)
-        return PARAMENT_STATUS_CUBLAS_FAILED;
-    // b_k = s X b_{k+1} - b_{k+2} + a_k I, k = M-2 .. 2, then b_1' (no a_1 I)
-    for (int k = M - 2; k >= 1; --k) {
-        const double2 diag = (k >= 2) ? c2(are[k], aim[k]) : c2(0);
-        if (zgemm(mode, GemmOp{X, false, 1}, GemmOp{b1, false, 1}, GemmOp{b2, false, 1}, b2, false, 1, c2(ca.s), c2(-1),
-                  diag, d, N, st) != hipSuccess)
+    if (pa::launch_assemble_c64(ca, X, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (power) {
+        // R = E - m_1 Y - (m_0 - 1) I by the power form (into X), then E = R + m_1 Y + (m_0 - 1) I
+        // in FP64, in place
+        double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
+        double2 mc[pa::kMaxCheb];
+        power_coefficients(M, m.tau * h->hnorm, are, aim, mr, mi);
+        for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
+        if (power_batched<float>(mode, X, d, N, M, m.tau, mc, false, b1, b2, reinterpret_cast<float2*>(gen[3]), &rd, st) !=
+            hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
-        std::swap(b1, b2);
+        pa::ChebArgs<double> fa = ca;
+        fa.s = m.tau;
+        fa.are[0] = mr[0]; fa.aim[0] = mi[0]; fa.are[1] = mr[1]; fa.aim[1] = mi[1];
+        if (pa::launch_first_order(fa, rd, rd, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+        wr = (rd == X) ? b1 : X;
+    } else {
+        const float2 aM = make_float2((float)are[M], (float)aim[M]);
+        const float2 saM = make_float2((float)(ca.s * are[M]), (float)(ca.s * aim[M]));
+        if (pa::launch_fill<float>(nullptr, make_float2(0, 0), aM, d, N, b2, st) != hipSuccess ||
+            pa::launch_fill<float>(X, saM, make_float2((float)are[M - 1], (float)aim[M - 1]), d, N, b1, st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        // b_k = s X b_{k+1} - b_{k+2} + a_k I, k = M-2 .. 2, then b_1' (no a_1 I)
+        for (int k = M - 2; k >= 1; --k) {
+            const double2 diag = (k >= 2) ? c2(are[k], aim[k]) : c2(0);
+            if (zgemm(mode, GemmOp{X, false, 1}, GemmOp{b1, false, 1}, GemmOp{b2, false, 1}, b2, false, 1, c2(ca.s), c2(-1),
+                      diag, d, N, st) != hipSuccess)
+                return PARAMENT_STATUS_CUBLAS_FAILED;
+            std::swap(b1, b2);
+        }
+        // R = s X b_1' - 2 b_2 (into b2), then E = (a_0 - 1) I + a_1 s X + R in FP64 (into b1)
+        if (zgemm(mode, GemmOp{X, false, 1}, GemmOp{b1, false, 1}, GemmOp{b2, false, 1}, b2, false, 1, c2(ca.s), c2(-2),
+                  c2(0), d, N, st) != hipSuccess ||
+            pa::launch_first_order(ca, b2, b1, st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+        rd = b1;
+        wr = X;
     }
-    // R = s X b_1' - 2 b_2 (into b2), then E = (a_0 - 1) I + a_1 s X + R in FP64 (into b1)
-    if (zgemm(mode, GemmOp{X, false, 1}, GemmOp{b1, false, 1}, GemmOp{b2, false, 1}, b2, false, 1, c2(ca.s), c2(-2), c2(0),
-              d, N, st) != hipSuccess ||
-        pa::launch_first_order(ca, b2, b1, st) != hipSuccess)
-        return PARAMENT_STATUS_CUBLAS_FAILED;
     mark(h, 1);
-    // lowest tree levels on complex64 (E' = A + B + A B), ping-pong b1 <-> X
-    float2* rd = b1;
-    float2* wr = X;
+    // lowest tree levels on complex64 (E' = A + B + A B), ping-pong rd <-> wr
     int r = N;
     for (int l = 0; l < kLowTreeLevels && r > 1; ++l) {
         const int half = r / 2;
@@ -679,7 +747,7 @@ Parament_ErrorCode run_mixed_fused(Context<R>* h, const MixedSrc& m, double2* ou
     // needs (M + 1) / 2 products instead of M - 1 (the power coefficients are then close to
     // (-i)^k / k! and the form is well conditioned).
     const double xarg = m.tau * hn2;
-    ma.ps = (xarg <= 1.0) ? 1 : 0;
+    ma.ps = power_form_ok(M, xarg) ? 1 : 0;
     if (ma.ps) {
         power_coefficients(M, xarg, bre, bim, ma.are, ma.aim);
         ma.ps_levels = power_split_levels(M, xarg);
@@ -804,20 +872,32 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         return PARAMENT_STATUS_SUCCESS;
     }
 
-    // Step-wise path: X, then M-1 batched GEMM launches, then ceil(log2 N) tree levels.
-    void* gen[3];
-    if (!gen_buffers(h, ws, (size_t)N * dd * sizeof(C), gen)) return ws.failure();
+    // Step-wise path: X, then M-1 batched GEMM launches (complex64: (M+1)/2 in the power form when
+    // tau Hnorm <= 1; complex128 keeps the reference's recurrence), then ceil(log2 N) tree levels.
+    const bool power = sizeof(R) == 4 && stepwise_power(M, tau * h->hnorm);
+    void* gen[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (!gen_buffers(h, ws, (size_t)N * dd * sizeof(C), gen, power && M >= 7 ? 4 : 3)) return ws.failure();
     C* X = reinterpret_cast<C*>(gen[0]);
     C* w1 = reinterpret_cast<C*>(gen[1]);
     C* w2 = reinterpret_cast<C*>(gen[2]);
     mark(h, 0);
     if (pa::launch_assemble<R>(ca, X, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
-    // the slices as E = U - I (a_0 - 1 on the diagonal of the last step)
-    double2 ak[pa::kMaxCheb];
-    for (int k = 0; k <= M; ++k) ak[k] = c2(k == 0 ? a0m1 : (double)ca.are[k], (double)ca.aim[k]);
     C* leaves = nullptr;
-    if (clenshaw_batched<R>(X, (int)d, N, M, (double)ca.s, ak, w1, w2, nullptr, &leaves, st) != hipSuccess)
-        return PARAMENT_STATUS_CUBLAS_FAILED;
+    if (power) {
+        double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
+        double2 mc[pa::kMaxCheb];
+        power_coefficients(M, tau * h->hnorm, are, aim, mr, mi);
+        for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
+        if (power_batched<R>(pa::GEMM_F32, X, (int)d, N, M, tau, mc, true, w1, w2, reinterpret_cast<C*>(gen[3]), &leaves,
+                             st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+    } else {
+        // the slices as E = U - I (a_0 - 1 on the diagonal of the last step)
+        double2 ak[pa::kMaxCheb];
+        for (int k = 0; k <= M; ++k) ak[k] = c2(k == 0 ? a0m1 : (double)ca.are[k], (double)ca.aim[k]);
+        if (clenshaw_batched<R>(X, (int)d, N, M, (double)ca.s, ak, w1, w2, nullptr, &leaves, st) != hipSuccess)
+            return PARAMENT_STATUS_CUBLAS_FAILED;
+    }
     mark(h, 1);
     // E_j = U_j - I for the N slices; reduce level by level (parament.cpp:675-714) on deviations
     // from identity: E' = A + B + A B.
@@ -1273,7 +1353,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     ca.are[0] = (R)(are[0] - 1.0);  // slices are written as E_j = U_j - I
     // The derivative kernel evaluates the same polynomial's derivative in powers of tau X when
     // tau Hnorm <= 1 ((M-1)/2 + 1 product passes instead of M - 1).
-    if (tau * h->hnorm <= 1.0) {
+    if (power_form_ok(M, tau * h->hnorm)) {
         double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
         power_coefficients(M, tau * h->hnorm, are, aim, mr, mi);
         ca.ps = 1;
@@ -1455,6 +1535,10 @@ struct EnsInst {
     double are[pa::kMaxCheb], aim[pa::kMaxCheb];
 };
 
+// Working buffers of N slices that a large instance's forward takes: X, the recurrence's or the
+// power form's two, and a fourth for the power form at M >= 7.
+inline int ens_large_buffers(const EnsInst& x) { return (stepwise_power(x.M, x.tau * x.hnorm) && x.M >= 7) ? 4 : 3; }
+
 struct EnsClassPlan {
     int D = 0;
     int2* d_slice = nullptr; int n_slice = 0;
@@ -1604,7 +1688,7 @@ Parament_ErrorCode ens_set_hamiltonians(Ensemble* e, const cuComplex* H) {
         // working memory: classes: leaves N + two tree scratch buffers (the multi-instance tree
         // writes ceil(N / 4) nodes, launch_tree tree_scratch_nodes(N)); large: X, b1, b2
         x.work_off = e->n_work;
-        e->n_work += x.cls >= 0 ? ((size_t)x.N + 2 * ens_tree_scratch(x.N)) * dd : 3 * (size_t)x.N * dd;
+        e->n_work += x.cls >= 0 ? ((size_t)x.N + 2 * ens_tree_scratch(x.N)) * dd : ens_large_buffers(x) * (size_t)x.N * dd;
         if (x.cls < 0) e->large.push_back((int)i);
     }
     if (hipMalloc(&e->d_work, e->n_work * sizeof(float2)) != hipSuccess) { hipGetLastError(); e->d_work = nullptr; return PARAMENT_STATUS_DEVICE_ALLOC_FAILED; }
@@ -1681,8 +1765,9 @@ hipError_t ens_enqueue_class(Ensemble* e, int c, const EnsCfg& cf, hipStream_t s
     return r;
 }
 
-// One large instance: the step-wise path on batched FP32 matrix-core GEMMs (as Context<float>).
-// n_cap > 0 (the tuner): only the first n_cap slices, which is enough to rank the GEMM tiles.
+// One large instance: the step-wise path on batched FP32 matrix-core GEMMs (as Context<float>,
+// power form included).  n_cap > 0 (the tuner): only the first n_cap slices, which is enough to
+// rank the GEMM tiles.
 hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) {
     const EnsInst& x = e->in[i];
     const int d = (int)x.d, N = n_cap > 0 ? std::min(x.N, n_cap) : x.N, M = x.M;
@@ -1690,6 +1775,7 @@ hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) 
     float2* X = e->d_work + x.work_off;
     float2* b1 = X + (size_t)N * dd;
     float2* b2 = b1 + (size_t)N * dd;
+    float2* b3 = ens_large_buffers(x) > 3 ? b2 + (size_t)N * dd : nullptr;
     pa::ChebArgs<float> ca;
     ca.ops = e->d_ops + x.ops_off; ca.ncoef = (int)x.amps; ca.d = d; ca.N = N; ca.M = M; ca.s = (float)(2.0 / x.hnorm);
     ca.ctl.c = e->d_c + x.c_off; ca.ctl.pts = (int)x.pts; ca.ctl.amps_in = (int)x.amps; ca.ctl.amps_set = (int)x.amps;
@@ -1700,10 +1786,18 @@ hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) 
     for (int k = 0; k <= M; ++k) ak[k] = c2(k == 0 ? x.are[0] - 1.0 : (double)ca.are[k], (double)ca.aim[k]);
     float2* leaves = nullptr;
     hipError_t r = pa::launch_assemble<float>(ca, X, st);
-    if (r == hipSuccess) r = clenshaw_batched<float>(X, d, N, M, (double)ca.s, ak, b1, b2, nullptr, &leaves, st, x.tile);
+    if (stepwise_power(M, x.tau * x.hnorm)) {
+        double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
+        double2 mc[pa::kMaxCheb];
+        power_coefficients(M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
+        for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
+        if (r == hipSuccess) r = power_batched<float>(pa::GEMM_F32, X, d, N, M, x.tau, mc, true, b1, b2, b3, &leaves, st, x.tile);
+    } else if (r == hipSuccess) {
+        r = clenshaw_batched<float>(X, d, N, M, (double)ca.s, ak, b1, b2, nullptr, &leaves, st, x.tile);
+    }
     // the leaves E_j in one buffer, the other free for the other half of the ping-pong
     float2* rd = leaves;
-    float2* wr = leaves == b1 ? b2 : b1;
+    float2* wr = leaves == b1 ? b2 : b1;   // in the power form the leaves are in X and b1 is free
     int n = N;
     while (r == hipSuccess && n > 1) {
         const int half = n / 2;
@@ -1953,7 +2047,7 @@ hipError_t ens_grad_instance(Ensemble* e, int i, float2* ws, const float2* tgt, 
     ca.ctl.mode = e->quad; ca.ctl.magnus = 0; ca.ctl.tau = (float)x.tau;
     for (int k = 0; k <= x.M; ++k) { ca.are[k] = (float)x.are[k]; ca.aim[k] = (float)x.aim[k]; }
     ca.are[0] = (float)(x.are[0] - 1.0);
-    if (L.fused && x.tau * x.hnorm <= 1.0) {
+    if (L.fused && power_form_ok(x.M, x.tau * x.hnorm)) {
         double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
         power_coefficients(x.M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
         ca.ps = 1;

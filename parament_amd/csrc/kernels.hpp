@@ -82,9 +82,9 @@ hipError_t launch_narrow(const double2* in, long n, float2* out, hipStream_t st)
 hipError_t launch_first_order(const ChebArgs<double>& a, const float2* R, float2* E, hipStream_t st);
 
 // Batched complex GEMM on the matrix cores (gemm.hip), any dim:
-//     C_b = alpha A_b B_b + beta Cin_b + diag I (+ A_b + B_b when add_ab),
+//     C_b = alpha A_b B_b + beta Cin_b + beta2 Cin2_b + diag I (+ A_b + B_b when add_ab),
 // strides in matrices of d x d (0 broadcasts one matrix).  Element types: complex128 where the
-// flag is set, complex64 otherwise (Cin has C's type).  GEMM_F64 needs a complex128 output and
+// flag is set, complex64 otherwise (Cin and Cin2 have C's type; C may be Cin or Cin2 itself).  GEMM_F64 needs a complex128 output and
 // takes complex128 or complex64 operands; the other modes run on complex64 throughout.
 enum : int { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X2 = 2, GEMM_BF16X3 = 3 };
 struct GemmArgs {
@@ -93,6 +93,8 @@ struct GemmArgs {
     const void* Cin; long sc;
     void* C; long so;
     double alpha_re, alpha_im, beta_re, beta_im, diag_re, diag_im;
+    const void* Cin2 = nullptr; long sc2 = 0;
+    double beta2_re = 0.0, beta2_im = 0.0;
     int d, batch, add_ab;
     int a_c128, b_c128, out_c128;
     int tile = 0;          // output tile 32 or 64, 16 = whole product per work-group (FP32, dims 33-128); 0 = automatic

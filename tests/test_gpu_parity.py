@@ -391,14 +391,26 @@ def test_stepwise_whole_product_dims(prec, d, n):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-@pytest.mark.parametrize("cycles", [3, 5, 7])
+@pytest.mark.parametrize("cycles", [3, 5, 7, 9, 11])
 def test_stepwise_manual_cycles(prec, cycles):
     """The batched recurrence takes its start values b_M, b_(M-1) through C_in in its first two
-    steps; with M = 3 the second of them is already the last."""
+    steps; with M = 3 the second of them is already the last.  Complex64 contexts take the power
+    form from M = 5 on (tau Hnorm <= 1 here): Horner's rule in S = Y^2 over two or three
+    alternating buffers, the last step in place over X."""
     h0, hks, c = case(1450 + cycles, 70, 2, 11)
     u, m = run_gpu(prec, h0, hks, c, 0.01, "none", cycles=cycles)
     assert m == cycles
     assert rel_err(u, oracle_for(prec, h0, hks, c, 0.01, "none", cycles=cycles)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32n"])
+def test_stepwise_power_form_tiny_step(prec):
+    """A manual M far above the table's at a tiny tau Hnorm: x^M would underflow the power form's
+    coefficients, so the step-wise path keeps the recurrence and the result stays finite."""
+    h0, hks, c = case(1460, 70, 2, 11)
+    u, m = run_gpu(prec, h0, hks, c, 1e-12, "none", cycles=29)
+    assert m == 29 and np.all(np.isfinite(u))
+    assert rel_err(u, oracle_for(prec, h0, hks, c, 1e-12, "none", cycles=29)) < TOL[prec]
 
 
 @pytest.mark.parametrize("prec", PRECS)
