@@ -369,18 +369,19 @@ hipError_t clenshaw_batched(const cplx<R>* X, int d, int cnt, int M, double s, c
 // (M + 1) / 2 GEMMs against the recurrence's M - 1, every Y term and diagonal in the FP64 GEMM
 // epilogue.  The polynomial is the one the recurrence evaluates; only the rounding differs.
 // With first_order false the last step leaves out m_1 Y + (m_0 - 1) I, which the caller adds in
-// FP64.  Buffers of cnt slices: S and w0, w1 as well when M >= 7; for M >= 5 the last step
-// overwrites X (which it reads in place).  *result: the buffer that holds E.
+// FP64.  Buffers of cnt slices: S and w0, w1 as well when M >= 7.  The last step writes out, or,
+// when out is null, w0 (M = 3) or X (M >= 5, read in place).  *result: the buffer that holds E.
 template <class R>
 hipError_t power_batched(int mode, cplx<R>* X, int d, int cnt, int M, double tau, const double2* m, bool first_order,
-                         cplx<R>* S, cplx<R>* w0, cplx<R>* w1, cplx<R>** result, hipStream_t st, int tile = 0) {
+                         cplx<R>* S, cplx<R>* w0, cplx<R>* w1, cplx<R>* out, cplx<R>** result, hipStream_t st,
+                         int tile = 0) {
     const bool w = sizeof(R) == 8;
     const int J = (M - 1) / 2;
     if (J < 1 || (J >= 3 && !w1)) return hipErrorInvalidValue;
     auto op = [&](const void* p) { return GemmOp{p, w, 1}; };
     const GemmOp none{nullptr, w, 0};
     hipError_t r = zgemm(mode, op(X), op(X), none, S, w, 1, c2(tau * tau), c2(0), c2(0), d, cnt, st, false, tile);
-    cplx<R>* cur = w0;
+    cplx<R>* cur = (J == 1 && out) ? out : w0;
     {
         const int j = J - 1;
         const bool lin = j > 0 || first_order;   // the step's own Y term and diagonal
@@ -390,7 +391,7 @@ hipError_t power_batched(int mode, cplx<R>* X, int d, int cnt, int M, double tau
     }
     for (int j = J - 2; j >= 0 && r == hipSuccess; --j) {
         const bool lin = j > 0 || first_order;
-        cplx<R>* o = j == 0 ? X : (cur == w0 ? w1 : w0);
+        cplx<R>* o = j == 0 ? (out ? out : X) : (cur == w0 ? w1 : w0);
         r = zgemm(mode, op(S), op(cur), lin ? op(X) : none, o, w, 1, c2(1), lin ? cscale(tau, m[2 * j + 1]) : c2(0),
                   lin ? m[2 * j] : c2(0), d, cnt, st, false, tile);
         cur = o;
@@ -645,8 +646,8 @@ Parament_ErrorCode run_mixed_stepwise(Context<R>* h, const MixedSrc& m, double2*
         double2 mc[pa::kMaxCheb];
         power_coefficients(M, m.tau * h->hnorm, are, aim, mr, mi);
         for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
-        if (power_batched<float>(mode, X, d, N, M, m.tau, mc, false, b1, b2, reinterpret_cast<float2*>(gen[3]), &rd, st) !=
-            hipSuccess)
+        if (power_batched<float>(mode, X, d, N, M, m.tau, mc, false, b1, b2, reinterpret_cast<float2*>(gen[3]), nullptr, &rd,
+                                 st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         pa::ChebArgs<double> fa = ca;
         fa.s = m.tau;
@@ -888,8 +889,8 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
         double2 mc[pa::kMaxCheb];
         power_coefficients(M, tau * h->hnorm, are, aim, mr, mi);
         for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
-        if (power_batched<R>(pa::GEMM_F32, X, (int)d, N, M, tau, mc, true, w1, w2, reinterpret_cast<C*>(gen[3]), &leaves,
-                             st) != hipSuccess)
+        if (power_batched<R>(pa::GEMM_F32, X, (int)d, N, M, tau, mc, true, w1, w2, reinterpret_cast<C*>(gen[3]), nullptr,
+                             &leaves, st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
     } else {
         // the slices as E = U - I (a_0 - 1 on the diagonal of the last step)
@@ -1125,18 +1126,23 @@ hipError_t grad_finish(const pa::ChebArgs<R>& ca, const cplx<R>* dg, const cplx<
 
 // Step-wise gradient (any dim, batched GEMMs).  Level l of the pairwise tree has n[l] nodes
 // (level 0: the N slices, kept as E = U - I in the leaf array); V holds levels 1 .. top and W the
-// costates of levels 0 .. top, all kept.  The per-slice arrays (X, two Clenshaw and two
-// derivative buffers) cover `chunk` slices at a time.
+// costates of levels 0 .. top, all kept.  The nwork per-slice arrays (X and the recurrence's
+// or the power form's buffers) cover `chunk` slices at a time.
 struct StepGradPlan {
-    int N = 0, top = 0, chunk = 1;
+    int N = 0, top = 0, chunk = 1, nwork = 5;
     std::vector<int> n;
     std::vector<size_t> voff, woff;   // matrix offsets of level l in V (l >= 1) and in W
     size_t vmats = 0, wmats = 0;
 };
 
-StepGradPlan step_grad_plan(int N, int d, size_t elem_bytes, unsigned chunk_req = 0) {
+// Per-slice arrays of the step-wise gradient: X and four buffers (S, dS, Q, dQ in the power form),
+// a fifth for the power form's Horner levels when M >= 7.
+inline int step_grad_work(int M, bool power) { return (power && M >= 7) ? 6 : 5; }
+
+StepGradPlan step_grad_plan(int N, int d, size_t elem_bytes, unsigned chunk_req = 0, int nwork = 5) {
     StepGradPlan p;
     p.N = N;
+    p.nwork = nwork;
     while ((1L << p.top) < N) ++p.top;
     p.voff.assign(p.top + 1, 0);
     p.woff.assign(p.top + 1, 0);
@@ -1145,16 +1151,16 @@ StepGradPlan step_grad_plan(int N, int d, size_t elem_bytes, unsigned chunk_req 
         if (l >= 1) { p.voff[l] = p.vmats; p.vmats += (size_t)p.n[l]; }
         p.woff[l] = p.wmats; p.wmats += (size_t)p.n[l];
     }
-    // the five per-slice arrays within 4 GiB
-    const size_t per_slice = 5 * (size_t)d * d * elem_bytes;
+    // the per-slice arrays within 4 GiB
+    const size_t per_slice = (size_t)nwork * d * d * elem_bytes;
     size_t chunk = std::max<size_t>(1, std::min<size_t>((size_t)N, ((size_t)4 << 30) / per_slice));
     if (chunk_req) chunk = std::min<size_t>(chunk, chunk_req);
     p.chunk = (int)chunk;
     return p;
 }
 
-// Buffers of one step-wise gradient: E (N slices), work (5 x chunk slices), V (vmats), W (wmats),
-// dg (ncoef x N values), gsum (one value).
+// Buffers of one step-wise gradient: E (N slices), work (nwork x chunk slices), V (vmats), W
+// (wmats), dg (ncoef x N values), gsum (one value).
 template <class R> struct StepGradBufs {
     cplx<R> *E = nullptr, *work = nullptr, *V = nullptr, *W = nullptr, *dg = nullptr, *gsum = nullptr;
 };
@@ -1165,8 +1171,11 @@ template <class R> struct StepGradBufs {
 // slice the derivative of the same recurrence in the direction of the slice's costate,
 //   db_k = s (W b_{k+1} + X db_{k+1}) - db_{k+2},   Q = s (W b_1 + X db_1) - 2 db_2,
 // so that Tr(W dU_j) = Tr(Q_j dX_j) (the polynomial's Frechet derivative is self-adjoint under
-// the trace pairing), and dg[t][j] = Tr(Q_j O_t).  ca carries the controls, M, s and the
-// coefficients with are[0] = a_0 - 1; ca.opsT the transposed control operators.
+// the trace pairing), and dg[t][j] = Tr(Q_j O_t).  When ca.ps is set and M >= 5 the slices and the
+// derivative take the power form instead (power_batched; the derivative of its Horner rule as in
+// the fused gradient kernel).  ca carries the controls, M, s and the coefficients with
+// are[0] = a_0 - 1 (pre/pim the power coefficients, pre[0] = m_0 - 1); ca.opsT the transposed
+// control operators.
 template <class R>
 hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, const StepGradBufs<R>& b,
                                  const cplx<R>* target, R* fid, R* grad, hipStream_t st) {
@@ -1175,18 +1184,28 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
     const size_t dd = (size_t)d * d;
     const C zero = pa::mk<R>(0, 0), one = pa::mk<R>(1, 0);
     const double s = (double)ca.s;
-    double2 a[pa::kMaxCheb];
-    for (int k = 0; k <= M; ++k) a[k] = c2((double)ca.are[k], (double)ca.aim[k]);
+    const bool w64 = sizeof(R) == 8;
+    const int mode = w64 ? pa::GEMM_F64 : pa::GEMM_F32;
+    const bool power = ca.ps && M >= 5;
+    if (power && p.nwork < step_grad_work(M, true)) return hipErrorInvalidValue;
+    const double tau = (double)ca.ctl.tau;
+    double2 a[pa::kMaxCheb], m[pa::kMaxCheb];
+    for (int k = 0; k <= M; ++k) {
+        a[k] = c2((double)ca.are[k], (double)ca.aim[k]);
+        m[k] = c2((double)ca.pre[k], (double)ca.pim[k]);
+    }
     C* X = b.work;
-    C* w[4];
-    for (int i = 0; i < 4; ++i) w[i] = b.work + (size_t)(1 + i) * p.chunk * dd;
+    C* w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int i = 0; i + 1 < p.nwork; ++i) w[i] = b.work + (size_t)(1 + i) * p.chunk * dd;
     hipError_t r = hipSuccess;
     // slices
     for (int j0 = 0; r == hipSuccess && j0 < N; j0 += p.chunk) {
         const int cnt = std::min(p.chunk, N - j0);
         ca.j0 = j0; ca.N = cnt;
         r = pa::launch_assemble<R>(ca, X, st);
-        if (r == hipSuccess) r = clenshaw_batched<R>(X, d, cnt, M, s, a, w[0], w[1], b.E + (size_t)j0 * dd, nullptr, st);
+        if (r != hipSuccess) break;
+        if (power) r = power_batched<R>(mode, X, d, cnt, M, tau, m, true, w[0], w[1], w[2], b.E + (size_t)j0 * dd, nullptr, st);
+        else r = clenshaw_batched<R>(X, d, cnt, M, s, a, w[0], w[1], b.E + (size_t)j0 * dd, nullptr, st);
     }
     // up-sweep
     const C* rd = b.E;
@@ -1225,6 +1244,47 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         auto gemm = [&](const C* A, const C* B, const C* Cin, C* out, double2 alpha, double2 beta, double2 diag) {
             if (r == hipSuccess) r = zgemm_native<R>(A, 1, B, 1, Cin, 1, out, 1, alpha, beta, diag, d, cnt, st);
         };
+        if (power) {
+            // Horner's rule in S = Y^2 (Y = tau X) differentiated in the direction V = tau W:
+            //   dS = Y V + V Y,  Q_J = m_{2J} I + m_{2J+1} Y,  dQ_J = m_{2J+1} V,
+            //   Q_j = S Q_{j+1} + m_{2j+1} Y + m_{2j} I,  dQ_j = dS Q_{j+1} + S dQ_{j+1} + m_{2j+1} V,
+            // and Q = dQ_0.  Level J-1 expands Q_J and dQ_J into the epilogue terms; below it each
+            // level forms dQ_j into a free buffer, then Q_j (j >= 1) over the dead dQ_{j+1}.
+            const GemmOp none{nullptr, w64, 0};
+            auto op = [&](const C* x) { return x ? GemmOp{x, w64, 1} : none; };
+            auto gemm2 = [&](const C* A, const C* B, const C* Cin, double2 beta, const C* Cin2, double2 beta2, C* out,
+                             double2 alpha, double2 diag) {
+                if (r == hipSuccess)
+                    r = zgemm(mode, op(A), op(B), op(Cin), out, w64, 1, alpha, beta, diag, d, cnt, st, false, 0, op(Cin2),
+                              beta2);
+            };
+            const int J = (M - 1) / 2;
+            C* S = w[0];
+            C* dS = w[1];
+            gemm(X, X, nullptr, S, c2(tau * tau), c2(0), c2(0));
+            gemm(X, Wj, nullptr, dS, c2(tau * tau), c2(0), c2(0));
+            gemm(Wj, X, dS, dS, c2(tau * tau), c2(1), c2(0));
+            const double2 aJ = cscale(tau, m[2 * J + 1]);
+            C* q = w[2];
+            C* dq = w[3];
+            C* spare = w[4];
+            gemm2(S, X, S, m[2 * J], X, cscale(tau, m[2 * J - 1]), q, aJ, m[2 * J - 2]);
+            gemm2(dS, X, dS, m[2 * J], Wj, cscale(tau, m[2 * J - 1]), dq, aJ, c2(0));
+            gemm(S, Wj, dq, dq, aJ, c2(1), c2(0));
+            for (int j = J - 2; j >= 0; --j) {
+                C* z = j == 0 ? X : spare;   // X is dead once no Q_j with j >= 1 is left to form
+                gemm(dS, q, Wj, z, c2(1), cscale(tau, m[2 * j + 1]), c2(0));
+                gemm(S, dq, z, z, c2(1), c2(1), c2(0));
+                if (j >= 1) {
+                    gemm(S, q, X, dq, c2(1), cscale(tau, m[2 * j + 1]), m[2 * j]);
+                    spare = q;
+                    q = dq;
+                }
+                dq = z;
+            }
+            if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
+            continue;
+        }
         gemm(X, X, X, w[0], cscale(s * s, a[M]), cscale(s, a[M - 1]), c2(a[M - 2].x - a[M].x, a[M - 2].y - a[M].y));
         gemm(X, Wj, Wj, w[2], cscale(s * s, a[M]), cscale(s, a[M - 1]), c2(0));
         gemm(Wj, X, w[2], w[2], cscale(s * s, a[M]), c2(1), c2(0));
@@ -1362,9 +1422,10 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
 
     // the fused path's tree layout (checkpointed or not); the step-wise path keeps the whole tree
     const GradPlan p = grad_plan(stepwise ? 1 : N, h->ckpt_seg, h->ckpt_chunk);
-    const StepGradPlan sp = step_grad_plan(stepwise ? N : 1, d, sizeof(C), h->ckpt_chunk);
+    const StepGradPlan sp = step_grad_plan(stepwise ? N : 1, d, sizeof(C), h->ckpt_chunk, step_grad_work(M, ca.ps != 0));
     const size_t mats_vw = p.chunk_mats + p.top_mats;
-    const size_t bytes_e = stepwise ? ((size_t)N + 5 * (size_t)sp.chunk) * dd * sizeof(C) : (size_t)p.chunk * dd * sizeof(C);
+    const size_t bytes_e =
+        stepwise ? ((size_t)N + (size_t)sp.nwork * sp.chunk) * dd * sizeof(C) : (size_t)p.chunk * dd * sizeof(C);
     const size_t bytes_v = (stepwise ? std::max<size_t>(sp.vmats, 1) : mats_vw) * dd * sizeof(C);
     const size_t bytes_w = (stepwise ? sp.wmats : mats_vw) * dd * sizeof(C);
     const size_t bytes_dg = (size_t)h->ncoef * N * sizeof(C);
@@ -1791,7 +1852,8 @@ hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) 
         double2 mc[pa::kMaxCheb];
         power_coefficients(M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
         for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
-        if (r == hipSuccess) r = power_batched<float>(pa::GEMM_F32, X, d, N, M, x.tau, mc, true, b1, b2, b3, &leaves, st, x.tile);
+        if (r == hipSuccess)
+            r = power_batched<float>(pa::GEMM_F32, X, d, N, M, x.tau, mc, true, b1, b2, b3, nullptr, &leaves, st, x.tile);
     } else if (r == hipSuccess) {
         r = clenshaw_batched<float>(X, d, N, M, (double)ca.s, ak, b1, b2, nullptr, &leaves, st, x.tile);
     }
@@ -1976,9 +2038,10 @@ EnsGradLayout ens_grad_layout(const EnsInst& x) {
         sz[0] = (size_t)L.p.chunk * dd; sz[1] = vw * dd; sz[2] = vw * dd;
     } else {
         // per-slice arrays within 1 GiB per instance
-        const unsigned cap = (unsigned)std::max<size_t>(1, ((size_t)1 << 30) / (5 * dd * sizeof(float2)));
-        L.sp = step_grad_plan(x.N, (int)x.d, sizeof(float2), cap);
-        sz[0] = ((size_t)x.N + 5 * (size_t)L.sp.chunk) * dd;
+        const int nwork = step_grad_work(x.M, power_form_ok(x.M, x.tau * x.hnorm));
+        const unsigned cap = (unsigned)std::max<size_t>(1, ((size_t)1 << 30) / (nwork * dd * sizeof(float2)));
+        L.sp = step_grad_plan(x.N, (int)x.d, sizeof(float2), cap, nwork);
+        sz[0] = ((size_t)x.N + (size_t)nwork * L.sp.chunk) * dd;
         sz[1] = std::max<size_t>(L.sp.vmats, 1) * dd;
         sz[2] = L.sp.wmats * dd;
     }
@@ -2047,7 +2110,7 @@ hipError_t ens_grad_instance(Ensemble* e, int i, float2* ws, const float2* tgt, 
     ca.ctl.mode = e->quad; ca.ctl.magnus = 0; ca.ctl.tau = (float)x.tau;
     for (int k = 0; k <= x.M; ++k) { ca.are[k] = (float)x.are[k]; ca.aim[k] = (float)x.aim[k]; }
     ca.are[0] = (float)(x.are[0] - 1.0);
-    if (L.fused && power_form_ok(x.M, x.tau * x.hnorm)) {
+    if (power_form_ok(x.M, x.tau * x.hnorm)) {
         double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
         power_coefficients(x.M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
         ca.ps = 1;

@@ -152,10 +152,12 @@ def test_complex64_magnus_gradient_matches_fp64():
     assert np.abs(g32 - g64).max() < 1e-4 * np.abs(g64).max()
 
 
-@pytest.mark.parametrize("cycles", [3, 5])
+@pytest.mark.parametrize("cycles", [3, 5, 7, 9])
 def test_stepwise_gradient_manual_cycles(cycles):
-    """A manual Chebyshev order (M = 3: the derivative's first two steps are all of it) against
-    central differences of the library's own FP64 propagator with the same order."""
+    """A manual Chebyshev order (M = 3: the derivative's first two steps are all of it; from M = 5
+    on the power form's Horner derivative, whose levels below J - 1 rotate through a spare buffer
+    from M = 7 on) against central differences of the library's own FP64 propagator with the
+    same order."""
     import parament_amd as parament
     h0, hk, c, t = _problem(4400 + cycles, 40, 2, 15)
     dt, h = 0.01, 1e-6
