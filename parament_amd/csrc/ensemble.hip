@@ -23,8 +23,26 @@ __global__ __launch_bounds__((64 * TileShape<D, T>::G)) void ens_slice_kernel(co
         Controls<float> ctl;
         ctl.c = I.c; ctl.pts = I.pts; ctl.amps_in = I.amps; ctl.amps_set = I.amps;
         ctl.mode = I.mode; ctl.magnus = 0; ctl.tau = I.tau;
-        cheb_slice<float, D, SH>(I.ops, I.ncoef, I.d, ctl, tk.y, I.M, I.s, I.are, I.aim, Xr, Xi, Br, Bi, coef,
-                                 I.leaves + (size_t)tk.y * I.d * I.d);
+        float2* out = I.leaves + (size_t)tk.y * I.d * I.d;
+        if (!I.ps) {
+            cheb_slice<float, D, SH>(I.ops, I.ncoef, I.d, ctl, tk.y, I.M, I.s, I.are, I.aim, Xr, Xi, Br, Bi, coef, out);
+            continue;
+        }
+        const int tid = threadIdx.x, wave = tid >> 6;
+        const int row0 = SH::row0(wave), col0 = SH::col0(wave), d = I.d;
+        for (int c = tid; c < I.ncoef; c += 64 * SH::G) coef[c] = slice_coefficient(ctl, c, tk.y);
+        __syncthreads();
+        const CTile<float, SH::T, SH::T> e =
+            power_slice_tile<float, D, SH>(I.ops, I.ncoef, d, I.M, I.tau, I.pre, I.pim, Xr, Xi, Br, Bi, coef);
+#pragma unroll
+        for (int m = 0; m < SH::T; ++m)
+#pragma unroll
+            for (int n = 0; n < SH::T; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = acc_row<float>(row0, m, v), col = acc_col(col0, n);
+                    if (row < d && col < d) out[row * d + col] = mk<float>(e.re[m][n][v], e.im[m][n][v]);
+                }
     }
 }
 

@@ -855,6 +855,14 @@ Parament_ErrorCode run(Context<R>* h, const cplx<R>* carr_dev, double dt, unsign
     if (fused) {
         // dims <= 16: wavefront-resident slices with the lowest tree levels folded in
         const bool small = pa::small_fold_supported((int)d, h->ncoef, sizeof(C));
+        // complex64 slices in the power form where the step-wise path takes it (complex128 keeps
+        // the reference's recurrence)
+        if (!small && sizeof(R) == 4 && stepwise_power(M, tau * h->hnorm)) {
+            double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
+            power_coefficients(M, tau * h->hnorm, are, aim, mr, mi);
+            ca.ps = 1;
+            for (int k = 0; k <= M; ++k) { ca.pre[k] = (R)mr[k]; ca.pim[k] = (R)mi[k]; }
+        }
         const int F = small ? pa::small_fold_levels(N) : 0;
         const int n_nodes = (N + (1 << F) - 1) >> F;
         const size_t nodes = (size_t)pa::tree_scratch_nodes<R>(n_nodes);
@@ -1761,6 +1769,11 @@ Parament_ErrorCode ens_set_hamiltonians(Ensemble* e, const cuComplex* H) {
         t.s = (float)(2.0 / x.hnorm); t.tau = (float)x.tau;
         for (int k = 0; k < pa::kMaxCheb; ++k) { t.are[k] = k <= x.M ? (float)x.are[k] : 0.f; t.aim[k] = k <= x.M ? (float)x.aim[k] : 0.f; }
         t.are[0] = (float)(x.are[0] - 1.0);
+        // the class kernels take the power form where the step-wise path does
+        t.ps = stepwise_power(x.M, x.tau * x.hnorm) ? 1 : 0;
+        double mr[pa::kMaxCheb] = {}, mi[pa::kMaxCheb] = {};
+        if (t.ps) power_coefficients(x.M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
+        for (int k = 0; k < pa::kMaxCheb; ++k) { t.pre[k] = (float)mr[k]; t.pim[k] = (float)mi[k]; }
         if (x.cls >= 0)
             for (int j = 0; j < x.N; ++j) slices[x.cls].push_back(make_int2((int)i, j));
     }
@@ -1912,6 +1925,12 @@ hipError_t ens_enqueue_loop(Ensemble* e) {
         ca.ctl.mode = e->quad; ca.ctl.magnus = 0; ca.ctl.tau = (float)x.tau;
         for (int k = 0; k <= x.M; ++k) { ca.are[k] = (float)x.are[k]; ca.aim[k] = (float)x.aim[k]; }
         ca.are[0] = (float)(x.are[0] - 1.0);
+        if (stepwise_power(x.M, x.tau * x.hnorm)) {   // as the class kernels
+            double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
+            power_coefficients(x.M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
+            ca.ps = 1;
+            for (int k = 0; k <= x.M; ++k) { ca.pre[k] = (float)mr[k]; ca.pim[k] = (float)mi[k]; }
+        }
         float2* leaves = e->d_work + x.work_off;
         ca.out = leaves;
         r = pa::launch_cheb_fused<float>(ca, e->stream);

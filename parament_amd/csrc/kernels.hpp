@@ -111,6 +111,8 @@ struct EnsInstance {
     int d, ncoef, amps, pts, mode, M;
     float s, tau;
     float are[kMaxCheb], aim[kMaxCheb];   // are[0] = a_0 - 1
+    int ps;                               // slices in the power form (power_slice_tile)
+    float pre[kMaxCheb], pim[kMaxCheb];   // its coefficients, pre[0] = m_0 - 1
 };
 struct EnsLevel {
     const float2* src;     // n_in nodes of one instance's tree level (E form)
