@@ -121,8 +121,11 @@ LIBSPEC double OneNorm_fp64(cuDoubleComplex *mat, unsigned int dim);
 /* ======================== Additions (no reference counterpart) ======================== */
 
 /* Arithmetic of a context's equiprop:
- *   PARAMENT_PRECISION_NATIVE: every step in the context's precision (the reference
- *       algorithm): FP64 for complex128, FP32 for complex64.  Default of complex128 contexts.
+ *   PARAMENT_PRECISION_NATIVE: every step in the context's precision: FP64 for complex128
+ *       (the reference algorithm), FP32 for complex64.  Default of complex128 contexts.  Where
+ *       tau Hnorm <= 1 and M >= 5, complex64 slices (dims above 16) evaluate the same degree-M
+ *       polynomial in powers of tau X (Horner's rule in (tau X)^2: (M + 1) / 2 products instead
+ *       of the recurrence's M - 1); complex128 keeps the reference's recurrence.
  *   PARAMENT_PRECISION_MIXED:  FP64 for Hnorm, the Bessel coefficients, the slice
  *       Hamiltonians, the first-order Chebyshev terms and the upper levels of the product
  *       tree; the higher-order Chebyshev remainder and the lowest three tree levels run as
@@ -255,7 +258,8 @@ LIBSPEC size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64 *handle);
  *   carr: amps[i] x pts[i] samples (carr layout of Parament_equiprop),
  *   out: dims[i] x dims[i] propagators.
  * Each instance computes what Parament_equiprop computes for it in PARAMENT_PRECISION_NATIVE
- * mode (FP32 throughout, FP32 iteration-count table, same quadrature and product order).  setExecution(0) (default) runs the whole ensemble as
+ * mode (FP32 throughout, FP32 iteration-count table, the power form where that mode takes it,
+ * same quadrature and product order).  setExecution(0) (default) runs the whole ensemble as
  * one captured HIP graph: the instances of dim <= 64 in multi-instance kernels per dim class
  * (16, 32, 64), the larger ones on batched GEMMs, on concurrent branches (up to four for the
  * larger ones); the tree block size and grid of each class and the GEMM tile per band of 16
