@@ -130,11 +130,12 @@ LIBSPEC double OneNorm_fp64(cuDoubleComplex *mat, unsigned int dim);
  *       Hamiltonians, the first-order Chebyshev terms and the upper levels of the product
  *       tree; the higher-order Chebyshev remainder and the lowest three tree levels run as
  *       split products with FP32 accumulation on the matrix cores (FP32-accurate products):
- *       scaled fp16 two-way splits (three fp16 products per product) in the fused kernel's
- *       power form (dim <= 64, tau Hnorm <= 1), bf16 three-way splits (six bf16 products)
- *       otherwise.  The environment variable PARAMENT_MIXED_SPLIT=bf16x3 (or f16x4, all
- *       four fp16 products) selects the other form of the fused kernel, for precision
- *       studies; it is read once per process.  Its accuracy (within 1e-5 of FP64 at the
+ *       scaled fp16 two-way splits (three fp16 products per product) in the power form
+ *       (tau Hnorm <= 1: the fused kernel at dim <= 64, the batched GEMMs above from M = 5 on),
+ *       bf16 three-way splits (six bf16 products) otherwise.  The environment variable
+ *       PARAMENT_MIXED_SPLIT=bf16x3 keeps the bf16 three-way split everywhere (f16x4: all four
+ *       fp16 products in the fused kernel), for precision studies; it is read once per
+ *       process.  Its accuracy (within 1e-5 of FP64 at the
  *       benchmark workloads) depends on the slices' actual norm ||tau X_j||, which the order M
  *       and the power form do not see: as in the reference, Hnorm sums the operators' norms
  *       without the control amplitudes.  Measured at the dim-64, 65,536-step workload with the

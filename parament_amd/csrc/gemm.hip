@@ -1,10 +1,14 @@
 // Batched complex GEMM on the gfx950 matrix cores for the step-wise (any-dim) path:
 //     C_b = alpha A_b B_b + beta Cin_b + beta2 Cin2_b + diag I  (+ A_b + B_b when add_ab)
 // over a batch of d x d row-major complex matrices, with the arithmetic of the products set by
-// the mode: FP64 (16x16x4 f64 instruction), FP32 (16x16x4 f32), or bf16 split products
+// the mode: FP64 (16x16x4 f64 instruction), FP32 (16x16x4 f32), bf16 split products
 // (16x16x32 bf16; every FP32 operand is split into 2 or 3 bf16 slices, s0 = bf16(x),
 // s1 = bf16(x - s0), s2 = bf16(x - s0 - s1), and the slice products with p + q < slices are
-// accumulated in FP32).  The epilogue is evaluated in FP64 and rounded once to the output type,
+// accumulated in FP32), or scaled fp16 two-way split products (16x16x32 f16: x S = h0 + h1 + r
+// with S = 2^14, h0 = fp16(x S), h1 = fp16(x S - h0), 22 bits in all; the three products
+// h0 h0', h0 h1', h1 h0' accumulated in FP32 and the sum scaled by 2^-28; operand entries must
+// stay below 4 in modulus, which holds for the power form's operands and the tree's deviations
+// from identity).  The epilogue is evaluated in FP64 and rounded once to the output type,
 // so a Clenshaw step's a_k I is added exactly.  Used for the Chebyshev steps
 // (parament.cpp:569-652) and the product-tree levels (parament.cpp:657-718) at dims where a
 // slice does not fit on chip.
@@ -16,12 +20,17 @@
 // slice then share that XCD's L2 for their A rows and B columns.
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace pa {
 
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8g __attribute__((ext_vector_type(8)));
 typedef float f32x4g __attribute__((ext_vector_type(4)));
+constexpr float kF16Scale = 16384.0f;                        // S of the fp16 split
+constexpr double kF16Unscale = 1.0 / (16384.0 * 16384.0);    // the products carry S^2
 
 constexpr int KT = 32;   // K chunk
 
@@ -210,16 +219,19 @@ __global__ __launch_bounds__(64 * NW) void zgemm_row_kernel(GemmArgs g, EpiIO<TA
         }
 }
 
-// bf16 split products: A chunk planes [slice][re/im][64][32 + 8] row-major, B chunk planes
+// Split products: A chunk planes [slice][re/im][64][32 + 8] row-major, B chunk planes
 // transposed [slice][re/im][64 cols][32 + 8].  Operand fragments of the 16x16x32 instruction:
 // lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7]; accumulator element v is
-// C[4(l/16) + v][l%16].
-template <int NS, int BT, class TA, class TB, class TC, class TO>
+// C[4(l/16) + v][l%16].  F16: the scaled fp16 two-way split (NS = 2), else bf16 slices.
+template <int NS, int BT, bool F16, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
+    typedef std::conditional_t<F16, _Float16, __bf16> H;
+    typedef std::conditional_t<F16, f16x8g, bf16x8> H8;
     constexpr int TW = BT / 32;   // 16x16 blocks per wave side (4 waves, 2 x 2)
     constexpr int S = KT + 8, PL = BT * S;
-    __shared__ __attribute__((aligned(16))) __bf16 Ap[2 * NS * PL];
-    __shared__ __attribute__((aligned(16))) __bf16 Bp[2 * NS * PL];
+    constexpr float pre = F16 ? kF16Scale : 1.0f;
+    __shared__ __attribute__((aligned(16))) H Ap[2 * NS * PL];
+    __shared__ __attribute__((aligned(16))) H Bp[2 * NS * PL];
     const int d = g.d, tiles = (d + BT - 1) / BT;
     const long total = (long)g.batch * tiles * tiles;
     const long w = work_item(total);
@@ -232,10 +244,12 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wr = (BT / 2) * (wave >> 1), wc = (BT / 2) * (wave & 1);
     const int r16 = lane & 15, kq = lane >> 4;
-    // acc[L][m][n]: L = 0 leading products, 1 the rest
-    f32x4g accr[2][TW][TW], acci[2][TW][TW];
+    // acc[L][m][n]: L = 0 leading products, 1 the rest (bf16: the rest carries up to 16 bits below
+    // the leading products; fp16: one accumulator, whose FP32 sum keeps the split's 22 bits)
+    constexpr int NL = F16 ? 1 : 2;
+    f32x4g accr[NL][TW][TW], acci[NL][TW][TW];
 #pragma unroll
-    for (int L = 0; L < 2; ++L)
+    for (int L = 0; L < NL; ++L)
 #pragma unroll
         for (int m = 0; m < TW; ++m)
 #pragma unroll
@@ -244,13 +258,15 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
                 for (int v = 0; v < 4; ++v) { accr[L][m][n][v] = 0.f; acci[L][m][n][v] = 0.f; }
     // two elements adjacent in k split together: each level's pair goes to LDS in one 4-byte store
     typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-    auto split_store2 = [&](__bf16* P, int o, float x0, float x1, int c) {
-        float r0 = x0, r1 = x1;
+    typedef _Float16 f16x2g __attribute__((ext_vector_type(2)));
+    typedef std::conditional_t<F16, f16x2g, bf16x2> H2;
+    auto split_store2 = [&](H* P, int o, float x0, float x1, int c) {
+        float r0 = x0 * pre, r1 = x1 * pre;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            bf16x2 h;
-            h[0] = (__bf16)r0; h[1] = (__bf16)r1;
-            *reinterpret_cast<bf16x2*>(P + (2 * s + c) * PL + o) = h;
+            H2 h;
+            h[0] = (H)r0; h[1] = (H)r1;
+            *reinterpret_cast<H2*>(P + (2 * s + c) * PL + o) = h;
             r0 -= (float)h[0]; r1 -= (float)h[1];
         }
     };
@@ -294,35 +310,39 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
         }
         __syncthreads();
         if (k0 + KT < d) fetch(k0 + KT);
-        bf16x8 br[TW][NS], bi[TW][NS];
+        H8 br[TW][NS], bi[TW][NS];
 #pragma unroll
         for (int n = 0; n < TW; ++n)
 #pragma unroll
             for (int q = 0; q < NS; ++q) {
                 const int o = (wc + 16 * n + r16) * S + 8 * kq;
-                br[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q) * PL + o);
-                bi[n][q] = *reinterpret_cast<const bf16x8*>(Bp + (2 * q + 1) * PL + o);
+                br[n][q] = *reinterpret_cast<const H8*>(Bp + (2 * q) * PL + o);
+                bi[n][q] = *reinterpret_cast<const H8*>(Bp + (2 * q + 1) * PL + o);
             }
+        auto mfma = [](H8 a, H8 b, f32x4g c) {
+            if constexpr (F16) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+            else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+        };
 #pragma unroll
         for (int m = 0; m < TW; ++m)
 #pragma unroll
             for (int p = 0; p < NS; ++p) {
                 const int o = (wr + 16 * m + r16) * S + 8 * kq;
-                const bf16x8 ar = *reinterpret_cast<const bf16x8*>(Ap + (2 * p) * PL + o);
-                const bf16x8 ai = *reinterpret_cast<const bf16x8*>(Ap + (2 * p + 1) * PL + o);
+                const H8 ar = *reinterpret_cast<const H8*>(Ap + (2 * p) * PL + o);
+                const H8 ai = *reinterpret_cast<const H8*>(Ap + (2 * p + 1) * PL + o);
                 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
                 u32x4 u = __builtin_bit_cast(u32x4, ai);
-                u ^= 0x80008000u;
-                const bf16x8 an = __builtin_bit_cast(bf16x8, u);
+                u ^= 0x80008000u;   // the sign bit of every 16-bit element
+                const H8 an = __builtin_bit_cast(H8, u);
 #pragma unroll
                 for (int n = 0; n < TW; ++n)
 #pragma unroll
                     for (int q = 0; q + p < NS; ++q) {
-                        const int L = (p + q) > 0 ? 1 : 0;
-                        accr[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br[n][q], accr[L][m][n], 0, 0, 0);
-                        acci[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi[n][q], acci[L][m][n], 0, 0, 0);
-                        accr[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(an, bi[n][q], accr[L][m][n], 0, 0, 0);
-                        acci[L][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br[n][q], acci[L][m][n], 0, 0, 0);
+                        const int L = (NL > 1 && p + q > 0) ? 1 : 0;
+                        accr[L][m][n] = mfma(ar, br[n][q], accr[L][m][n]);
+                        acci[L][m][n] = mfma(ar, bi[n][q], acci[L][m][n]);
+                        accr[L][m][n] = mfma(an, bi[n][q], accr[L][m][n]);
+                        acci[L][m][n] = mfma(ai, br[n][q], acci[L][m][n]);
                     }
             }
         __syncthreads();
@@ -335,9 +355,10 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             for (int v = 0; v < 4; ++v) {
                 const int gi = ti * BT + wr + 16 * m + 4 * kq + v, gj = tj * BT + wc + 16 * n + r16;
                 if (gi >= d || gj >= d) continue;
+                constexpr double post = F16 ? kF16Unscale : 1.0;
                 epi_store(g, io, A, B, b, dd, (long)gi * d + gj, gi == gj,
-                          (double)accr[0][m][n][v] + (double)accr[1][m][n][v],
-                          (double)acci[0][m][n][v] + (double)acci[1][m][n][v]);
+                          ((double)accr[0][m][n][v] + (NL > 1 ? (double)accr[NL - 1][m][n][v] : 0.0)) * post,
+                          ((double)acci[0][m][n][v] + (NL > 1 ? (double)acci[NL - 1][m][n][v] : 0.0)) * post);
             }
 }
 
@@ -351,8 +372,9 @@ hipError_t launch_tiled(const GemmArgs& g, int mode, const EpiIO<TA, TB, TC, TO>
     switch (mode) {
         case GEMM_F64: hipLaunchKernelGGL((zgemm_fp_kernel<double, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
         case GEMM_F32: hipLaunchKernelGGL((zgemm_fp_kernel<float, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_BF16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_BF16X3: hipLaunchKernelGGL((zgemm_split_kernel<3, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_BF16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, BT, false, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_BF16X3: hipLaunchKernelGGL((zgemm_split_kernel<3, BT, false, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+        case GEMM_F16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, BT, true, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -140,10 +140,10 @@ int power_split_levels_h2(int m, double x) {
     return bits;
 }
 
-// Split scheme of the fused mixed kernel's power form: 3 (default) or 4 = scaled fp16 two-way
-// split with three or four products per FP32-accurate product (mixed_h2.hip), 0 = bf16 three-way
-// split with six (mixed.hip).  PARAMENT_MIXED_SPLIT = f16x3 | f16x4 | bf16x3 selects one, for
-// comparisons.
+// Split scheme of the mixed power form: 3 (default) or 4 = scaled fp16 two-way split with three or
+// four products per FP32-accurate product (mixed_h2.hip; the batched GEMMs take three in both), 0 =
+// bf16 three-way split with six (mixed.hip, and the batched GEMMs' bf16 mode).
+// PARAMENT_MIXED_SPLIT = f16x3 | f16x4 | bf16x3 selects one, for comparisons.
 int mixed_split_scheme() {
     static const int scheme = [] {
         const char* v = getenv("PARAMENT_MIXED_SPLIT");
@@ -613,14 +613,18 @@ template <class R> bool gen_buffers(Context<R>* h, Workspace& ws, size_t bytes, 
 }
 
 // Step-wise form (any dim): the Clenshaw GEMMs (or the power form's) batched over all slices on
-// complex64 data with the products of the context's mode, every GEMM epilogue in FP64.
+// complex64 data with the products of the context's mode, every GEMM epilogue in FP64.  In the
+// default mode the power form runs on scaled fp16 two-way split products (its operands Y = tau X,
+// S = Y^2, the Horner brackets and the tree's deviations stay below 4 in modulus while
+// ||tau X_j||_2 <= 1, as in the fused kernel), the recurrence on bf16 three-way splits.
 template <class R>
 Parament_ErrorCode run_mixed_stepwise(Context<R>* h, const MixedSrc& m, double2* out_dev, Workspace& ws) {
     const int d = (int)h->dim, N = m.N, M = m.M;
     const size_t dd = (size_t)d * d;
-    const int mode = gemm_mode_of(h->precision);
-    hipStream_t st = h->stream;
     const bool power = stepwise_power(M, m.tau * h->hnorm);
+    const bool f16 = power && h->precision == PARAMENT_PRECISION_MIXED && mixed_split_scheme() != 0;
+    const int mode = f16 ? pa::GEMM_F16X2 : gemm_mode_of(h->precision);
+    hipStream_t st = h->stream;
     void* gen[4] = {nullptr, nullptr, nullptr, nullptr};
     if (!gen_buffers(h, ws, (size_t)N * dd * sizeof(float2), gen, power && M >= 7 ? 4 : 3)) return ws.failure();
     double are[pa::kMaxCheb], aim[pa::kMaxCheb];
@@ -638,7 +642,11 @@ Parament_ErrorCode run_mixed_stepwise(Context<R>* h, const MixedSrc& m, double2*
     float2* rd = b1;   // the slices E_j
     float2* wr = X;    // the other half of the tree's ping-pong
     mark(h, 0);
-    if (pa::launch_assemble_c64(ca, X, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
+    // the fp16 split takes Y = tau X itself (entries below 1 while ||tau X||_2 <= 1), so the buffer
+    // holds Y and the power form runs with tau = 1
+    pa::ChebArgs<double> xa = ca;
+    if (f16) xa.asm_scale = m.tau;
+    if (pa::launch_assemble_c64(xa, X, st) != hipSuccess) return PARAMENT_STATUS_CUBLAS_FAILED;
     if (power) {
         // R = E - m_1 Y - (m_0 - 1) I by the power form (into X), then E = R + m_1 Y + (m_0 - 1) I
         // in FP64, in place
@@ -646,8 +654,8 @@ Parament_ErrorCode run_mixed_stepwise(Context<R>* h, const MixedSrc& m, double2*
         double2 mc[pa::kMaxCheb];
         power_coefficients(M, m.tau * h->hnorm, are, aim, mr, mi);
         for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
-        if (power_batched<float>(mode, X, d, N, M, m.tau, mc, false, b1, b2, reinterpret_cast<float2*>(gen[3]), nullptr, &rd,
-                                 st) != hipSuccess)
+        if (power_batched<float>(mode, X, d, N, M, f16 ? 1.0 : m.tau, mc, false, b1, b2, reinterpret_cast<float2*>(gen[3]),
+                                 nullptr, &rd, st) != hipSuccess)
             return PARAMENT_STATUS_CUBLAS_FAILED;
         pa::ChebArgs<double> fa = ca;
         fa.s = m.tau;

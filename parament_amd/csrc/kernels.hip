@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(ChebArgs<R> a, TO* __rest
     for (int s = 0; s < kAsmSlices; ++s)
         if (s < ns) {
             TO o;
-            o.x = x[s].x; o.y = x[s].y;
+            o.x = x[s].x * a.asm_scale; o.y = x[s].y * a.asm_scale;
             X[(size_t)(j0 + s) * dd + e] = o;
         }
 }

@@ -17,6 +17,7 @@ template <class R> struct ChebArgs {
     int N;                 // slices computed by this launch: j0 .. j0 + N - 1
     int M;                 // Chebyshev order (odd, >= 3)
     R s;                   // 2 / Hnorm
+    R asm_scale = R(1);    // step-wise assembly only: X_j is written multiplied by this
     R are[kMaxCheb], aim[kMaxCheb];   // a_k = (-i)^k J_k(tau Hnorm); are[0] holds a_0 - 1
     cplx<R>* out;          // N x d x d, E_j = U_j - I (out[0] is slice j0)
     int j0 = 0;            // first slice
@@ -86,7 +87,8 @@ hipError_t launch_first_order(const ChebArgs<double>& a, const float2* R, float2
 // strides in matrices of d x d (0 broadcasts one matrix).  Element types: complex128 where the
 // flag is set, complex64 otherwise (Cin and Cin2 have C's type; C may be Cin or Cin2 itself).  GEMM_F64 needs a complex128 output and
 // takes complex128 or complex64 operands; the other modes run on complex64 throughout.
-enum : int { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X2 = 2, GEMM_BF16X3 = 3 };
+// GEMM_F16X2: scaled fp16 two-way split products (22 bits), for operands below 4 in modulus.
+enum : int { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X2 = 2, GEMM_BF16X3 = 3, GEMM_F16X2 = 4 };
 struct GemmArgs {
     const void* A; long sa;
     const void* B; long sb;

@@ -345,9 +345,10 @@ def test_full_size_heisenberg6_complex64_vs_oracle():
 
 def test_full_size_ising8_split_products():
     """The precision-variant workload at full size (dim-256 transverse-field Ising chain, 8 local
-    sigma_x controls, 16,384 steps, no quadrature) on the batched GEMMs: bf16 three-way split
-    products and FP32 products, with FP64 first-order terms and upper tree levels, stay within
-    1e-6 of the FP64 path; the FP64 propagator is unitary to 1e-10."""
+    sigma_x controls, 16,384 steps, no quadrature) on the batched GEMMs: the default mixed mode
+    (the power form on scaled fp16 two-way split products here) and FP32 products, with FP64
+    first-order terms and upper tree levels, stay within 1e-6 of the FP64 path; the FP64
+    propagator is unitary to 1e-10."""
     import parament_amd as parament
     from parament_amd import workloads
     h0, hk, c, dt, quad = workloads.ising8()
@@ -449,6 +450,40 @@ def test_mixed_stepwise_magnus(variant, tol):
     u, m = run_gpu("fp64", h0, hks, c, 0.01, "simpson", True, mixed=variant)
     ref = eo.equiprop(h0, hks, c, 0.01, mode="simpson", use_magnus=True, cycles=m)
     assert FLOOR[variant] < rel_err(u, ref) < tol
+
+
+def test_mixed_split_scheme_variable():
+    """PARAMENT_MIXED_SPLIT=bf16x3 (read once per process, so set in a child process) keeps the bf16
+    three-way split in the fused kernel (dim 40) and in the step-wise power form (dim 72), where the
+    default runs scaled fp16 two-way splits: both forms within 1e-5 of the oracle, and apart from
+    each other by more than FP64 rounding (the variable took effect)."""
+    import json
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import json, sys\n"
+        "from tests.test_gpu_parity import case, run_gpu\n"
+        "out = {}\n"
+        "for d in (40, 72):\n"
+        "    h0, hks, c = case(1800 + d, d, 2, 33)\n"
+        "    u, m = run_gpu('fp64', h0, hks, c, 0.01, 'none', mixed=True)\n"
+        "    out[str(d)] = [u.real.tolist(), u.imag.tolist(), m]\n"
+        "print(json.dumps(out))\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, PARAMENT_MIXED_SPLIT="bf16x3")
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    other = json.loads(r.stdout.strip().splitlines()[-1])
+    for d in (40, 72):
+        h0, hks, c = case(1800 + d, d, 2, 33)
+        u, m = run_gpu("fp64", h0, hks, c, 0.01, "none", mixed=True)
+        re, im, m2 = other[str(d)]
+        u2 = np.asarray(re) + 1j * np.asarray(im)
+        ref = eo.equiprop(h0, hks, c, 0.01, mode="none", cycles=m)
+        assert m2 == m
+        assert rel_err(u, ref) < 1e-5 and rel_err(u2, ref) < 1e-5, (d, rel_err(u, ref), rel_err(u2, ref))
+        assert rel_err(u, u2) > 1e-10, d
 
 
 @pytest.mark.parametrize("variant", ["mixed", "mixed_fp32", "mixed_bf16x2"])
