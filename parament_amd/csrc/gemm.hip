@@ -93,7 +93,8 @@ __device__ __forceinline__ void epi_store(const GemmArgs& g, const EpiIO<TA, TB,
 
 // FP64 / FP32 matrix-core tiles: A chunk [64][32] (stride SA), B chunk [32][64] (stride SB).
 // The next K chunk is fetched into registers while the matrix cores work on the current one.
-template <class R, int BT, class TA, class TB, class TC, class TO>
+// EDGE: d is not a multiple of BT, so loads and stores past d are masked (otherwise none is).
+template <class R, int BT, bool EDGE, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
     constexpr int TW = BT / 32;   // 16x16 blocks per wave side (4 waves, 2 x 2)
     constexpr int SA = KT + Pad<R>::A, SB = BT + Pad<R>::B;
@@ -118,14 +119,14 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
             const int e = tid + 256 * u;
             const int gi = ti * BT + e / KT, gk = k0 + e % KT;
             ra[u].x = 0; ra[u].y = 0;
-            if (gi < d && gk < d) ra[u] = A[(long)gi * d + gk];
+            if (!EDGE || (gi < d && gk < d)) ra[u] = A[(long)gi * d + gk];
         }
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int e = tid + 256 * u;
             const int gk = k0 + e / BT, gj = tj * BT + e % BT;
             rb[u].x = 0; rb[u].y = 0;
-            if (gk < d && gj < d) rb[u] = B[(long)gk * d + gj];
+            if (!EDGE || (gk < d && gj < d)) rb[u] = B[(long)gk * d + gj];
         }
     };
     CTile<R, TW, TW> acc;
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
         __syncthreads();
         if (k0 + KT < d) fetch(k0 + KT);
         // a last chunk runs only the k-steps that reach d (the rest of it is zero)
-        wave_cgemm<R, TW, TW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc, min(KT, (d - k0 + 3) & ~3));
+        wave_cgemm<R, TW, TW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc, EDGE ? min(KT, (d - k0 + 3) & ~3) : KT);
         __syncthreads();
     }
     // epilogue in FP64
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = ti * BT + acc_row<R>(wr, m, v), gj = tj * BT + acc_col(wc, n);
-                if (gi >= d || gj >= d) continue;
+                if (EDGE && (gi >= d || gj >= d)) continue;
                 epi_store(g, io, A, B, b, dd, (long)gi * d + gj, gi == gj, acc.re[m][n][v], acc.im[m][n][v]);
             }
 }
@@ -222,8 +223,9 @@ __global__ __launch_bounds__(64 * NW) void zgemm_row_kernel(GemmArgs g, EpiIO<TA
 // Split products: A chunk planes [slice][re/im][64][32 + 8] row-major, B chunk planes
 // transposed [slice][re/im][64 cols][32 + 8].  Operand fragments of the 16x16x32 instruction:
 // lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7]; accumulator element v is
-// C[4(l/16) + v][l%16].  F16: the scaled fp16 two-way split (NS = 2), else bf16 slices.
-template <int NS, int BT, bool F16, class TA, class TB, class TC, class TO>
+// C[4(l/16) + v][l%16].  F16: the scaled fp16 two-way split (NS = 2), else bf16 slices.  EDGE:
+// d is not a multiple of BT, so loads and stores past d are masked (otherwise none is).
+template <int NS, int BT, bool F16, bool EDGE, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
     typedef std::conditional_t<F16, _Float16, __bf16> H;
     typedef std::conditional_t<F16, f16x8g, bf16x8> H8;
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 ra[u][h].x = 0; ra[u][h].y = 0;
-                if (gi < d && gk + h < d) ra[u][h] = A[(long)gi * d + gk + h];
+                if (!EDGE || (gi < d && gk + h < d)) ra[u][h] = A[(long)gi * d + gk + h];
             }
         }
 #pragma unroll
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 rb[u][h].x = 0; rb[u][h].y = 0;
-                if (gk + h < d && gj < d) rb[u][h] = B[(long)(gk + h) * d + gj];
+                if (!EDGE || (gk + h < d && gj < d)) rb[u][h] = B[(long)(gk + h) * d + gj];
             }
         }
     };
@@ -347,6 +349,48 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             }
         __syncthreads();
     }
+    if constexpr (F16 && sizeof(TC) == 8 && sizeof(TO) == 8) {
+        // the fp16 split serves the power form (complex64 in and out): its epilogue in FP32, the
+        // 2^-28 of the products folded into alpha (each term rounded once more than in FP64,
+        // below the output's own rounding)
+        const float a_re = (float)(g.alpha_re * kF16Unscale), a_im = (float)(g.alpha_im * kF16Unscale);
+        const float b_re = (float)g.beta_re, b_im = (float)g.beta_im;
+        const float c_re = (float)g.beta2_re, c_im = (float)g.beta2_im;
+        const float g_re = (float)g.diag_re, g_im = (float)g.diag_im;
+        const long ib = b * g.sc * dd, ib2 = b * g.sc2 * dd, ob = b * g.so * dd;
+#pragma unroll
+        for (int m = 0; m < TW; ++m)
+#pragma unroll
+            for (int n = 0; n < TW; ++n)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int gi = ti * BT + wr + 16 * m + 4 * kq + v, gj = tj * BT + wc + 16 * n + r16;
+                    if (EDGE && (gi >= d || gj >= d)) continue;
+                    const long o = (long)gi * d + gj;
+                    const float xr = accr[0][m][n][v], xi = acci[0][m][n][v];
+                    float yr = a_re * xr - a_im * xi, yi = a_re * xi + a_im * xr;
+                    if (io.Cin) {
+                        const TC cv = io.Cin[ib + o];
+                        yr = __builtin_fmaf(b_re, cv.x, __builtin_fmaf(-b_im, cv.y, yr));
+                        yi = __builtin_fmaf(b_re, cv.y, __builtin_fmaf(b_im, cv.x, yi));
+                    }
+                    if (io.Cin2) {
+                        const TC cv = io.Cin2[ib2 + o];
+                        yr = __builtin_fmaf(c_re, cv.x, __builtin_fmaf(-c_im, cv.y, yr));
+                        yi = __builtin_fmaf(c_re, cv.y, __builtin_fmaf(c_im, cv.x, yi));
+                    }
+                    if (g.add_ab) {
+                        const TA av = A[o];
+                        const TB bv = B[o];
+                        yr += av.x + bv.x; yi += av.y + bv.y;
+                    }
+                    if (gi == gj) { yr += g_re; yi += g_im; }
+                    TO out;
+                    out.x = yr; out.y = yi;
+                    io.C[ob + o] = out;
+                }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < TW; ++m)
 #pragma unroll
@@ -354,7 +398,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = ti * BT + wr + 16 * m + 4 * kq + v, gj = tj * BT + wc + 16 * n + r16;
-                if (gi >= d || gj >= d) continue;
+                if (EDGE && (gi >= d || gj >= d)) continue;
                 constexpr double post = F16 ? kF16Unscale : 1.0;
                 epi_store(g, io, A, B, b, dd, (long)gi * d + gj, gi == gj,
                           ((double)accr[0][m][n][v] + (NL > 1 ? (double)accr[NL - 1][m][n][v] : 0.0)) * post,
@@ -370,11 +414,19 @@ hipError_t launch_tiled(const GemmArgs& g, int mode, const EpiIO<TA, TB, TC, TO>
     if (total > 0x7fffffffL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)total), block(256);
     switch (mode) {
-        case GEMM_F64: hipLaunchKernelGGL((zgemm_fp_kernel<double, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_F32: hipLaunchKernelGGL((zgemm_fp_kernel<float, BT, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_BF16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, BT, false, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_BF16X3: hipLaunchKernelGGL((zgemm_split_kernel<3, BT, false, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
-        case GEMM_F16X2: hipLaunchKernelGGL((zgemm_split_kernel<2, BT, true, TA, TB, TC, TO>), grid, block, 0, st, g, io); break;
+#define PA_FP(R)                                                                                                   \
+    if (g.d % BT) hipLaunchKernelGGL((zgemm_fp_kernel<R, BT, true, TA, TB, TC, TO>), grid, block, 0, st, g, io);   \
+    else hipLaunchKernelGGL((zgemm_fp_kernel<R, BT, false, TA, TB, TC, TO>), grid, block, 0, st, g, io);
+        case GEMM_F64: PA_FP(double) break;
+        case GEMM_F32: PA_FP(float) break;
+#undef PA_FP
+#define PA_SPLIT(NS, F16)                                                                                              \
+    if (g.d % BT) hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, true, TA, TB, TC, TO>), grid, block, 0, st, g, io); \
+    else hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, false, TA, TB, TC, TO>), grid, block, 0, st, g, io);
+        case GEMM_BF16X2: PA_SPLIT(2, false) break;
+        case GEMM_BF16X3: PA_SPLIT(3, false) break;
+        case GEMM_F16X2: PA_SPLIT(2, true) break;
+#undef PA_SPLIT
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
