@@ -273,15 +273,22 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
         }
     };
     // the next K chunk is fetched into registers while the matrix cores work on the current one;
-    // thread element pairs: A (r, k), (r, k+1) and B (k, c), (k+1, c), k even
+    // thread element pairs: A (r, k), (r, k+1) and B (k, c), (k+1, c), k even.  Which thread takes
+    // which pair: with 64-wide tiles, the 16-lane groups g of a wave take A rows 4 apart and B pair
+    // columns rotated by g, so that each 4-byte LDS store of a wave lands in 64 different banks (row
+    // stride S / 2 = 20 words); the loads stay coalesced in runs of 16 lanes.
     constexpr int NP = BT * KT / 512;
+    const int g16 = lane >> 4;
+    auto a_row = [&](int u) { return BT == 64 ? 16 * u + 4 * g16 + wave : (tid + 256 * u) / (KT / 2); };
+    auto a_kp = [&](int u) { return BT == 64 ? (lane & 15) : (tid + 256 * u) % (KT / 2); };
+    auto b_col = [&](int u) { return BT == 64 ? lane : (tid + 256 * u) % BT; };
+    auto b_kp = [&](int u) { return BT == 64 ? 4 * u + ((wave + g16) & 3) : (tid + 256 * u) / BT; };
     TA ra[NP][2];
     TB rb[NP][2];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
-            const int q = tid + 256 * u;
-            const int gi = ti * BT + q / (KT / 2), gk = k0 + 2 * (q % (KT / 2));
+            const int gi = ti * BT + a_row(u), gk = k0 + 2 * a_kp(u);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 ra[u][h].x = 0; ra[u][h].y = 0;
@@ -290,8 +297,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
         }
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
-            const int q = tid + 256 * u;
-            const int gk = k0 + 2 * (q / BT), gj = tj * BT + q % BT;
+            const int gk = k0 + 2 * b_kp(u), gj = tj * BT + b_col(u);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 rb[u][h].x = 0; rb[u][h].y = 0;
@@ -303,8 +309,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
     for (int k0 = 0; k0 < d; k0 += KT) {
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
-            const int q = tid + 256 * u;
-            const int oa = (q / (KT / 2)) * S + 2 * (q % (KT / 2)), ob = (q % BT) * S + 2 * (q / BT);
+            const int oa = a_row(u) * S + 2 * a_kp(u), ob = b_col(u) * S + 2 * b_kp(u);
             split_store2(Ap, oa, (float)ra[u][0].x, (float)ra[u][1].x, 0);
             split_store2(Ap, oa, (float)ra[u][0].y, (float)ra[u][1].y, 1);
             split_store2(Bp, ob, (float)rb[u][0].x, (float)rb[u][1].x, 0);
