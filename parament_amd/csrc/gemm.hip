@@ -285,14 +285,33 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
     auto b_kp = [&](int u) { return BT == 64 ? 4 * u + ((wave + g16) & 3) : (tid + 256 * u) / BT; };
     TA ra[NP][2];
     TB rb[NP][2];
+    // without edges, each thread's element pointers are formed once and advanced by one K chunk
+    // per fetch (KT along A's rows, KT rows of B)
+    const TA* pa[NP];
+    const TB* pb[NP];
+#pragma unroll
+    for (int u = 0; u < NP; ++u) {
+        pa[u] = A + (long)(ti * BT + a_row(u)) * d + 2 * a_kp(u);
+        pb[u] = B + (long)(2 * b_kp(u)) * d + tj * BT + b_col(u);
+    }
     auto fetch = [&](int k0) {
+        if constexpr (!EDGE) {
+#pragma unroll
+            for (int u = 0; u < NP; ++u) {
+                ra[u][0] = pa[u][0]; ra[u][1] = pa[u][1];
+                rb[u][0] = pb[u][0]; rb[u][1] = pb[u][d];
+                pa[u] += KT;
+                pb[u] += (long)KT * d;
+            }
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < NP; ++u) {
             const int gi = ti * BT + a_row(u), gk = k0 + 2 * a_kp(u);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 ra[u][h].x = 0; ra[u][h].y = 0;
-                if (!EDGE || (gi < d && gk + h < d)) ra[u][h] = A[(long)gi * d + gk + h];
+                if (gi < d && gk + h < d) ra[u][h] = A[(long)gi * d + gk + h];
             }
         }
 #pragma unroll
@@ -301,7 +320,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 rb[u][h].x = 0; rb[u][h].y = 0;
-                if (!EDGE || (gk + h < d && gj < d)) rb[u][h] = B[(long)(gk + h) * d + gj];
+                if (gk + h < d && gj < d) rb[u][h] = B[(long)(gk + h) * d + gj];
             }
         }
     };
