@@ -46,6 +46,21 @@ __device__ __forceinline__ void stage(const cplx<R>* __restrict__ src, int d, R*
     }
 }
 
+// The same for the conjugate transpose of a d x d row-major matrix (no diagonal term).
+template <class R, int D>
+__device__ __forceinline__ void stage_ct(const cplx<R>* __restrict__ src, int d, R* Pr, R* Pi, int S) {
+    for (int e = threadIdx.x; e < D * D; e += 64 * GL<R, D>::G) {
+        const int i = e / D, k = e % D;
+        R xr = R(0), xi = R(0);
+        if (i < d && k < d) {
+            const cplx<R> v = src[k * d + i];
+            xr = v.x;
+            xi = -v.y;
+        }
+        Pr[i * S + k] = xr; Pi[i * S + k] = xi;
+    }
+}
+
 template <class R, int D>
 __device__ __forceinline__ void tile_store(const CTile<R, GL<R, D>::T, GL<R, D>::T>& w, int d, cplx<R>* dst) {
     constexpr int T = GL<R, D>::T;
@@ -170,13 +185,54 @@ __global__ __launch_bounds__(64 * Shape<D>::G) void down_level_kernel(const cplx
     tile_store<R, D>(w, d, dst);
 }
 
+// The root's children and g in one launch: child c (block c < 2) gets W(c) = t^H V(c + 1) or
+// V(c - 1) t^H (the root's costate is t^H itself, staged from t), and block 2 forms
+// g = Tr(t^H (I + e)) from the root's deviation e (FP64 partial sums, fixed reduction order) into
+// gout[0].  Replaces the root launch and the first down-sweep level.
+template <class R, int D>
+__global__ __launch_bounds__(64 * Shape<D>::G) void down_root_kernel(const cplx<R>* __restrict__ V,
+                                                                        const cplx<R>* __restrict__ t,
+                                                                        const cplx<R>* __restrict__ u, int d,
+                                                                        cplx<R>* __restrict__ Wc,
+                                                                        cplx<R>* __restrict__ gout) {
+    constexpr int G = GL<R, D>::G, T = GL<R, D>::T, SA = GL<R, D>::SA, SB = GL<R, D>::SB;
+    __shared__ R Ar[D * SA], Ai[D * SA], Br[D * SB], Bi[D * SB];
+    __shared__ double sr[64 * G], si[64 * G];
+    const int c = blockIdx.x, dd = d * d, wave = threadIdx.x >> 6;
+    if (c == 2) {
+        double pr = 0.0, pi = 0.0;
+        for (int e = threadIdx.x; e < dd; e += 64 * G) {
+            const cplx<R> a = t[e];
+            const double bx = (double)u[e].x + (e / d == e % d ? 1.0 : 0.0), by = (double)u[e].y;
+            pr += (double)a.x * bx + (double)a.y * by;   // conj(a) b
+            pi += (double)a.x * by - (double)a.y * bx;
+        }
+        sr[threadIdx.x] = pr; si[threadIdx.x] = pi;
+        __syncthreads();
+        for (int o = 32 * G; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) { sr[threadIdx.x] += sr[threadIdx.x + o]; si[threadIdx.x] += si[threadIdx.x + o]; }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) gout[0] = mk<R>((R)sr[0], (R)si[0]);
+        return;
+    }
+    const cplx<R>* S = V + (size_t)(c ^ 1) * dd;   // E = V - I of the sibling
+    if (c == 0) { stage_ct<R, D>(t, d, Ar, Ai, SA); stage<R, D>(S, d, Br, Bi, SB, R(1)); }
+    else { stage<R, D>(S, d, Ar, Ai, SA, R(1)); stage_ct<R, D>(t, d, Br, Bi, SB); }
+    __syncthreads();
+    CTile<R, T, T> w;
+    wave_cgemm<R, T, T, D, SA, SB>(Ar, Ai, Br, Bi, Shape<D>::row0(wave), Shape<D>::col0(wave), w);
+    tile_store<R, D>(w, d, Wc + (size_t)c * dd);
+}
+
 // Per slice j = j0 + jl (jl < a.N; j0 even): X_j, the costate W_j from the level-1 costate of
 // its parent and the E = U - I of its sibling (E holds this launch's leaves, W1 their
 // parents), then Q_j = Dp(X_j)[W_j] and dg[t * dg_stride + j] = Tr(Q_j O_t).
 template <class R, int D, bool PS>
 __global__ __launch_bounds__(64 * Shape<D>::G, 2) void grad_slice_kernel(ChebArgs<R> a, const cplx<R>* __restrict__ E,
                                                                          const cplx<R>* __restrict__ W1,
-                                                                         cplx<R>* __restrict__ dg, int dg_stride) {
+                                                                         cplx<R>* __restrict__ dg, int dg_stride,
+                                                                         GradDirect<R> gd) {
     constexpr int G = GL<R, D>::G, T = GL<R, D>::T, SA = GL<R, D>::SA, SB = GL<R, D>::SB;
     __shared__ R Xr[D * SA], Xi[D * SA], Wr[D * SA], Wi[D * SA], Br[D * SB], Bi[D * SB], Dr[D * SB], Di[D * SB];
     __shared__ cplx<R> coef[kGradCoeffs];
@@ -402,16 +458,40 @@ __global__ __launch_bounds__(64 * Shape<D>::G, 2) void grad_slice_kernel(ChebArg
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
                     if (t0 + q >= a.ncoef) break;
-                    if (G == 1) dg[(size_t)(t0 + q) * dg_stride + j] = mk<R>(pr[q], pi[q]);
-                    else red[(t0 + q) * G + wave] = mk<R>(pr[q], pi[q]);
+                    if (G > 1) {
+                        red[(t0 + q) * G + wave] = mk<R>(pr[q], pi[q]);
+                    } else if (gd.grad) {   // no quadrature: dF/dc_tj = 2 Re(conj(g) dg_tj) / d^2
+                        if (t0 + q < gd.amps) {
+                            const cplx<R> g = gd.g[0];
+                            gd.grad[(size_t)(t0 + q) * gd.pts + j] = R(2) * (g.x * pr[q] + g.y * pi[q]) * (R(1) / ((R)d * (R)d));
+                        }
+                    } else {
+                        dg[(size_t)(t0 + q) * dg_stride + j] = mk<R>(pr[q], pi[q]);
+                    }
                 }
+        }
+        if (G == 1 && gd.grad && gd.fid && j == 0 && tid == 0) {
+            const cplx<R> g = gd.g[0];
+            gd.fid[0] = (g.x * g.x + g.y * g.y) * (R(1) / ((R)d * (R)d));
         }
         if (G > 1) {
             __syncthreads();
             for (int t = tid; t < a.ncoef; t += 64 * G) {
                 cplx<R> sum = red[t * G];
                 for (int w = 1; w < G; ++w) sum = cadd(sum, red[t * G + w]);
-                dg[(size_t)t * dg_stride + j] = sum;
+                if (gd.grad) {
+                    // no quadrature: sample j is slice j, dF/dc_tj = 2 Re(conj(g) dg_tj) / d^2
+                    if (t < gd.amps) {
+                        const cplx<R> g = gd.g[0];
+                        gd.grad[(size_t)t * gd.pts + j] = R(2) * (g.x * sum.x + g.y * sum.y) * (R(1) / ((R)d * (R)d));
+                    }
+                } else {
+                    dg[(size_t)t * dg_stride + j] = sum;
+                }
+            }
+            if (gd.grad && gd.fid && j == 0 && tid == 0) {
+                const cplx<R> g = gd.g[0];
+                gd.fid[0] = (g.x * g.x + g.y * g.y) * (R(1) / ((R)d * (R)d));
             }
         }
         __syncthreads();
@@ -608,18 +688,30 @@ hipError_t launch_down_level(const cplx<R>* V, int n_child, int d, const cplx<R>
     return hipGetLastError();
 }
 
+// Level top - 1 (two children) of the down-sweep together with g (down_root_kernel).
+template <class R>
+hipError_t launch_down_root(const cplx<R>* V, const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* Wc, cplx<R>* g,
+                            hipStream_t st) {
+    if (d <= 0 || d > 32) return hipErrorInvalidValue;
+    if (d <= 16) hipLaunchKernelGGL((down_root_kernel<R, 16>), dim3(3), dim3(64 * Shape<16>::G), 0, st, V, t, u, d, Wc, g);
+    else hipLaunchKernelGGL((down_root_kernel<R, 32>), dim3(3), dim3(64 * Shape<32>::G), 0, st, V, t, u, d, Wc, g);
+    return hipGetLastError();
+}
+
 template <class R>
 hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx<R>* W1, cplx<R>* dg, int dg_stride,
-                              hipStream_t st) {
+                              hipStream_t st, GradDirect<R> gd) {
     if (a.d <= 0 || a.d > 32 || a.N <= 0 || (a.j0 & 1) || a.M < 3 || a.M >= kMaxCheb || a.ncoef > kGradCoeffs)
         return hipErrorInvalidValue;
+    // the direct gradient needs one sample per slice and the traces of plain controls
+    if (gd.grad && (a.ctl.mode != QUAD_NONE || a.ctl.magnus || !gd.g || gd.amps > a.ncoef)) return hipErrorInvalidValue;
     const dim3 grid(a.N);
     if (a.d <= 16) {
-        if (a.ps) hipLaunchKernelGGL((grad_slice_kernel<R, 16, true>), grid, dim3(64 * Shape<16>::G), 0, st, a, E, W1, dg, dg_stride);
-        else hipLaunchKernelGGL((grad_slice_kernel<R, 16, false>), grid, dim3(64 * Shape<16>::G), 0, st, a, E, W1, dg, dg_stride);
+        if (a.ps) hipLaunchKernelGGL((grad_slice_kernel<R, 16, true>), grid, dim3(64 * Shape<16>::G), 0, st, a, E, W1, dg, dg_stride, gd);
+        else hipLaunchKernelGGL((grad_slice_kernel<R, 16, false>), grid, dim3(64 * Shape<16>::G), 0, st, a, E, W1, dg, dg_stride, gd);
     } else {
-        if (a.ps) hipLaunchKernelGGL((grad_slice_kernel<R, 32, true>), grid, dim3(64 * Shape<32>::G), 0, st, a, E, W1, dg, dg_stride);
-        else hipLaunchKernelGGL((grad_slice_kernel<R, 32, false>), grid, dim3(64 * Shape<32>::G), 0, st, a, E, W1, dg, dg_stride);
+        if (a.ps) hipLaunchKernelGGL((grad_slice_kernel<R, 32, true>), grid, dim3(64 * Shape<32>::G), 0, st, a, E, W1, dg, dg_stride, gd);
+        else hipLaunchKernelGGL((grad_slice_kernel<R, 32, false>), grid, dim3(64 * Shape<32>::G), 0, st, a, E, W1, dg, dg_stride, gd);
     }
     return hipGetLastError();
 }
@@ -660,8 +752,10 @@ hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pt
 #define PA_GRAD_INSTANTIATE(R)                                                                                       \
     template hipError_t launch_up_block<R>(const cplx<R>*, int, int, int, const TreeStore<R>&, hipStream_t);         \
     template hipError_t launch_down_level<R>(const cplx<R>*, int, int, const cplx<R>*, cplx<R>*, hipStream_t);      \
+    template hipError_t launch_down_root<R>(const cplx<R>*, const cplx<R>*, const cplx<R>*, int, cplx<R>*, cplx<R>*, \
+                                            hipStream_t);                                                            \
     template hipError_t launch_grad_slices<R>(const ChebArgs<R>&, const cplx<R>*, const cplx<R>*, cplx<R>*, int,     \
-                                              hipStream_t);                                                          \
+                                              hipStream_t, GradDirect<R>);                                           \
     template hipError_t launch_grad_root<R>(const cplx<R>*, const cplx<R>*, int, cplx<R>*, cplx<R>*, hipStream_t);   \
     template hipError_t launch_transpose<R>(const cplx<R>*, int, int, cplx<R>*, hipStream_t);                        \
     template hipError_t launch_grad_trace<R>(const cplx<R>*, int, int, const cplx<R>*, int, cplx<R>*, int, int,     \

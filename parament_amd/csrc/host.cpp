@@ -1370,13 +1370,27 @@ hipError_t grad_fused_enqueue(pa::ChebArgs<R> ca, const GradPlan& p, const Fused
         return up_levels(b.E, 0, upto, cnt,
                          [&](int l) { return (l == p.seg) ? vt(p.seg) + (size_t)(j0 >> p.seg) * dd : vc(l); });
     };
+    // without quadrature or Magnus terms a sample is a slice: the slice kernel writes the gradient
+    // itself and no chain-rule launch follows
+    pa::GradDirect<R> direct;
+    if (ca.ctl.mode == pa::QUAD_NONE && !ca.ctl.magnus && ca.ctl.amps_in <= ca.ncoef) {
+        direct.g = b.gsum; direct.grad = grad; direct.fid = fid; direct.pts = ca.ctl.pts; direct.amps = ca.ctl.amps_in;
+    }
     if (h) mark(h, 0);
     hipError_t r = hipSuccess;
     for (int c = 0; r == hipSuccess && c < p.nchunks; ++c) r = forward_chunk(c, p.seg);
     if (r == hipSuccess) r = up_levels(vt(p.seg), p.seg, p.top, p.n[p.seg], [&](int l) { return vt(l); });
-    if (r == hipSuccess) r = pa::launch_grad_root<R>(target, vt(p.top), d, wt(p.top), b.gsum, st);
+    // the root's costate t^H is needed only by its two children: when they are a top level, one
+    // launch forms them and g; otherwise (a single slice, or the segment level at the top) the
+    // root launch writes W(root)
+    const bool merged = p.top >= 1 && p.top - 1 >= p.seg && p.n[p.top - 1] == 2;
+    if (r == hipSuccess) {
+        r = merged ? pa::launch_down_root<R>(vt(p.top - 1), target, vt(p.top), d, wt(p.top - 1), b.gsum, st)
+                   : pa::launch_grad_root<R>(target, vt(p.top), d, wt(p.top), b.gsum, st);
+    }
     if (h) mark(h, 1);
-    for (int l = p.top - 1; r == hipSuccess && l >= p.seg; --l) r = pa::launch_down_level<R>(vt(l), p.n[l], d, wt(l + 1), wt(l), st);
+    for (int l = p.top - (merged ? 2 : 1); r == hipSuccess && l >= p.seg; --l)
+        r = pa::launch_down_level<R>(vt(l), p.n[l], d, wt(l + 1), wt(l), st);
     for (int c = 0; r == hipSuccess && c < p.nchunks; ++c) {
         const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
         if (p.nchunks > 1) r = forward_chunk(c, p.seg - 1);
@@ -1386,9 +1400,9 @@ hipError_t grad_fused_enqueue(pa::ChebArgs<R> ca, const GradPlan& p, const Fused
         }
         const C* W1 = (p.seg == 1) ? wt(1) + (size_t)(j0 >> 1) * dd : wc(1);
         ca.j0 = j0; ca.N = cnt;
-        if (r == hipSuccess) r = pa::launch_grad_slices<R>(ca, b.E, W1, b.dg, N, st);
+        if (r == hipSuccess) r = pa::launch_grad_slices<R>(ca, b.E, W1, b.dg, N, st, direct);
     }
-    if (r == hipSuccess) r = grad_finish<R>(ca, b.dg, b.gsum, N, grad, fid, st);
+    if (r == hipSuccess && !direct.grad) r = grad_finish<R>(ca, b.dg, b.gsum, N, grad, fid, st);
     return r;
 }
 

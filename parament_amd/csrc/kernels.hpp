@@ -142,8 +142,23 @@ template <class R> hipError_t launch_up_block(const cplx<R>* in, int n_in, int d
                                               hipStream_t s);
 template <class R> hipError_t launch_down_level(const cplx<R>* V, int n_child, int d, const cplx<R>* Wp, cplx<R>* Wc,
                                                 hipStream_t st);
+// The root's two children (V: their deviations; W(root) = t^H, staged from the target t) and
+// g = Tr(t^H (I + u)) from the root's deviation u, into g[0]: the root launch and the first
+// down-sweep level in one.
+template <class R> hipError_t launch_down_root(const cplx<R>* V, const cplx<R>* t, const cplx<R>* u, int d,
+                                               cplx<R>* Wc, cplx<R>* g, hipStream_t st);
+// With gd.grad set (no quadrature, no Magnus terms): the slice kernel maps its traces to the gradient
+// itself, grad[t * pts + j] = 2 Re(conj(g) Tr(Q_j O_t)) / d^2 for t < amps with g = gd.g[0], and
+// writes F = |g|^2 / d^2 to gd.fid (if set); dg is then unused.
+template <class R> struct GradDirect {
+    const cplx<R>* g = nullptr;
+    R* grad = nullptr;
+    R* fid = nullptr;
+    int pts = 0, amps = 0;
+};
 template <class R> hipError_t launch_grad_slices(const ChebArgs<R>& a, const cplx<R>* E, const cplx<R>* W1,
-                                                 cplx<R>* dg, int dg_stride, hipStream_t st);
+                                                 cplx<R>* dg, int dg_stride, hipStream_t st,
+                                                 GradDirect<R> gd = GradDirect<R>());
 // out[t] = in[t]^T for t < count d x d matrices
 template <class R> hipError_t launch_transpose(const cplx<R>* in, int count, int d, cplx<R>* out, hipStream_t st);
 // dg[t * dg_stride + j0 + j] = Tr(Q_j O_t), O_t^T = opsT[t], for the n matrices Q_j (any d)
