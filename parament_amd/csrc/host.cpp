@@ -1380,21 +1380,26 @@ hipError_t grad_fused_enqueue(pa::ChebArgs<R> ca, const GradPlan& p, const Fused
     hipError_t r = hipSuccess;
     for (int c = 0; r == hipSuccess && c < p.nchunks; ++c) r = forward_chunk(c, p.seg);
     if (r == hipSuccess) r = up_levels(vt(p.seg), p.seg, p.top, p.n[p.seg], [&](int l) { return vt(l); });
-    // the root's costate t^H is needed only by its two children: when they are a top level, one
-    // launch forms them and g; otherwise (a single slice, or the segment level at the top) the
-    // root launch writes W(root)
-    const bool merged = p.top >= 1 && p.top - 1 >= p.seg && p.n[p.top - 1] == 2;
+    // the root's costate t^H is needed only by its two children: one launch forms them and g, from
+    // the top arrays (checkpointed layout) or from the chunk arrays (whole tree, one chunk);
+    // otherwise (one or two slices) the root launch writes W(root)
+    const bool merged_top = p.top >= 1 && p.top - 1 >= p.seg && p.n[p.top - 1] == 2;
+    const bool merged_chunk = !merged_top && p.top >= 2 && p.seg == p.top && p.nchunks == 1 && p.n[p.top - 1] == 2;
     if (r == hipSuccess) {
-        r = merged ? pa::launch_down_root<R>(vt(p.top - 1), target, vt(p.top), d, wt(p.top - 1), b.gsum, st)
-                   : pa::launch_grad_root<R>(target, vt(p.top), d, wt(p.top), b.gsum, st);
+        if (merged_top) r = pa::launch_down_root<R>(vt(p.top - 1), target, vt(p.top), d, wt(p.top - 1), b.gsum, st);
+        else if (!merged_chunk) r = pa::launch_grad_root<R>(target, vt(p.top), d, wt(p.top), b.gsum, st);
     }
     if (h) mark(h, 1);
-    for (int l = p.top - (merged ? 2 : 1); r == hipSuccess && l >= p.seg; --l)
+    for (int l = p.top - (merged_top ? 2 : 1); r == hipSuccess && l >= p.seg; --l)
         r = pa::launch_down_level<R>(vt(l), p.n[l], d, wt(l + 1), wt(l), st);
     for (int c = 0; r == hipSuccess && c < p.nchunks; ++c) {
         const int j0 = c * p.chunk, cnt = std::min(p.chunk, N - j0);
         if (p.nchunks > 1) r = forward_chunk(c, p.seg - 1);
         for (int l = p.seg - 1; r == hipSuccess && l >= 1; --l) {
+            if (merged_chunk && l == p.top - 1) {
+                r = pa::launch_down_root<R>(vc(l), target, vt(p.top), d, wc(l), b.gsum, st);
+                continue;
+            }
             const C* parent = (l + 1 == p.seg) ? wt(p.seg) + (size_t)(j0 >> p.seg) * dd : wc(l + 1);
             r = pa::launch_down_level<R>(vc(l), count(cnt, l), d, parent, wc(l), st);
         }
