@@ -371,17 +371,20 @@ hipError_t clenshaw_batched(const cplx<R>* X, int d, int cnt, int M, double s, c
 // With first_order false the last step leaves out m_1 Y + (m_0 - 1) I, which the caller adds in
 // FP64.  Buffers of cnt slices: S and w0, w1 as well when M >= 7.  The last step writes out, or,
 // when out is null, w0 (M = 3) or X (M >= 5, read in place).  *result: the buffer that holds E.
+// keep (optional, J buffers of cnt slices): S goes to keep[0] and each B_j (j >= 1) to keep[j]
+// instead, so that a caller can read them back (S, w0 and w1 are then unused).
 template <class R>
 hipError_t power_batched(int mode, cplx<R>* X, int d, int cnt, int M, double tau, const double2* m, bool first_order,
                          cplx<R>* S, cplx<R>* w0, cplx<R>* w1, cplx<R>* out, cplx<R>** result, hipStream_t st,
-                         int tile = 0) {
+                         int tile = 0, cplx<R>* const* keep = nullptr) {
     const bool w = sizeof(R) == 8;
     const int J = (M - 1) / 2;
-    if (J < 1 || (J >= 3 && !w1)) return hipErrorInvalidValue;
+    if (J < 1 || (J >= 3 && !w1 && !keep)) return hipErrorInvalidValue;
+    if (keep) S = keep[0];
     auto op = [&](const void* p) { return GemmOp{p, w, 1}; };
     const GemmOp none{nullptr, w, 0};
     hipError_t r = zgemm(mode, op(X), op(X), none, S, w, 1, c2(tau * tau), c2(0), c2(0), d, cnt, st, false, tile);
-    cplx<R>* cur = (J == 1 && out) ? out : w0;
+    cplx<R>* cur = (J == 1 && out) ? out : (keep && J > 1 ? keep[J - 1] : w0);
     {
         const int j = J - 1;
         const bool lin = j > 0 || first_order;   // the step's own Y term and diagonal
@@ -391,7 +394,7 @@ hipError_t power_batched(int mode, cplx<R>* X, int d, int cnt, int M, double tau
     }
     for (int j = J - 2; j >= 0 && r == hipSuccess; --j) {
         const bool lin = j > 0 || first_order;
-        cplx<R>* o = j == 0 ? (out ? out : X) : (cur == w0 ? w1 : w0);
+        cplx<R>* o = j == 0 ? (out ? out : X) : (keep ? keep[j] : (cur == w0 ? w1 : w0));
         r = zgemm(mode, op(S), op(cur), lin ? op(X) : none, o, w, 1, c2(1), lin ? cscale(tau, m[2 * j + 1]) : c2(0),
                   lin ? m[2 * j] : c2(0), d, cnt, st, false, tile);
         cur = o;
@@ -1146,6 +1149,7 @@ hipError_t grad_finish(const pa::ChebArgs<R>& ca, const cplx<R>* dg, const cplx<
 // or the power form's buffers) cover `chunk` slices at a time.
 struct StepGradPlan {
     int N = 0, top = 0, chunk = 1, nwork = 5;
+    int keep = 0;   // matrices per slice kept from the slices' pass for the derivative (S, Q_1 .. Q_{J-1})
     std::vector<int> n;
     std::vector<size_t> voff, woff;   // matrix offsets of level l in V (l >= 1) and in W
     size_t vmats = 0, wmats = 0;
@@ -1154,6 +1158,14 @@ struct StepGradPlan {
 // Per-slice arrays of the step-wise gradient: X and four buffers (S, dS, Q, dQ in the power form),
 // a fifth for the power form's Horner levels when M >= 7.
 inline int step_grad_work(int M, bool power) { return (power && M >= 7) ? 6 : 5; }
+// The power form's S and Horner brackets Q_1 .. Q_{J-1} of every slice, kept from the slices' pass
+// so that the derivative need not form them again (J of its products per slice fewer): J matrices
+// per slice, when they take at most max_bytes.
+inline int step_grad_keep(int M, bool power, int N, int d, size_t elem_bytes, size_t max_bytes) {
+    const int J = (M - 1) / 2;
+    if (!power || M < 5) return 0;
+    return (size_t)J * N * d * d * elem_bytes <= max_bytes ? J : 0;
+}
 
 StepGradPlan step_grad_plan(int N, int d, size_t elem_bytes, unsigned chunk_req = 0, int nwork = 5) {
     StepGradPlan p;
@@ -1175,10 +1187,10 @@ StepGradPlan step_grad_plan(int N, int d, size_t elem_bytes, unsigned chunk_req 
     return p;
 }
 
-// Buffers of one step-wise gradient: E (N slices), work (nwork x chunk slices), V (vmats), W
-// (wmats), dg (ncoef x N values), gsum (one value).
+// Buffers of one step-wise gradient: E (N slices), work (nwork x chunk slices), keep (plan.keep x N
+// slices, when kept), V (vmats), W (wmats), dg (ncoef x N values), gsum (one value).
 template <class R> struct StepGradBufs {
-    cplx<R> *E = nullptr, *work = nullptr, *V = nullptr, *W = nullptr, *dg = nullptr, *gsum = nullptr;
+    cplx<R> *E = nullptr, *work = nullptr, *keep = nullptr, *V = nullptr, *W = nullptr, *dg = nullptr, *gsum = nullptr;
 };
 
 // The adjoint gradient on batched GEMMs: slices E_j by the Clenshaw recurrence (as equiprop's
@@ -1213,6 +1225,11 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
     C* X = b.work;
     C* w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     for (int i = 0; i + 1 < p.nwork; ++i) w[i] = b.work + (size_t)(1 + i) * p.chunk * dd;
+    // the kept S and Q_j of slices j0 .. : keep[0] = S, keep[j] = Q_j
+    const bool kept = power && p.keep == (M - 1) / 2 && b.keep;
+    auto keep_at = [&](int j0, C** kq) {
+        for (int k = 0; k < p.keep; ++k) kq[k] = b.keep + ((size_t)k * N + j0) * dd;
+    };
     hipError_t r = hipSuccess;
     // slices
     for (int j0 = 0; r == hipSuccess && j0 < N; j0 += p.chunk) {
@@ -1220,7 +1237,11 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         ca.j0 = j0; ca.N = cnt;
         r = pa::launch_assemble<R>(ca, X, st);
         if (r != hipSuccess) break;
-        if (power) r = power_batched<R>(mode, X, d, cnt, M, tau, m, true, w[0], w[1], w[2], b.E + (size_t)j0 * dd, nullptr, st);
+        C* kq[pa::kMaxCheb / 2 + 1];
+        if (kept) keep_at(j0, kq);
+        if (power)
+            r = power_batched<R>(mode, X, d, cnt, M, tau, m, true, w[0], w[1], w[2], b.E + (size_t)j0 * dd, nullptr, st, 0,
+                                 kept ? kq : nullptr);
         else r = clenshaw_batched<R>(X, d, cnt, M, s, a, w[0], w[1], b.E + (size_t)j0 * dd, nullptr, st);
     }
     // up-sweep
@@ -1277,6 +1298,26 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
             const int J = (M - 1) / 2;
             C* S = w[0];
             C* dS = w[1];
+            if (kept) {
+                // S and every Q_j of these slices were kept by the slices' pass: per level two products
+                C* kq[pa::kMaxCheb / 2 + 1];
+                keep_at(j0, kq);
+                S = kq[0];
+                gemm(X, Wj, nullptr, dS, c2(tau * tau), c2(0), c2(0));
+                gemm(Wj, X, dS, dS, c2(tau * tau), c2(1), c2(0));
+                const double2 aJ = cscale(tau, m[2 * J + 1]);
+                C* dq = w[3];
+                gemm2(dS, X, dS, m[2 * J], Wj, cscale(tau, m[2 * J - 1]), dq, aJ, c2(0));
+                gemm(S, Wj, dq, dq, aJ, c2(1), c2(0));
+                for (int j = J - 2; j >= 0; --j) {
+                    C* z = dq == w[3] ? w[2] : w[3];
+                    gemm(dS, kq[j + 1], Wj, z, c2(1), cscale(tau, m[2 * j + 1]), c2(0));
+                    gemm(S, dq, z, z, c2(1), c2(1), c2(0));
+                    dq = z;
+                }
+                if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
+                continue;
+            }
             gemm(X, X, nullptr, S, c2(tau * tau), c2(0), c2(0));
             gemm(X, Wj, nullptr, dS, c2(tau * tau), c2(0), c2(0));
             gemm(Wj, X, dS, dS, c2(tau * tau), c2(1), c2(0));
@@ -1457,10 +1498,12 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
 
     // the fused path's tree layout (checkpointed or not); the step-wise path keeps the whole tree
     const GradPlan p = grad_plan(stepwise ? 1 : N, h->ckpt_seg, h->ckpt_chunk);
-    const StepGradPlan sp = step_grad_plan(stepwise ? N : 1, d, sizeof(C), h->ckpt_chunk, step_grad_work(M, ca.ps != 0));
+    StepGradPlan sp = step_grad_plan(stepwise ? N : 1, d, sizeof(C), h->ckpt_chunk, step_grad_work(M, ca.ps != 0));
+    // the slices' S and Q_j kept for the derivative within 8 GiB (checkpointing asks for less memory: not kept)
+    if (stepwise && !h->ckpt_seg) sp.keep = step_grad_keep(M, ca.ps != 0, N, d, sizeof(C), (size_t)8 << 30);
     const size_t mats_vw = p.chunk_mats + p.top_mats;
     const size_t bytes_e =
-        stepwise ? ((size_t)N + (size_t)sp.nwork * sp.chunk) * dd * sizeof(C) : (size_t)p.chunk * dd * sizeof(C);
+        stepwise ? ((size_t)N * (1 + sp.keep) + (size_t)sp.nwork * sp.chunk) * dd * sizeof(C) : (size_t)p.chunk * dd * sizeof(C);
     const size_t bytes_v = (stepwise ? std::max<size_t>(sp.vmats, 1) : mats_vw) * dd * sizeof(C);
     const size_t bytes_w = (stepwise ? sp.wmats : mats_vw) * dd * sizeof(C);
     const size_t bytes_dg = (size_t)h->ncoef * N * sizeof(C);
@@ -1480,6 +1523,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
         if (stepwise) {
             StepGradBufs<R> sb;
             sb.E = E; sb.work = E + (size_t)N * dd; sb.V = V; sb.W = W; sb.dg = dg; sb.gsum = gsum;
+            sb.keep = sp.keep ? sb.work + (size_t)sp.nwork * sp.chunk * dd : nullptr;
             mark(h, 0);
             r = grad_stepwise_enqueue<R>(ca, sp, sb, target_dev, fid_dev, grad_dev, st);
             mark(h, 1);
@@ -2087,7 +2131,9 @@ EnsGradLayout ens_grad_layout(const EnsInst& x) {
         const int nwork = step_grad_work(x.M, power_form_ok(x.M, x.tau * x.hnorm));
         const unsigned cap = (unsigned)std::max<size_t>(1, ((size_t)1 << 30) / (nwork * dd * sizeof(float2)));
         L.sp = step_grad_plan(x.N, (int)x.d, sizeof(float2), cap, nwork);
-        sz[0] = ((size_t)x.N + (size_t)nwork * L.sp.chunk) * dd;
+        // the slices' S and Q_j kept for the derivative within 4 GiB per instance
+        L.sp.keep = step_grad_keep(x.M, power_form_ok(x.M, x.tau * x.hnorm), x.N, (int)x.d, sizeof(float2), (size_t)4 << 30);
+        sz[0] = ((size_t)x.N * (1 + L.sp.keep) + (size_t)nwork * L.sp.chunk) * dd;
         sz[1] = std::max<size_t>(L.sp.vmats, 1) * dd;
         sz[2] = L.sp.wmats * dd;
     }
@@ -2172,6 +2218,7 @@ hipError_t ens_grad_instance(Ensemble* e, int i, float2* ws, const float2* tgt, 
     StepGradBufs<float> sb;
     sb.E = ws + L.off[0]; sb.work = sb.E + (size_t)x.N * x.d * x.d; sb.V = ws + L.off[1]; sb.W = ws + L.off[2];
     sb.dg = ws + L.off[3]; sb.gsum = ws + L.off[4];
+    sb.keep = L.sp.keep ? sb.work + (size_t)L.sp.nwork * L.sp.chunk * x.d * x.d : nullptr;
     return grad_stepwise_enqueue<float>(ca, L.sp, sb, t, f, g, st);
 }
 
