@@ -242,8 +242,10 @@ LIBSPEC Parament_ErrorCode Parament_gateFidelityGradientDevice_fp64(Parament_Con
  * the working set is O(N / segment + chunk) matrices for the cost of one more forward pass.
  * Results are identical with and without checkpointing.  Dims above 32 always keep the whole
  * tree and process their per-slice arrays in chunks of chunk_slices slices (0: as many as fit
- * in 4 GiB); segment_slices does not apply to them.  getGradientWorkspace returns the
- * device bytes the last gradient call worked in. */
+ * in 4 GiB); with segment_slices = 0 they also keep the power form's S and Horner brackets of
+ * every slice for the derivative ((M - 1) / 2 more matrices per slice, when that takes at most
+ * 8 GiB), and any other segment_slices forms them again instead.  getGradientWorkspace returns
+ * the device bytes the last gradient call worked in. */
 LIBSPEC Parament_ErrorCode Parament_setGradientCheckpointing(Parament_Context_f32 *handle, unsigned int segment_slices,
     unsigned int chunk_slices);
 LIBSPEC Parament_ErrorCode Parament_setGradientCheckpointing_fp64(Parament_Context_f64 *handle,
