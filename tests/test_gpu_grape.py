@@ -199,7 +199,9 @@ def test_fused_gradient_manual_cycles(d, cycles, dt):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_stepwise_gradient_chunks_agree(prec):
-    """Per-slice arrays processed 7 slices at a time give the whole-batch result."""
+    """Per-slice arrays processed 7 slices at a time give the whole-batch result; the default call
+    also keeps the slices' S and Horner brackets for the derivative, the checkpointed one forms
+    them again."""
     h0, hk, c, t = _problem(4300, 40, 2, 61)
     f0, g0 = gpu_grad(prec, h0, hk, c, 0.05, "none", t)
     f1, g1 = gpu_grad(prec, h0, hk, c, 0.05, "none", t, segment=1, chunk=7)
