@@ -219,8 +219,11 @@ LIBSPEC Parament_ErrorCode Parament_getKernelTimes_fp64(Parament_Context_f64 *ha
  * values dF/dc[k*pts + j] (quadrature chain rule included), fidelity (optional) receives F.
  * For callers optimising controls (GRAPE loops, the PyTorch autograd wrapper).  Any dim (dims
  * above 32 run on batched GEMMs and keep the whole product tree), every quadrature mode
- * including Magnus.  The Device variants take device pointers and are asynchronous on the
- * context's stream; they read only the real parts of carr_dev. */
+ * including Magnus.  A complex64 context in PARAMENT_PRECISION_MIXED (its default) runs the
+ * batched GEMMs of the power form (tau Hnorm <= 1, M >= 5) on scaled fp16 two-way split
+ * products, with the costates formed from target^H / ||target||_F and the factor applied to
+ * the traces; its other modes use FP32 products.  The Device variants take device pointers and
+ * are asynchronous on the context's stream; they read only the real parts of carr_dev. */
 LIBSPEC Parament_ErrorCode Parament_gateFidelityGradient(Parament_Context_f32 *handle, const cuComplex *carr,
     double dt, unsigned int pts, unsigned int amps, const cuComplex *target, float *fidelity, float *grad);
 LIBSPEC Parament_ErrorCode Parament_gateFidelityGradient_fp64(Parament_Context_f64 *handle, const cuDoubleComplex *carr,
@@ -262,7 +265,9 @@ LIBSPEC size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64 *handle);
  *   out: dims[i] x dims[i] propagators.
  * Each instance computes what Parament_equiprop computes for it in PARAMENT_PRECISION_NATIVE
  * mode (FP32 throughout, FP32 iteration-count table, the power form where that mode takes it,
- * same quadrature and product order).  setExecution(0) (default) runs the whole ensemble as
+ * same quadrature and product order), except that instances above dim 64 run the power form's
+ * products and their product tree on scaled fp16 two-way splits (FP32-accurate, as the
+ * default mode's step-wise path).  setExecution(0) (default) runs the whole ensemble as
  * one captured HIP graph: the instances of dim <= 64 in multi-instance kernels per dim class
  * (16, 32, 64), the larger ones on batched GEMMs, on concurrent branches (up to four for the
  * larger ones); the tree block size and grid of each class and the GEMM tile per band of 16
@@ -284,7 +289,7 @@ LIBSPEC Parament_ErrorCode Parament_ensembleEquipropDevice(Parament_Ensemble *en
 LIBSPEC Parament_ErrorCode Parament_ensembleSynchronize(Parament_Ensemble *ensemble);
 LIBSPEC Parament_ErrorCode Parament_ensemblePeekAtLastError(Parament_Ensemble *ensemble);
 /* Gate-fidelity gradients of every instance (as Parament_gateFidelityGradient computes them
- * for it in complex64; real controls).  targets packed like out (dims[i] x dims[i] row-major
+ * for it in a complex64 context's default mode; real controls).  targets packed like out (dims[i] x dims[i] row-major
  * each), fidelity[i] = F_i (optional), grad packed like carr: amps[i] x pts[i] real values
  * dF_i/dc per instance.  setExecution(0): the instances on concurrent branches (fused kernels
  * for dims <= 32, batched GEMMs above), captured as one HIP graph per (targets, fidelity,

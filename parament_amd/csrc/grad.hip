@@ -498,20 +498,21 @@ __global__ __launch_bounds__(64 * Shape<D>::G, 2) void grad_slice_kernel(ChebArg
     }
 }
 
-// Root of the gradient sweep, one launch: blocks 0 .. nb-1 write the root costate W = t^H, block nb
-// forms g = Tr(t^H (I + e)) from the root's deviation e (FP64 partial sums, fixed reduction
-// order) into gout[0].
+// Root of the gradient sweep, one launch: blocks 0 .. nb-1 write the root costate W = t^H (times
+// wscale[0].y when wscale is given), block nb forms g = Tr(t^H (I + e)) from the root's deviation
+// e (FP64 partial sums, fixed reduction order) into gout[0].
 template <class R>
 __global__ __launch_bounds__(256) void grad_root_kernel(const cplx<R>* __restrict__ t, const cplx<R>* __restrict__ u,
                                                         int d, int nb, cplx<R>* __restrict__ wroot,
-                                                        cplx<R>* __restrict__ gout) {
+                                                        cplx<R>* __restrict__ gout, const cplx<R>* __restrict__ wscale) {
     __shared__ double sr[256], si[256];
     if ((int)blockIdx.x < nb) {
         const int e = blockIdx.x * 256 + threadIdx.x;
         if (e >= d * d) return;
         const int i = e / d, k = e % d;
         const cplx<R> v = t[k * d + i];
-        wroot[e] = mk<R>(v.x, -v.y);
+        const R f = wscale ? wscale[0].y : R(1);
+        wroot[e] = mk<R>(f * v.x, -f * v.y);
         return;
     }
     double pr = 0.0, pi = 0.0;
@@ -528,6 +529,30 @@ __global__ __launch_bounds__(256) void grad_root_kernel(const cplx<R>* __restric
         __syncthreads();
     }
     if (threadIdx.x == 0) gout[0] = mk<R>((R)sr[0], (R)si[0]);
+}
+
+// out[0] = (c, 1 / c) with c = ||t||_F (1 for t = 0), one work-group: every entry of a costate
+// formed from t^H / c by products of (near-)unitary factors is below 1 in modulus, since it is
+// at most ||t||_2 / c.  FP64 partial sums, fixed reduction order.
+template <class R>
+__global__ __launch_bounds__(256) void target_scale_kernel(const cplx<R>* __restrict__ t, int d,
+                                                           cplx<R>* __restrict__ out) {
+    __shared__ double sq[256];
+    double a = 0.0;
+    for (long e = threadIdx.x; e < (long)d * d; e += 256) {
+        const cplx<R> v = t[e];
+        a += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    sq[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sq[threadIdx.x] += sq[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double c = sq[0] > 0.0 ? sqrt(sq[0]) : 1.0;
+        out[0] = mk<R>((R)c, (R)(1.0 / c));
+    }
 }
 
 // grad[k * pts + p] = sum_j w_jp 2 Re(conj(g) dg[k][j]) / d^2 (quadrature chain rule), F = |g|^2 / d^2.
@@ -557,11 +582,12 @@ __global__ void grad_finish_kernel(const cplx<R>* __restrict__ dg, const cplx<R>
 
 // Traces of the step-wise gradient: dg[t * dg_stride + j0 + j] = Tr(Q_j O_t) for the n slices
 // of Q (one work-group per slice), from the transposed operators, so that both operands are
-// read along rows.  Eight operators per pass over Q_j.
+// read along rows, times scale[0].x when scale is given.  Eight operators per pass over Q_j.
 template <class R>
 __global__ __launch_bounds__(256) void grad_trace_kernel(const cplx<R>* __restrict__ Q, int d,
                                                          const cplx<R>* __restrict__ opsT, int ncoef,
-                                                         cplx<R>* __restrict__ dg, int dg_stride, int j0) {
+                                                         cplx<R>* __restrict__ dg, int dg_stride, int j0,
+                                                         const cplx<R>* __restrict__ scale) {
     constexpr int KB = 8;
     __shared__ R sr[KB][4], si[KB][4];
     const long dd = (long)d * d;
@@ -591,8 +617,9 @@ __global__ __launch_bounds__(256) void grad_trace_kernel(const cplx<R>* __restri
         __syncthreads();
         if (threadIdx.x < KB && t0 + (int)threadIdx.x < ncoef) {
             const int u = threadIdx.x;
+            const R f = scale ? scale[0].x : R(1);
             dg[(size_t)(t0 + u) * dg_stride + j0 + j] =
-                mk<R>((sr[u][0] + sr[u][1]) + (sr[u][2] + sr[u][3]), (si[u][0] + si[u][1]) + (si[u][2] + si[u][3]));
+                mk<R>(f * ((sr[u][0] + sr[u][1]) + (sr[u][2] + sr[u][3])), f * ((si[u][0] + si[u][1]) + (si[u][2] + si[u][3])));
         }
         __syncthreads();
     }
@@ -600,9 +627,9 @@ __global__ __launch_bounds__(256) void grad_trace_kernel(const cplx<R>* __restri
 
 template <class R>
 hipError_t launch_grad_trace(const cplx<R>* Q, int n, int d, const cplx<R>* opsT, int ncoef, cplx<R>* dg, int dg_stride,
-                             int j0, hipStream_t st) {
+                             int j0, hipStream_t st, const cplx<R>* scale) {
     if (n <= 0 || d <= 0 || ncoef <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((grad_trace_kernel<R>), dim3(n), dim3(256), 0, st, Q, d, opsT, ncoef, dg, dg_stride, j0);
+    hipLaunchKernelGGL((grad_trace_kernel<R>), dim3(n), dim3(256), 0, st, Q, d, opsT, ncoef, dg, dg_stride, j0, scale);
     return hipGetLastError();
 }
 
@@ -734,9 +761,17 @@ hipError_t launch_transpose(const cplx<R>* in, int count, int d, cplx<R>* out, h
 }
 
 template <class R>
-hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g, hipStream_t st) {
+hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g, hipStream_t st,
+                            const cplx<R>* wscale) {
     const int nb = (d * d + 255) / 256;
-    hipLaunchKernelGGL((grad_root_kernel<R>), dim3(nb + 1), dim3(256), 0, st, t, u, d, nb, wroot, g);
+    hipLaunchKernelGGL((grad_root_kernel<R>), dim3(nb + 1), dim3(256), 0, st, t, u, d, nb, wroot, g, wscale);
+    return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_target_scale(const cplx<R>* t, int d, cplx<R>* out, hipStream_t st) {
+    if (d <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((target_scale_kernel<R>), dim3(1), dim3(256), 0, st, t, d, out);
     return hipGetLastError();
 }
 
@@ -756,10 +791,12 @@ hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pt
                                             hipStream_t);                                                            \
     template hipError_t launch_grad_slices<R>(const ChebArgs<R>&, const cplx<R>*, const cplx<R>*, cplx<R>*, int,     \
                                               hipStream_t, GradDirect<R>);                                           \
-    template hipError_t launch_grad_root<R>(const cplx<R>*, const cplx<R>*, int, cplx<R>*, cplx<R>*, hipStream_t);   \
+    template hipError_t launch_grad_root<R>(const cplx<R>*, const cplx<R>*, int, cplx<R>*, cplx<R>*, hipStream_t,   \
+                                            const cplx<R>*);                                                         \
+    template hipError_t launch_target_scale<R>(const cplx<R>*, int, cplx<R>*, hipStream_t);                          \
     template hipError_t launch_transpose<R>(const cplx<R>*, int, int, cplx<R>*, hipStream_t);                        \
     template hipError_t launch_grad_trace<R>(const cplx<R>*, int, int, const cplx<R>*, int, cplx<R>*, int, int,     \
-                                             hipStream_t);                                                           \
+                                             hipStream_t, const cplx<R>*);                                           \
     template hipError_t launch_grad_finish<R>(const cplx<R>*, const cplx<R>*, int, int, int, int, int, R*, R*,       \
                                               hipStream_t);                                                          \
     template hipError_t launch_grad_finish_magnus<R>(const cplx<R>*, const cplx<R>*, const Controls<R>&, int, int,  \

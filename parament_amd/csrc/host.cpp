@@ -177,7 +177,7 @@ struct GradKey {
     const void *carr, *target, *fid, *grad, *ops, *E, *V, *W, *dg, *aux;
     double dt, hnorm;
     unsigned pts, amps, amps_set, dim, seg, chunk;
-    int quad, ncoef, M, device;
+    int quad, ncoef, M, device, precision;
     GradKey() { memset(this, 0, sizeof(*this)); }
     bool operator==(const GradKey& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
@@ -1150,6 +1150,7 @@ hipError_t grad_finish(const pa::ChebArgs<R>& ca, const cplx<R>* dg, const cplx<
 struct StepGradPlan {
     int N = 0, top = 0, chunk = 1, nwork = 5;
     int keep = 0;   // matrices per slice kept from the slices' pass for the derivative (S, Q_1 .. Q_{J-1})
+    bool split = false;   // complex64 power form: every product on scaled fp16 two-way splits
     std::vector<int> n;
     std::vector<size_t> voff, woff;   // matrix offsets of level l in V (l >= 1) and in W
     size_t vmats = 0, wmats = 0;
@@ -1204,6 +1205,12 @@ template <class R> struct StepGradBufs {
 // the fused gradient kernel).  ca carries the controls, M, s and the coefficients with
 // are[0] = a_0 - 1 (pre/pim the power coefficients, pre[0] = m_0 - 1); ca.opsT the transposed
 // control operators.
+// With p.split (complex64, power form) every product runs on scaled fp16 two-way splits, whose
+// operands must stay below 4 in modulus: the slices are assembled as Y = tau X (||Y||_2 <= 1, so
+// S, the brackets and the deviations E = U - I stay below 4 as in the forward), the costates run
+// from t^H / ||t||_F (entries of every costate below 1; the traces take the factor back), and
+// the derivative is taken in the direction W itself (dS = Y W + W Y below 2, the dQ_j below e),
+// its last level scaled by tau.
 template <class R>
 hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, const StepGradBufs<R>& b,
                                  const cplx<R>* target, R* fid, R* grad, hipStream_t st) {
@@ -1213,10 +1220,21 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
     const C zero = pa::mk<R>(0, 0), one = pa::mk<R>(1, 0);
     const double s = (double)ca.s;
     const bool w64 = sizeof(R) == 8;
-    const int mode = w64 ? pa::GEMM_F64 : pa::GEMM_F32;
     const bool power = ca.ps && M >= 5;
+    const bool split = !w64 && power && p.split;
+    const int mode = w64 ? pa::GEMM_F64 : (split ? pa::GEMM_F16X2 : pa::GEMM_F32);
     if (power && p.nwork < step_grad_work(M, true)) return hipErrorInvalidValue;
     const double tau = (double)ca.ctl.tau;
+    // Y = ty X_buf (the slice buffer), the derivative's direction tv W, its last level times fin
+    const double ty = split ? 1.0 : tau, tv = split ? 1.0 : tau, fin = split ? tau : 1.0;
+    if (split) ca.asm_scale = (R)tau;
+    const C* wscale = split ? b.gsum + 1 : nullptr;
+    // C = alpha A B + beta Cin + diag I (+ A + B) over cnt products with the given matrix strides
+    auto gm = [&](const C* A, long sa, const C* B, long sb, const C* Cin, long sc, C* out, long so, double2 alpha,
+                  double2 beta, double2 diag, int cnt, bool add_ab) {
+        return zgemm(mode, GemmOp{A, w64, sa}, GemmOp{B, w64, sb}, GemmOp{Cin, w64, sc}, out, w64, so, alpha, beta, diag, d,
+                     cnt, st, add_ab);
+    };
     double2 a[pa::kMaxCheb], m[pa::kMaxCheb];
     for (int k = 0; k <= M; ++k) {
         a[k] = c2((double)ca.are[k], (double)ca.aim[k]);
@@ -1240,7 +1258,7 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         C* kq[pa::kMaxCheb / 2 + 1];
         if (kept) keep_at(j0, kq);
         if (power)
-            r = power_batched<R>(mode, X, d, cnt, M, tau, m, true, w[0], w[1], w[2], b.E + (size_t)j0 * dd, nullptr, st, 0,
+            r = power_batched<R>(mode, X, d, cnt, M, ty, m, true, w[0], w[1], w[2], b.E + (size_t)j0 * dd, nullptr, st, 0,
                                  kept ? kq : nullptr);
         else r = clenshaw_batched<R>(X, d, cnt, M, s, a, w[0], w[1], b.E + (size_t)j0 * dd, nullptr, st);
     }
@@ -1249,11 +1267,12 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
     for (int l = 1; r == hipSuccess && l <= p.top; ++l) {
         const int n = p.n[l - 1], half = n / 2;
         C* wr = b.V + p.voff[l] * dd;
-        if (half) r = zgemm_native<R>(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, c2(1), c2(0), c2(0), d, half, st, true);
+        if (half) r = gm(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, c2(1), c2(0), c2(0), half, true);
         if (r == hipSuccess && (n & 1)) r = pa::launch_fill<R>(rd + (size_t)(n - 1) * dd, one, zero, d, 1, wr + half * dd, st);
         rd = wr;
     }
-    if (r == hipSuccess) r = pa::launch_grad_root<R>(target, rd, d, b.W + p.woff[p.top] * dd, b.gsum, st);
+    if (r == hipSuccess && split) r = pa::launch_target_scale<R>(target, d, b.gsum + 1, st);
+    if (r == hipSuccess) r = pa::launch_grad_root<R>(target, rd, d, b.W + p.woff[p.top] * dd, b.gsum, st, wscale);
     // down-sweep
     for (int l = p.top - 1; r == hipSuccess && l >= 0; --l) {
         const C* Vl = l == 0 ? b.E : b.V + p.voff[l] * dd;
@@ -1261,8 +1280,8 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         C* Wc = b.W + p.woff[l] * dd;
         const int n = p.n[l], half = n / 2;
         if (half) {
-            r = zgemm_native<R>(Wp, 1, Vl + dd, 2, Wp, 1, Wc, 2, c2(1), c2(1), c2(0), d, half, st);
-            if (r == hipSuccess) r = zgemm_native<R>(Vl, 2, Wp, 1, Wp, 1, Wc + dd, 2, c2(1), c2(1), c2(0), d, half, st);
+            r = gm(Wp, 1, Vl + dd, 2, Wp, 1, Wc, 2, c2(1), c2(1), c2(0), half, false);
+            if (r == hipSuccess) r = gm(Vl, 2, Wp, 1, Wp, 1, Wc + dd, 2, c2(1), c2(1), c2(0), half, false);
         }
         if (r == hipSuccess && (n & 1))
             r = pa::launch_fill<R>(Wp + (size_t)(n / 2) * dd, one, zero, d, 1, Wc + (size_t)(n - 1) * dd, st);
@@ -1279,10 +1298,10 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         const C* Wj = b.W + (p.woff[0] + (size_t)j0) * dd;
         if (p.chunk < N) r = pa::launch_assemble<R>(ca, X, st);   // one chunk: X is still in place
         auto gemm = [&](const C* A, const C* B, const C* Cin, C* out, double2 alpha, double2 beta, double2 diag) {
-            if (r == hipSuccess) r = zgemm_native<R>(A, 1, B, 1, Cin, 1, out, 1, alpha, beta, diag, d, cnt, st);
+            if (r == hipSuccess) r = gm(A, 1, B, 1, Cin, 1, out, 1, alpha, beta, diag, cnt, false);
         };
         if (power) {
-            // Horner's rule in S = Y^2 (Y = tau X) differentiated in the direction V = tau W:
+            // Horner's rule in S = Y^2 (Y = ty X_buf) differentiated in the direction V = tv W:
             //   dS = Y V + V Y,  Q_J = m_{2J} I + m_{2J+1} Y,  dQ_J = m_{2J+1} V,
             //   Q_j = S Q_{j+1} + m_{2j+1} Y + m_{2j} I,  dQ_j = dS Q_{j+1} + S dQ_{j+1} + m_{2j+1} V,
             // and Q = dQ_0.  Level J-1 expands Q_J and dQ_J into the epilogue terms; below it each
@@ -1303,43 +1322,43 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
                 C* kq[pa::kMaxCheb / 2 + 1];
                 keep_at(j0, kq);
                 S = kq[0];
-                gemm(X, Wj, nullptr, dS, c2(tau * tau), c2(0), c2(0));
-                gemm(Wj, X, dS, dS, c2(tau * tau), c2(1), c2(0));
-                const double2 aJ = cscale(tau, m[2 * J + 1]);
+                gemm(X, Wj, nullptr, dS, c2(ty * tv), c2(0), c2(0));
+                gemm(Wj, X, dS, dS, c2(ty * tv), c2(1), c2(0));
                 C* dq = w[3];
-                gemm2(dS, X, dS, m[2 * J], Wj, cscale(tau, m[2 * J - 1]), dq, aJ, c2(0));
-                gemm(S, Wj, dq, dq, aJ, c2(1), c2(0));
+                gemm2(dS, X, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
+                gemm(S, Wj, dq, dq, cscale(tv, m[2 * J + 1]), c2(1), c2(0));
                 for (int j = J - 2; j >= 0; --j) {
                     C* z = dq == w[3] ? w[2] : w[3];
-                    gemm(dS, kq[j + 1], Wj, z, c2(1), cscale(tau, m[2 * j + 1]), c2(0));
-                    gemm(S, dq, z, z, c2(1), c2(1), c2(0));
+                    const double f = j == 0 ? fin : 1.0;
+                    gemm(dS, kq[j + 1], Wj, z, c2(f), cscale(f * tv, m[2 * j + 1]), c2(0));
+                    gemm(S, dq, z, z, c2(f), c2(1), c2(0));
                     dq = z;
                 }
-                if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
+                if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st, wscale);
                 continue;
             }
-            gemm(X, X, nullptr, S, c2(tau * tau), c2(0), c2(0));
-            gemm(X, Wj, nullptr, dS, c2(tau * tau), c2(0), c2(0));
-            gemm(Wj, X, dS, dS, c2(tau * tau), c2(1), c2(0));
-            const double2 aJ = cscale(tau, m[2 * J + 1]);
+            gemm(X, X, nullptr, S, c2(ty * ty), c2(0), c2(0));
+            gemm(X, Wj, nullptr, dS, c2(ty * tv), c2(0), c2(0));
+            gemm(Wj, X, dS, dS, c2(ty * tv), c2(1), c2(0));
             C* q = w[2];
             C* dq = w[3];
             C* spare = w[4];
-            gemm2(S, X, S, m[2 * J], X, cscale(tau, m[2 * J - 1]), q, aJ, m[2 * J - 2]);
-            gemm2(dS, X, dS, m[2 * J], Wj, cscale(tau, m[2 * J - 1]), dq, aJ, c2(0));
-            gemm(S, Wj, dq, dq, aJ, c2(1), c2(0));
+            gemm2(S, X, S, m[2 * J], X, cscale(ty, m[2 * J - 1]), q, cscale(ty, m[2 * J + 1]), m[2 * J - 2]);
+            gemm2(dS, X, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
+            gemm(S, Wj, dq, dq, cscale(tv, m[2 * J + 1]), c2(1), c2(0));
             for (int j = J - 2; j >= 0; --j) {
                 C* z = j == 0 ? X : spare;   // X is dead once no Q_j with j >= 1 is left to form
-                gemm(dS, q, Wj, z, c2(1), cscale(tau, m[2 * j + 1]), c2(0));
-                gemm(S, dq, z, z, c2(1), c2(1), c2(0));
+                const double f = j == 0 ? fin : 1.0;
+                gemm(dS, q, Wj, z, c2(f), cscale(f * tv, m[2 * j + 1]), c2(0));
+                gemm(S, dq, z, z, c2(f), c2(1), c2(0));
                 if (j >= 1) {
-                    gemm(S, q, X, dq, c2(1), cscale(tau, m[2 * j + 1]), m[2 * j]);
+                    gemm(S, q, X, dq, c2(1), cscale(ty, m[2 * j + 1]), m[2 * j]);
                     spare = q;
                     q = dq;
                 }
                 dq = z;
             }
-            if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st);
+            if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st, wscale);
             continue;
         }
         gemm(X, X, X, w[0], cscale(s * s, a[M]), cscale(s, a[M - 1]), c2(a[M - 2].x - a[M].x, a[M - 2].y - a[M].y));
@@ -1501,6 +1520,9 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     StepGradPlan sp = step_grad_plan(stepwise ? N : 1, d, sizeof(C), h->ckpt_chunk, step_grad_work(M, ca.ps != 0));
     // the slices' S and Q_j kept for the derivative within 8 GiB (checkpointing asks for less memory: not kept)
     if (stepwise && !h->ckpt_seg) sp.keep = step_grad_keep(M, ca.ps != 0, N, d, sizeof(C), (size_t)8 << 30);
+    // complex64 in the default mode: the power form's products, the sweeps and the derivative on
+    // the scaled fp16 split, as the forward's step-wise path
+    sp.split = sizeof(R) == 4 && ca.ps && M >= 5 && h->precision == PARAMENT_PRECISION_MIXED && mixed_split_scheme() != 0;
     const size_t mats_vw = p.chunk_mats + p.top_mats;
     const size_t bytes_e =
         stepwise ? ((size_t)N * (1 + sp.keep) + (size_t)sp.nwork * sp.chunk) * dd * sizeof(C) : (size_t)p.chunk * dd * sizeof(C);
@@ -1541,7 +1563,7 @@ Parament_ErrorCode gradient(Context<R>* h, const cplx<R>* carr_dev, double dt, u
     key.E = E; key.V = V; key.W = W; key.dg = dg; key.aux = gsum;
     key.dt = dt; key.hnorm = h->hnorm; key.pts = pts; key.amps = amps; key.amps_set = h->amps; key.dim = h->dim;
     key.seg = h->ckpt_seg; key.chunk = h->ckpt_chunk; key.quad = h->quad; key.ncoef = h->ncoef; key.M = M;
-    key.device = h->device;
+    key.device = h->device; key.precision = h->precision;
     if (h->grad_exec && key == h->grad_key)
         return hipGraphLaunch(h->grad_exec, h->stream) == hipSuccess ? PARAMENT_STATUS_SUCCESS : PARAMENT_STATUS_CUBLAS_FAILED;
     if (!(key == h->grad_prev)) {   // first call with these arguments: launch directly
@@ -1910,9 +1932,12 @@ hipError_t ens_enqueue_class(Ensemble* e, int c, const EnsCfg& cf, hipStream_t s
     return r;
 }
 
-// One large instance: the step-wise path on batched FP32 matrix-core GEMMs (as Context<float>,
-// power form included).  n_cap > 0 (the tuner): only the first n_cap slices, which is enough to
-// rank the GEMM tiles.
+// One large instance: the step-wise path on batched matrix-core GEMMs.  Where the power form
+// applies (tau Hnorm <= 1) its products and the tree's run on scaled fp16 two-way splits, as in a
+// complex64 context's default mode: the slices are assembled as Y = tau X (entries below 1), and
+// S = Y^2, the Horner brackets and the deviations E = U - I stay below 4 in modulus.  Otherwise
+// the recurrence and the tree run on FP32 products.  n_cap > 0 (the tuner): only the first n_cap
+// slices, which is enough to rank the GEMM tiles.
 hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) {
     const EnsInst& x = e->in[i];
     const int d = (int)x.d, N = n_cap > 0 ? std::min(x.N, n_cap) : x.N, M = x.M;
@@ -1930,14 +1955,17 @@ hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) 
     double2 ak[pa::kMaxCheb];
     for (int k = 0; k <= M; ++k) ak[k] = c2(k == 0 ? x.are[0] - 1.0 : (double)ca.are[k], (double)ca.aim[k]);
     float2* leaves = nullptr;
+    const bool power = stepwise_power(M, x.tau * x.hnorm);
+    const int mode = power ? pa::GEMM_F16X2 : pa::GEMM_F32;
+    if (power) ca.asm_scale = (float)x.tau;
     hipError_t r = pa::launch_assemble<float>(ca, X, st);
-    if (stepwise_power(M, x.tau * x.hnorm)) {
+    if (power) {
         double mr[pa::kMaxCheb], mi[pa::kMaxCheb];
         double2 mc[pa::kMaxCheb];
         power_coefficients(M, x.tau * x.hnorm, x.are, x.aim, mr, mi);
         for (int k = 0; k <= M; ++k) mc[k] = c2(mr[k], mi[k]);
         if (r == hipSuccess)
-            r = power_batched<float>(pa::GEMM_F32, X, d, N, M, x.tau, mc, true, b1, b2, b3, nullptr, &leaves, st, x.tile);
+            r = power_batched<float>(mode, X, d, N, M, 1.0, mc, true, b1, b2, b3, nullptr, &leaves, st, x.tile);
     } else if (r == hipSuccess) {
         r = clenshaw_batched<float>(X, d, N, M, (double)ca.s, ak, b1, b2, nullptr, &leaves, st, x.tile);
     }
@@ -1947,7 +1975,8 @@ hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) 
     int n = N;
     while (r == hipSuccess && n > 1) {
         const int half = n / 2;
-        r = zgemm_native<float>(rd + dd, 2, rd, 2, nullptr, 0, wr, 1, c2(1), c2(0), c2(0), d, half, st, true, x.tile);
+        r = zgemm(mode, GemmOp{rd + dd, false, 2}, GemmOp{rd, false, 2}, GemmOp{nullptr, false, 0}, wr, false, 1, c2(1),
+                  c2(0), c2(0), d, half, st, true, x.tile);
         if (r == hipSuccess && n % 2)
             r = pa::launch_fill<float>(rd + (size_t)(n - 1) * dd, make_float2(1, 0), make_float2(0, 0), d, 1, wr + half * dd, st);
         n = half + (n % 2);
@@ -2133,6 +2162,8 @@ EnsGradLayout ens_grad_layout(const EnsInst& x) {
         L.sp = step_grad_plan(x.N, (int)x.d, sizeof(float2), cap, nwork);
         // the slices' S and Q_j kept for the derivative within 4 GiB per instance
         L.sp.keep = step_grad_keep(x.M, power_form_ok(x.M, x.tau * x.hnorm), x.N, (int)x.d, sizeof(float2), (size_t)4 << 30);
+        // the power form on the scaled fp16 split, as the forward's large instances
+        L.sp.split = x.M >= 5 && power_form_ok(x.M, x.tau * x.hnorm);
         sz[0] = ((size_t)x.N * (1 + L.sp.keep) + (size_t)nwork * L.sp.chunk) * dd;
         sz[1] = std::max<size_t>(L.sp.vmats, 1) * dd;
         sz[2] = L.sp.wmats * dd;

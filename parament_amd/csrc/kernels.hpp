@@ -161,12 +161,16 @@ template <class R> hipError_t launch_grad_slices(const ChebArgs<R>& a, const cpl
                                                  GradDirect<R> gd = GradDirect<R>());
 // out[t] = in[t]^T for t < count d x d matrices
 template <class R> hipError_t launch_transpose(const cplx<R>* in, int count, int d, cplx<R>* out, hipStream_t st);
-// dg[t * dg_stride + j0 + j] = Tr(Q_j O_t), O_t^T = opsT[t], for the n matrices Q_j (any d)
+// dg[t * dg_stride + j0 + j] = Tr(Q_j O_t), O_t^T = opsT[t], for the n matrices Q_j (any d), times
+// scale[0].x when scale (a device value) is given
 template <class R> hipError_t launch_grad_trace(const cplx<R>* Q, int n, int d, const cplx<R>* opsT, int ncoef,
-                                                cplx<R>* dg, int dg_stride, int j0, hipStream_t st);
-// root costate wroot = t^H and g = Tr(t^H u), one launch
+                                                cplx<R>* dg, int dg_stride, int j0, hipStream_t st,
+                                                const cplx<R>* scale = nullptr);
+// root costate wroot = t^H (times wscale[0].y when given) and g = Tr(t^H u), one launch
 template <class R> hipError_t launch_grad_root(const cplx<R>* t, const cplx<R>* u, int d, cplx<R>* wroot, cplx<R>* g,
-                                               hipStream_t st);
+                                               hipStream_t st, const cplx<R>* wscale = nullptr);
+// out[0] = (c, 1 / c), c = ||t||_F (1 for t = 0): the costate scale of the split-product gradient
+template <class R> hipError_t launch_target_scale(const cplx<R>* t, int d, cplx<R>* out, hipStream_t st);
 template <class R> hipError_t launch_grad_finish(const cplx<R>* dg, const cplx<R>* g, int N, int pts, int amps,
                                                  int mode, int d, R* grad, R* fid, hipStream_t st);
 // the chain rule of the Magnus coefficients (commutator terms included) to the raw samples

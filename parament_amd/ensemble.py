@@ -2,9 +2,10 @@
 
 No counterpart in the reference package: many systems of different dimension, control count
 and length solved together.  Each instance gets what ``Parament('fp32').equiprop`` would return
-for it (same iteration-count table, quadrature and product order); the ensemble runs as one
-captured HIP graph with multi-instance kernels for the small dims, batched GEMMs for dims above
-64, concurrent branches and tuned tile shapes.
+for it (same iteration-count table, quadrature and product order; FP32 products up to dim 64,
+FP32-accurate fp16 split products above, as the default mode's step-wise path); the ensemble
+runs as one captured HIP graph with multi-instance kernels for the small dims, batched GEMMs for
+dims above 64, concurrent branches and tuned tile shapes.
 """
 from __future__ import annotations
 

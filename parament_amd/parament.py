@@ -254,9 +254,12 @@ class Parament:
         'native': the context's own precision throughout (the reference's arithmetic; default of
         'fp64' contexts).  'mixed': FP64 for the norm, the Chebyshev coefficients, the slice
         Hamiltonians, the first-order terms and the upper product-tree levels, FP32-accurate
-        bf16 three-way split products for the rest (default of 'fp32' contexts, which keep the
-        wavefront-resident FP32 kernels for dims <= 16).  'mixed_fp32' and 'mixed_bf16x2': the
-        same with FP32 or bf16 hi+lo products."""
+        split products for the rest: scaled fp16 two-way splits where the slice polynomial is
+        evaluated in powers of tau X (tau Hnorm <= 1), bf16 three-way splits for the recurrence
+        (default of 'fp32' contexts, which keep the wavefront-resident FP32 kernels for dims
+        <= 16).  'mixed_fp32' and 'mixed_bf16x2': the same with FP32 or bf16 hi+lo products.
+        gate_fidelity_gradient of an 'fp32' context above dim 32 runs its products on the fp16
+        splits in 'mixed' (power form) and on FP32 products in the other modes."""
         code = _PRECISION.get(mode)
         if code is None:
             raise ValueError("mode must be 'native', 'mixed', 'mixed_fp32' or 'mixed_bf16x2'")

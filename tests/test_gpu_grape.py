@@ -104,6 +104,30 @@ def test_complex64_stepwise_gradient_matches_block_expm_oracle(d, dt):
     assert err < 1e-4, err
 
 
+def test_complex64_stepwise_gradient_target_scale():
+    """The default complex64 step-wise gradient runs its products on fp16 splits and its costates
+    from t^H / ||t||_F: a target scaled by 3 gives 9 times the gradient, a zero target a zero
+    gradient, and the FP32-product mode (native) the same gradient."""
+    import parament_amd as parament
+    h0, hk, c, t = _problem(4150, 48, 2, 33)
+    f_ref, grad_ref = go.fidelity_gradient(h0, hk, c, 0.05, "none", t)
+    res = {}
+    for mode in ("mixed", "native"):
+        with parament.Parament("fp32") as p:
+            p.set_hamiltonian(h0, *hk)
+            p.set_precision_mode(mode)
+            assert p.precision_mode == mode
+            res[mode] = [p.gate_fidelity_gradient(0.05, *c, target=x) for x in (t, 3.0 * t, np.zeros_like(t))]
+    scale = np.abs(grad_ref).max()
+    for mode, ((f1, g1), (f3, g3), (f0, g0)) in res.items():
+        assert abs(f1 - f_ref) < 1e-5 * max(f_ref, 1e-3), mode
+        assert np.abs(g1 - grad_ref).max() < 1e-4 * scale, mode
+        assert abs(f3 - 9 * f1) < 1e-5 * 9 * max(f1, 1e-3), mode
+        assert np.abs(g3 - 9 * g1).max() < 1e-4 * 9 * scale, mode
+        assert f0 == 0.0 and np.all(g0 == 0.0), mode
+    assert np.abs(res["mixed"][0][1] - res["native"][0][1]).max() < 1e-4 * scale
+
+
 @pytest.mark.parametrize("pts,mode", [(1, "none"), (2, "none"), (3, "simpson"), (5, "midpoint"), (33, "none")])
 def test_stepwise_gradient_few_slices(pts, mode):
     """One slice (no tree), two, and odd counts that carry a node up a level."""
