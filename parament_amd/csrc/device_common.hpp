@@ -178,13 +178,15 @@ template <class R> struct Pad;
 template <> struct Pad<float> { static constexpr int A = 2, B = 16; };
 template <> struct Pad<double> { static constexpr int A = 1, B = 8; };
 
-// Work split of one d x d product: D = 64 and D = 32 use 4 waves, each owning one quadrant
-// (32x32 or 16x16); D = 16 uses a single wave owning the whole matrix.
+// Work split of one D x D product (D = 16, 32, 48, 64): (D/16)^2 waves, each owning one 16x16
+// output tile, row-major over the tile grid (D = 16: a single wave).
 template <int D> struct Shape {
-    static constexpr int G = (D == 16) ? 1 : (D == 32 ? 4 : 16);
+    static_assert(D % 16 == 0 && D >= 16 && D <= 64, "16x16 tiles");
+    static constexpr int W = D / 16;
+    static constexpr int G = W * W;
     static constexpr int T = 1;
-    static __device__ __forceinline__ int row0(int w) { return D == 16 ? 0 : (D == 32 ? 16 * (w >> 1) : 16 * (w >> 2)); }
-    static __device__ __forceinline__ int col0(int w) { return D == 16 ? 0 : (D == 32 ? 16 * (w & 1) : 16 * (w & 3)); }
+    static __device__ __forceinline__ int row0(int w) { return 16 * (w / W); }
+    static __device__ __forceinline__ int col0(int w) { return 16 * (w % W); }
 };
 
 }  // namespace pa

@@ -64,11 +64,13 @@ hipError_t launch_cheb_fused(const ChebArgs<R>& a, hipStream_t st) {
     if (a.ps) {
         if (a.d <= 16) hipLaunchKernelGGL((power_fused_kernel<R, 16>), dim3(grid < 32768 ? grid : 32768), dim3(64 * Shape<16>::G), 0, st, a);
         else if (a.d <= 32) hipLaunchKernelGGL((power_fused_kernel<R, 32>), dim3(grid), dim3(64 * Shape<32>::G), 0, st, a);
+        else if (a.d <= 48) hipLaunchKernelGGL((power_fused_kernel<R, 48>), dim3(grid), dim3(64 * Shape<48>::G), 0, st, a);
         else hipLaunchKernelGGL((power_fused_kernel<R, 64>), dim3(grid), dim3(64 * Shape<64>::G), 0, st, a);
         return hipGetLastError();
     }
     if (a.d <= 16) hipLaunchKernelGGL((cheb_fused_kernel<R, 16>), dim3(grid < 32768 ? grid : 32768), dim3(64 * Shape<16>::G), 0, st, a);
     else if (a.d <= 32) hipLaunchKernelGGL((cheb_fused_kernel<R, 32>), dim3(grid), dim3(64 * Shape<32>::G), 0, st, a);
+    else if (a.d <= 48) hipLaunchKernelGGL((cheb_fused_kernel<R, 48>), dim3(grid), dim3(64 * Shape<48>::G), 0, st, a);
     else hipLaunchKernelGGL((cheb_fused_kernel<R, 64>), dim3(grid), dim3(64 * Shape<64>::G), 0, st, a);
     return hipGetLastError();
 }
@@ -466,10 +468,12 @@ hipError_t launch_tree(const void* leaves, bool leaves_are_c64, int N, int d, cp
         if (split) {
             if (d <= 16) launch_tree_split<R, 16>(src, n, d, out, fin, st);
             else if (d <= 32) launch_tree_split<R, 32>(src, n, d, out, fin, st);
+            else if (d <= 48) launch_tree_split<R, 48>(src, n, d, out, fin, st);
             else launch_tree_split<R, 64>(src, n, d, out, fin, st);
         } else {
             if (d <= 16) launch_tree_level<R, 16>(src, n, d, out, fin, st);
             else if (d <= 32) launch_tree_level<R, 32>(src, n, d, out, fin, st);
+            else if (d <= 48) launch_tree_level<R, 48>(src, n, d, out, fin, st);
             else launch_tree_level<R, 64>(src, n, d, out, fin, st);
         }
         return n_out;

@@ -247,6 +247,20 @@ def test_mixed_fused_tree_levels(d, n):
     assert rel_err(u, ref) < 1e-5
 
 
+@pytest.mark.parametrize("prec", ["fp32n", "fp64"])
+@pytest.mark.parametrize("d,n,dt", [(40, 2051, 0.02), (48, 1030, 0.02), (33, 1029, 0.12), (48, 4099, 0.01)])
+def test_fused_48_class_tree_levels(prec, d, n, dt):
+    """Dims 33-48 run the 48-wide fused slice kernels (nine waves of 16 x 16 tiles) and the
+    48-wide tree: eight-node block levels while more than 1,024 nodes remain, then pairwise
+    levels split over work-groups.  Complex64 NATIVE takes the power form for the small steps;
+    complex128, and dt = 0.12 (tau Hnorm = 1.14), the recurrence.  (At tau Hnorm = 2.8 plain
+    complex64 arithmetic, numpy's included, lands 4.6e-5 from the oracle, beyond the bar.)"""
+    h0, hks, c = case(1150 + n + d, d, 2, n)
+    u, m = run_gpu(prec, h0, hks, c, dt, "none")
+    ref = oracle_for(prec, h0, hks, c, dt, "none")
+    assert rel_err(u, ref) < TOL[prec], (m,)
+
+
 @pytest.mark.parametrize("d,n", [(16, 40), (64, 300), (64, 1100)])
 def test_mixed_large_step_keeps_recurrence(d, n):
     """tau Hnorm > 1 selects the Clenshaw recurrence of the mixed kernel (small steps use the
