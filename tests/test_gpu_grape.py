@@ -163,14 +163,18 @@ def test_magnus_gradient_matches_central_differences(d):
         _fd_check(p, c, dt, t, grad, [(0, 0), (1, 1), (2, 2), (0, 7), (1, 10), (2, 13), (0, 19), (1, 20), (2, 20)])
 
 
-def test_complex64_magnus_gradient_matches_fp64():
+@pytest.mark.parametrize("d,dt", [(12, 0.05), (40, 0.05), (40, 0.01)])
+def test_complex64_magnus_gradient_matches_fp64(d, dt):
+    """Magnus gradients of a complex64 context against the complex128 ones: fused (d = 12), and
+    on the batched GEMMs with the recurrence (tau Hnorm = 3.0) and with the power form on fp16
+    split products (tau Hnorm = 0.61, M = 7)."""
     import parament_amd as parament
-    h0, hk, c, t = _problem(4600, 12, 3, 33)
+    h0, hk, c, t = _problem(4600, d, 3, 33)
     out = {}
     for prec in ("fp64", "fp32"):
         with parament.Parament(prec) as p:
             p.set_hamiltonian(h0, *hk, use_magnus=True, quadrature_mode="simpson")
-            out[prec] = p.gate_fidelity_gradient(0.05, *c, target=t)
+            out[prec] = p.gate_fidelity_gradient(dt, *c, target=t)
     (f64, g64), (f32, g32) = out["fp64"], out["fp32"]
     assert abs(f32 - f64) < 1e-5 * max(f64, 1e-3)
     assert np.abs(g32 - g64).max() < 1e-4 * np.abs(g64).max()
