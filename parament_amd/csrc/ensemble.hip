@@ -1,5 +1,5 @@
 // Multi-instance kernels of the ensemble solver: many independent equiprop problems of
-// different dims, controls and lengths, each dim class (16 / 32 / 64 after padding) served by
+// different dims, controls and lengths, each dim class (16 / 32 / 48 / 64 after padding) served by
 // one persistent kernel that walks a task table.  A slice task (instance i, slice j) runs the
 // fused Chebyshev slice body (parament.cpp:569-652) with instance i's operators, controls and
 // coefficients; a tree task (level descriptor l, block b) reduces 2^LOGB nodes of one
@@ -74,7 +74,8 @@ hipError_t tree_launch(const EnsLevel* lv, const int2* tasks, int ntasks, int gr
 }  // namespace
 
 bool ens_shape_valid(int D, int T) {
-    return (D == 16 && T == 1) || (D == 32 && (T == 1 || T == 2)) || (D == 64 && (T == 1 || T == 2));
+    return (D == 16 && T == 1) || (D == 32 && (T == 1 || T == 2)) || (D == 48 && T == 1) ||
+           (D == 64 && (T == 1 || T == 2));
 }
 
 hipError_t launch_ens_slices(int D, int T, const EnsInstance* inst, const int2* tasks, int ntasks, int grid, hipStream_t st) {
@@ -82,6 +83,7 @@ hipError_t launch_ens_slices(int D, int T, const EnsInstance* inst, const int2* 
     if (grid <= 0 || !ens_shape_valid(D, T)) return hipErrorInvalidValue;
     if (D == 16) return slice_launch<16, 1>(inst, tasks, ntasks, grid, st);
     if (D == 32) return T == 1 ? slice_launch<32, 1>(inst, tasks, ntasks, grid, st) : slice_launch<32, 2>(inst, tasks, ntasks, grid, st);
+    if (D == 48) return slice_launch<48, 1>(inst, tasks, ntasks, grid, st);
     return T == 1 ? slice_launch<64, 1>(inst, tasks, ntasks, grid, st) : slice_launch<64, 2>(inst, tasks, ntasks, grid, st);
 }
 
@@ -97,6 +99,7 @@ hipError_t launch_ens_tree(int D, int T, int logb, const EnsLevel* lv, const int
         if (T == 1) { PA_ENS_TREE(32, 1) }
         PA_ENS_TREE(32, 2)
     }
+    if (D == 48) { PA_ENS_TREE(48, 1) }
     if (T == 1) { PA_ENS_TREE(64, 1) }
     PA_ENS_TREE(64, 2)
 #undef PA_ENS_TREE

@@ -1671,15 +1671,17 @@ template <class R> Parament_ErrorCode kernel_times(Context<R>* h, double* a, dou
     return h->last_error = PARAMENT_STATUS_SUCCESS;
 }
 // ----- ensemble of independent complex64 problems (ensemble.hip + gemm.hip) -----
-// Instances of dim <= 64 are padded to a dim class (16, 32, 64) and run by the multi-instance
+// Instances of dim <= 64 are padded to a dim class (16, 32, 48, 64) and run by the multi-instance
 // kernels, one slice kernel and one tree-level kernel per class and level; larger instances run
 // the step-wise path on batched GEMMs.  The classes and the large instances go on separate
 // branches (streams forked from and joined to the ensemble's stream), the whole solve is
 // captured once as a hipGraph and replayed.  The tuner picks, per class, the tree block size and
 // the persistent grid, and per band of 16 dims the output tile of the large instances' GEMMs, by
 // timing each candidate on the ensemble itself; large instances run on up to four branches.
-constexpr int kEnsSide = 8;        // side streams: 3 classes + up to 5 large-instance branches
-constexpr int kEnsClasses = 3;     // dim classes 16, 32, 64
+constexpr int kEnsSide = 8;        // side streams: 4 classes + up to 4 large-instance branches
+constexpr int kEnsClasses = 4;     // dim classes 16, 32, 48, 64
+constexpr int kEnsClassDim[kEnsClasses] = {16, 32, 48, 64};
+inline int ens_class_of(unsigned d) { return d <= 16 ? 0 : (d <= 32 ? 1 : (d <= 48 ? 2 : (d <= 64 ? 3 : -1))); }
 
 // nodes of one tree scratch buffer of an ensemble instance with N slices
 size_t ens_tree_scratch(int N) {
@@ -1846,7 +1848,7 @@ Parament_ErrorCode ens_set_hamiltonians(Ensemble* e, const cuComplex* H) {
         x.M = select_fp32(x.hnorm, x.tau);
         if (x.M < 3) return PARAMENT_STATUS_SELECT_SMALLER_DT;
         bessel(x.M, x.tau * x.hnorm, x.are, x.aim);
-        x.cls = x.d <= 16 ? 0 : (x.d <= 32 ? 1 : (x.d <= 64 ? 2 : -1));
+        x.cls = ens_class_of(x.d);
         // working memory: classes: leaves N + two tree scratch buffers (the multi-instance tree
         // writes ceil(N / 4) nodes, launch_tree tree_scratch_nodes(N)); large: X, b1, b2
         x.work_off = e->n_work;
@@ -1874,7 +1876,7 @@ Parament_ErrorCode ens_set_hamiltonians(Ensemble* e, const cuComplex* H) {
     if (!e->d_inst) return PARAMENT_STATUS_DEVICE_ALLOC_FAILED;
     for (int c = 0; c < kEnsClasses; ++c) {
         EnsClassPlan& P = e->cls[c];
-        P.D = 16 << c;
+        P.D = kEnsClassDim[c];
         P.n_slice = (int)slices[c].size();
         if (P.n_slice) {
             P.d_slice = ens_upload(e, slices[c]);
@@ -2060,7 +2062,7 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
     if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
     for (int c = 0; c < kEnsClasses; ++c) {
         if (!e->cls[c].n_slice) continue;
-        const int D = 16 << c;
+        const int D = kEnsClassDim[c];
         // the tree block size on the default grid first, then the persistent grid size for the
         // better of those (the grid size has not changed the block size chosen).  One wave split
         // (T = 1): the split of a slice over two waves' tiles was slower in every class whenever
@@ -2532,8 +2534,11 @@ Parament_ErrorCode Parament_ensembleTune(Parament_Ensemble* ensemble) {
 Parament_ErrorCode Parament_ensembleGetTuning(Parament_Ensemble* ensemble, int* config) {
     if (!ensemble || !config) return PARAMENT_STATUS_INVALID_VALUE;
     auto* e = reinterpret_cast<Ensemble*>(ensemble);
-    for (int c = 0; c < kEnsClasses; ++c) {
-        config[3 * c] = e->cfg[c].T; config[3 * c + 1] = e->cfg[c].logb; config[3 * c + 2] = e->cfg[c].grid_mult;
+    // classes 16, 32 and 64 (the 48 class is tuned like the others but not reported)
+    const int reported[3] = {0, 1, 3};
+    for (int q = 0; q < 3; ++q) {
+        const EnsCfg& cf = e->cfg[reported[q]];
+        config[3 * q] = cf.T; config[3 * q + 1] = cf.logb; config[3 * q + 2] = cf.grid_mult;
     }
     config[9] = e->branches;
     return PARAMENT_STATUS_SUCCESS;
