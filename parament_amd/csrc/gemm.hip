@@ -225,7 +225,7 @@ __global__ __launch_bounds__(64 * NW) void zgemm_row_kernel(GemmArgs g, EpiIO<TA
 // lane l holds A[l%16][8(l/16) .. +7] and Bt[l%16][8(l/16) .. +7]; accumulator element v is
 // C[4(l/16) + v][l%16].  F16: the scaled fp16 two-way split (NS = 2), else bf16 slices.  EDGE:
 // d is not a multiple of BT, so loads and stores past d are masked (otherwise none is).
-template <int NS, int BT, bool F16, bool EDGE, class TA, class TB, class TC, class TO>
+template <int NS, int BT, bool F16, bool EDGE, bool DUAL, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
     typedef std::conditional_t<F16, _Float16, __bf16> H;
     typedef std::conditional_t<F16, f16x8g, bf16x8> H8;
@@ -243,6 +243,12 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
     const long dd = (long)d * d;
     const TA* A = io.A + b * g.sa * dd;
     const TB* B = io.B + b * g.sb * dd;
+    // DUAL: a second product g.A2 g.B2 accumulated by a second pass of the K loop (its own
+    // instantiation, so that the single-product kernels keep their registers); KA, KB: the pair
+    // the K loop reads (A and B stay the epilogue's)
+    const TA* KA = A;
+    const TB* KB = B;
+    constexpr int npass = DUAL ? 2 : 1;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wr = (BT / 2) * (wave >> 1), wc = (BT / 2) * (wave & 1);
     const int r16 = lane & 15, kq = lane >> 4;
@@ -289,11 +295,14 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
     // per fetch (KT along A's rows, KT rows of B)
     const TA* pa[NP];
     const TB* pb[NP];
+    auto init_ptrs = [&]() {
 #pragma unroll
-    for (int u = 0; u < NP; ++u) {
-        pa[u] = A + (long)(ti * BT + a_row(u)) * d + 2 * a_kp(u);
-        pb[u] = B + (long)(2 * b_kp(u)) * d + tj * BT + b_col(u);
-    }
+        for (int u = 0; u < NP; ++u) {
+            pa[u] = KA + (long)(ti * BT + a_row(u)) * d + 2 * a_kp(u);
+            pb[u] = KB + (long)(2 * b_kp(u)) * d + tj * BT + b_col(u);
+        }
+    };
+    init_ptrs();
     auto fetch = [&](int k0) {
         if constexpr (!EDGE) {
 #pragma unroll
@@ -311,7 +320,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 ra[u][h].x = 0; ra[u][h].y = 0;
-                if (gi < d && gk + h < d) ra[u][h] = A[(long)gi * d + gk + h];
+                if (gi < d && gk + h < d) ra[u][h] = KA[(long)gi * d + gk + h];
             }
         }
 #pragma unroll
@@ -320,10 +329,17 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 rb[u][h].x = 0; rb[u][h].y = 0;
-                if (gk + h < d && gj < d) rb[u][h] = B[(long)(gk + h) * d + gj];
+                if (gk + h < d && gj < d) rb[u][h] = KB[(long)(gk + h) * d + gj];
             }
         }
     };
+#pragma unroll 1
+    for (int pass = 0; pass < npass; ++pass) {
+    if (DUAL && pass) {
+        KA = static_cast<const TA*>(g.A2) + b * g.sa2 * dd;
+        KB = static_cast<const TB*>(g.B2) + b * g.sb2 * dd;
+        init_ptrs();
+    }
     fetch(0);
     for (int k0 = 0; k0 < d; k0 += KT) {
 #pragma unroll
@@ -373,6 +389,7 @@ __global__ __launch_bounds__(256) void zgemm_split_kernel(GemmArgs g, EpiIO<TA, 
             }
         __syncthreads();
     }
+    }   // pass
     if constexpr (F16 && sizeof(TC) == 8 && sizeof(TO) == 8) {
         // the fp16 split serves the power form (complex64 in and out): its epilogue in FP32, the
         // 2^-28 of the products folded into alpha (each term rounded once more than in FP64,
@@ -444,12 +461,15 @@ hipError_t launch_tiled(const GemmArgs& g, int mode, const EpiIO<TA, TB, TC, TO>
         case GEMM_F64: PA_FP(double) break;
         case GEMM_F32: PA_FP(float) break;
 #undef PA_FP
-#define PA_SPLIT(NS, F16)                                                                                              \
-    if (g.d % BT) hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, true, TA, TB, TC, TO>), grid, block, 0, st, g, io); \
-    else hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, false, TA, TB, TC, TO>), grid, block, 0, st, g, io);
-        case GEMM_BF16X2: PA_SPLIT(2, false) break;
-        case GEMM_BF16X3: PA_SPLIT(3, false) break;
-        case GEMM_F16X2: PA_SPLIT(2, true) break;
+#define PA_SPLIT(NS, F16, DU)                                                                                          \
+    if (g.d % BT) hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, true, DU, TA, TB, TC, TO>), grid, block, 0, st, g, io); \
+    else hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, false, DU, TA, TB, TC, TO>), grid, block, 0, st, g, io);
+        case GEMM_BF16X2: PA_SPLIT(2, false, false) break;
+        case GEMM_BF16X3: PA_SPLIT(3, false, false) break;
+        case GEMM_F16X2:
+            if (g.A2) { PA_SPLIT(2, true, true) }
+            else { PA_SPLIT(2, true, false) }
+            break;
 #undef PA_SPLIT
         default: return hipErrorInvalidValue;
     }
@@ -496,6 +516,7 @@ hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
 // low-precision tree levels) and complex128 ones; the FP32 and split modes run on complex64.
 hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st) {
     if (g.d <= 0 || g.batch < 0) return hipErrorInvalidValue;
+    if ((g.A2 || g.B2) && (!g.A2 || !g.B2 || mode != GEMM_F16X2 || g.add_ab)) return hipErrorInvalidValue;
     if (mode == GEMM_F64) {
         if (!g.out_c128) return hipErrorInvalidValue;
         if (g.a_c128 && g.b_c128) return launch_typed<double2, double2, double2, double2>(g, mode, st);

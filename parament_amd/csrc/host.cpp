@@ -325,6 +325,19 @@ hipError_t zgemm_native(const cplx<R>* A, long sa, const cplx<R>* B, long sb, co
     return zgemm(w ? pa::GEMM_F64 : pa::GEMM_F32, GemmOp{A, w, sa}, GemmOp{B, w, sb}, GemmOp{Cin, w, sc}, C, w, so, alpha,
                  beta, diag, d, batch, st, add_ab, tile);
 }
+// C = alpha (A B + A2 B2) + beta Cin + beta2 Cin2 + diag I: two products summed in one launch
+// (the fp16 split mode; complex64, unit matrix strides over the batch).
+inline hipError_t zgemm_sum(int mode, const float2* A, const float2* B, const float2* A2, const float2* B2,
+                            const float2* Cin, double2 beta, const float2* Cin2, double2 beta2, float2* C, double2 alpha,
+                            double2 diag, int d, int batch, hipStream_t st) {
+    pa::GemmArgs g;
+    g.A = A; g.sa = 1; g.B = B; g.sb = 1; g.A2 = A2; g.sa2 = 1; g.B2 = B2; g.sb2 = 1;
+    g.Cin = Cin; g.sc = Cin ? 1 : 0; g.Cin2 = Cin2; g.sc2 = Cin2 ? 1 : 0; g.C = C; g.so = 1;
+    g.alpha_re = alpha.x; g.alpha_im = alpha.y; g.beta_re = beta.x; g.beta_im = beta.y;
+    g.beta2_re = beta2.x; g.beta2_im = beta2.y; g.diag_re = diag.x; g.diag_im = diag.y;
+    g.d = d; g.batch = batch; g.add_ab = 0; g.a_c128 = g.b_c128 = g.out_c128 = 0;
+    return pa::launch_zgemm(g, mode, st);
+}
 inline double2 c2(double re, double im = 0.0) { return pa::mk<double>(re, im); }
 inline double2 cmul2(double2 x, double2 y) { return c2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x); }
 inline double2 cscale(double f, double2 x) { return c2(f * x.x, f * x.y); }
@@ -1314,6 +1327,14 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
                     r = zgemm(mode, op(A), op(B), op(Cin), out, w64, 1, alpha, beta, diag, d, cnt, st, false, 0, op(Cin2),
                               beta2);
             };
+            // split only: alpha (A B + A2 B2) + beta Cin + beta2 Cin2 + diag I in one launch
+            auto gsum = [&](const C* A, const C* B, const C* A2, const C* B2, const C* Cin, double2 beta, const C* Cin2,
+                            double2 beta2, C* out, double2 alpha, double2 diag) {
+                if (r != hipSuccess) return;
+                if constexpr (sizeof(R) == 4)
+                    r = zgemm_sum(mode, A, B, A2, B2, Cin, beta, Cin2, beta2, out, alpha, diag, d, cnt, st);
+                else r = hipErrorInvalidValue;
+            };
             const int J = (M - 1) / 2;
             C* S = w[0];
             C* dS = w[1];
@@ -1322,9 +1343,22 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
                 C* kq[pa::kMaxCheb / 2 + 1];
                 keep_at(j0, kq);
                 S = kq[0];
+                C* dq = w[3];
+                if (split) {
+                    // each pair of products that share an output in one launch (ty = tv = 1)
+                    gsum(X, Wj, Wj, X, nullptr, c2(0), nullptr, c2(0), dS, c2(1), c2(0));
+                    gsum(dS, X, S, Wj, dS, m[2 * J], Wj, m[2 * J - 1], dq, m[2 * J + 1], c2(0));
+                    for (int j = J - 2; j >= 0; --j) {
+                        C* z = dq == w[3] ? w[2] : w[3];
+                        const double f = j == 0 ? fin : 1.0;
+                        gsum(dS, kq[j + 1], S, dq, Wj, cscale(f, m[2 * j + 1]), nullptr, c2(0), z, c2(f), c2(0));
+                        dq = z;
+                    }
+                    if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st, wscale);
+                    continue;
+                }
                 gemm(X, Wj, nullptr, dS, c2(ty * tv), c2(0), c2(0));
                 gemm(Wj, X, dS, dS, c2(ty * tv), c2(1), c2(0));
-                C* dq = w[3];
                 gemm2(dS, X, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
                 gemm(S, Wj, dq, dq, cscale(tv, m[2 * J + 1]), c2(1), c2(0));
                 for (int j = J - 2; j >= 0; --j) {
@@ -1338,19 +1372,28 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
                 continue;
             }
             gemm(X, X, nullptr, S, c2(ty * ty), c2(0), c2(0));
-            gemm(X, Wj, nullptr, dS, c2(ty * tv), c2(0), c2(0));
-            gemm(Wj, X, dS, dS, c2(ty * tv), c2(1), c2(0));
+            if (split) gsum(X, Wj, Wj, X, nullptr, c2(0), nullptr, c2(0), dS, c2(1), c2(0));
+            else {
+                gemm(X, Wj, nullptr, dS, c2(ty * tv), c2(0), c2(0));
+                gemm(Wj, X, dS, dS, c2(ty * tv), c2(1), c2(0));
+            }
             C* q = w[2];
             C* dq = w[3];
             C* spare = w[4];
             gemm2(S, X, S, m[2 * J], X, cscale(ty, m[2 * J - 1]), q, cscale(ty, m[2 * J + 1]), m[2 * J - 2]);
-            gemm2(dS, X, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
-            gemm(S, Wj, dq, dq, cscale(tv, m[2 * J + 1]), c2(1), c2(0));
+            if (split) gsum(dS, X, S, Wj, dS, m[2 * J], Wj, m[2 * J - 1], dq, m[2 * J + 1], c2(0));
+            else {
+                gemm2(dS, X, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
+                gemm(S, Wj, dq, dq, cscale(tv, m[2 * J + 1]), c2(1), c2(0));
+            }
             for (int j = J - 2; j >= 0; --j) {
                 C* z = j == 0 ? X : spare;   // X is dead once no Q_j with j >= 1 is left to form
                 const double f = j == 0 ? fin : 1.0;
-                gemm(dS, q, Wj, z, c2(f), cscale(f * tv, m[2 * j + 1]), c2(0));
-                gemm(S, dq, z, z, c2(f), c2(1), c2(0));
+                if (split) gsum(dS, q, S, dq, Wj, cscale(f, m[2 * j + 1]), nullptr, c2(0), z, c2(f), c2(0));
+                else {
+                    gemm(dS, q, Wj, z, c2(f), cscale(f * tv, m[2 * j + 1]), c2(0));
+                    gemm(S, dq, z, z, c2(f), c2(1), c2(0));
+                }
                 if (j >= 1) {
                     gemm(S, q, X, dq, c2(1), cscale(ty, m[2 * j + 1]), m[2 * j]);
                     spare = q;

@@ -97,6 +97,10 @@ struct GemmArgs {
     double alpha_re, alpha_im, beta_re, beta_im, diag_re, diag_im;
     const void* Cin2 = nullptr; long sc2 = 0;
     double beta2_re = 0.0, beta2_im = 0.0;
+    // a second product added before the epilogue, C = alpha (A B + A2 B2) + ..., complex64
+    // operands, the fp16 split mode only (A2 and B2 both set, no add_ab)
+    const void* A2 = nullptr; long sa2 = 0;
+    const void* B2 = nullptr; long sb2 = 0;
     int d, batch, add_ab;
     int a_c128, b_c128, out_c128;
     int tile = 0;          // output tile 32 or 64, 16 = whole product per work-group (FP32, dims 33-128); 0 = automatic
@@ -104,7 +108,7 @@ struct GemmArgs {
 hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st);
 
 // Ensemble kernels (ensemble.hip): per-instance parameters in a device table, task tables
-// of (instance, slice) and (level descriptor, block); complex64.  Dim classes D = 16, 32, 64
+// of (instance, slice) and (level descriptor, block); complex64.  Dim classes D = 16, 32, 48, 64
 // (instances padded up), wave split T (TileShape<D, T>), tree block size 2^logb (2 or 3).
 struct EnsInstance {
     const float2* ops;     // (1 + ncoef) d x d: H0, H_1..
