@@ -270,8 +270,8 @@ LIBSPEC size_t Parament_getGradientWorkspace_fp64(Parament_Context_f64 *handle);
  * default mode's step-wise path).  setExecution(0) (default) runs the whole ensemble as
  * one captured HIP graph: the instances of dim <= 64 in multi-instance kernels per dim class
  * (16, 32, 48, 64), the larger ones on batched GEMMs, on concurrent branches (up to four for the
- * larger ones); the tree block size and grid of each class and the GEMM tile per band of 16
- * dims are tuned on first use (or by ensembleTune).  setExecution(1) runs one instance
+ * larger ones); the tree block size and grid of each class are tuned on first use (or by
+ * ensembleTune).  setExecution(1) runs one instance
  * after the other (the per-instance loop).  getTuning fills config[10]: (T, log2 tree block,
  * grid multiple) for classes 16, 32, 64, then the number of branches (the 48 class is tuned
  * alike and not reported). */

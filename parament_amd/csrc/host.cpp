@@ -11,7 +11,6 @@
 
 #include <algorithm>
 #include <functional>
-#include <map>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -1735,7 +1734,7 @@ struct EnsInst {
     unsigned d = 0, amps = 0, pts = 0;
     double dt = 0, tau = 0, hnorm = 0;
     int N = 0, M = 0, cls = -1;                // cls: class index, -1 for the batched-GEMM path
-    int tile = 0;                               // batched-GEMM tile (tuned: 64, 32 or 16 = whole product; 0 = automatic)
+    int tile = 0;                               // batched-GEMM tile (0 = automatic)
     size_t ops_off = 0, c_off = 0, out_off = 0; // element offsets in the packed arrays
     size_t work_off = 0;                        // element offset of its working memory
     size_t opsT_off = 0;                        // element offset of its transposed controls (gradient)
@@ -1981,11 +1980,10 @@ hipError_t ens_enqueue_class(Ensemble* e, int c, const EnsCfg& cf, hipStream_t s
 // applies (tau Hnorm <= 1) its products and the tree's run on scaled fp16 two-way splits, as in a
 // complex64 context's default mode: the slices are assembled as Y = tau X (entries below 1), and
 // S = Y^2, the Horner brackets and the deviations E = U - I stay below 4 in modulus.  Otherwise
-// the recurrence and the tree run on FP32 products.  n_cap > 0 (the tuner): only the first n_cap
-// slices, which is enough to rank the GEMM tiles.
-hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st, int n_cap = 0) {
+// the recurrence and the tree run on FP32 products.
+hipError_t ens_enqueue_large(Ensemble* e, int i, hipStream_t st) {
     const EnsInst& x = e->in[i];
-    const int d = (int)x.d, N = n_cap > 0 ? std::min(x.N, n_cap) : x.N, M = x.M;
+    const int d = (int)x.d, N = x.N, M = x.M;
     const size_t dd = (size_t)d * d;
     float2* X = e->d_work + x.work_off;
     float2* b1 = X + (size_t)N * dd;
@@ -2126,28 +2124,11 @@ Parament_ErrorCode ens_tune(Ensemble* e) {
             trial(cf);
         }
     }
-    // output tile of the large instances' batched GEMMs.  The best tile follows the dimension:
-    // instances are grouped by dim in bands of 16, the band's instance with the most slices is
-    // timed with each tile, and the whole band takes the fastest.
-    std::map<unsigned, int> rep_of_band;   // band -> instance timed
-    for (int i : e->large) {
-        const unsigned band = (e->in[i].d + 15) / 16;
-        const auto hit = rep_of_band.find(band);
-        if (hit == rep_of_band.end() || e->in[i].N > e->in[hit->second].N) rep_of_band[band] = i;
-    }
-    std::map<unsigned, int> tile_of_band;
-    for (const auto& br : rep_of_band) {
-        const int i = br.second;
-        float best = 1e30f;
-        int best_tile = 64;
-        for (int tile : {64, 32, 16}) {   // 16: the whole-product kernel
-            e->in[i].tile = tile;
-            const float t = ens_time(e, [&] { return ens_enqueue_large(e, i, e->stream, 512); }, 1);
-            if (t < best) { best = t; best_tile = tile; }
-        }
-        tile_of_band[br.first] = best_tile;
-    }
-    for (int i : e->large) e->in[i].tile = tile_of_band[(e->in[i].d + 15) / 16];
+    // the large instances' batched GEMMs take the automatic output tile (the smaller padded
+    // product volume).  Timing each tile on the first 512 slices of one instance per band of 16
+    // dims chose tiles that made the 512-system ensemble's 106 large systems 10% slower (179
+    // against 162 ms as a graph) once their products ran on the fp16 split.
+    for (int i : e->large) e->in[i].tile = 0;
     // large-instance branches: four (or one per large instance when fewer).  Timing the branch
     // counts costs whole solves of the large instances, and four has been fastest wherever it
     // was timed (512-system ensemble: 1, 2, 4 branches)

@@ -5,7 +5,7 @@ and length solved together.  Each instance gets what ``Parament('fp32').equiprop
 for it (same iteration-count table, quadrature and product order; FP32 products up to dim 64,
 FP32-accurate fp16 split products above, as the default mode's step-wise path); the ensemble
 runs as one captured HIP graph with multi-instance kernels for the small dims, batched GEMMs for
-dims above 64, concurrent branches and tuned tile shapes.
+dims above 64, concurrent branches and tuned tree block sizes and grids.
 """
 from __future__ import annotations
 
