@@ -94,7 +94,7 @@ __device__ __forceinline__ void epi_store(const GemmArgs& g, const EpiIO<TA, TB,
 // FP64 / FP32 matrix-core tiles: A chunk [64][32] (stride SA), B chunk [32][64] (stride SB).
 // The next K chunk is fetched into registers while the matrix cores work on the current one.
 // EDGE: d is not a multiple of BT, so loads and stores past d are masked (otherwise none is).
-template <class R, int BT, bool EDGE, class TA, class TB, class TC, class TO>
+template <class R, int BT, bool EDGE, bool DUAL, class TA, class TB, class TC, class TO>
 __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB, TC, TO> io) {
     constexpr int TW = BT / 32;   // 16x16 blocks per wave side (4 waves, 2 x 2)
     constexpr int SA = KT + Pad<R>::A, SB = BT + Pad<R>::B;
@@ -109,6 +109,10 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
     const long dd = (long)d * d;
     const TA* A = io.A + b * g.sa * dd;
     const TB* B = io.B + b * g.sb * dd;
+    // DUAL: a second product g.A2 g.B2 accumulated by a second pass of the K loop (its own
+    // instantiation); KA, KB: the pair the K loop reads (A and B stay the epilogue's)
+    const TA* KA = A;
+    const TB* KB = B;
     const int tid = threadIdx.x, wave = tid >> 6;
     const int wr = (BT / 2) * (wave >> 1), wc = (BT / 2) * (wave & 1);
     TA ra[NE];
@@ -119,18 +123,24 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
             const int e = tid + 256 * u;
             const int gi = ti * BT + e / KT, gk = k0 + e % KT;
             ra[u].x = 0; ra[u].y = 0;
-            if (!EDGE || (gi < d && gk < d)) ra[u] = A[(long)gi * d + gk];
+            if (!EDGE || (gi < d && gk < d)) ra[u] = KA[(long)gi * d + gk];
         }
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int e = tid + 256 * u;
             const int gk = k0 + e / BT, gj = tj * BT + e % BT;
             rb[u].x = 0; rb[u].y = 0;
-            if (!EDGE || (gk < d && gj < d)) rb[u] = B[(long)gk * d + gj];
+            if (!EDGE || (gk < d && gj < d)) rb[u] = KB[(long)gk * d + gj];
         }
     };
     CTile<R, TW, TW> acc;
     acc.zero();
+#pragma unroll 1
+    for (int pass = 0; pass < (DUAL ? 2 : 1); ++pass) {
+    if (DUAL && pass) {
+        KA = static_cast<const TA*>(g.A2) + b * g.sa2 * dd;
+        KB = static_cast<const TB*>(g.B2) + b * g.sb2 * dd;
+    }
     fetch(0);
     for (int k0 = 0; k0 < d; k0 += KT) {
 #pragma unroll
@@ -145,6 +155,7 @@ __global__ __launch_bounds__(256) void zgemm_fp_kernel(GemmArgs g, EpiIO<TA, TB,
         wave_cgemm<R, TW, TW, KT, SA, SB, false>(Ar, Ai, Br, Bi, wr, wc, acc, EDGE ? min(KT, (d - k0 + 3) & ~3) : KT);
         __syncthreads();
     }
+    }   // pass
     // epilogue in FP64
 #pragma unroll
     for (int m = 0; m < TW; ++m)
@@ -455,12 +466,14 @@ hipError_t launch_tiled(const GemmArgs& g, int mode, const EpiIO<TA, TB, TC, TO>
     if (total > 0x7fffffffL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)total), block(256);
     switch (mode) {
-#define PA_FP(R)                                                                                                   \
-    if (g.d % BT) hipLaunchKernelGGL((zgemm_fp_kernel<R, BT, true, TA, TB, TC, TO>), grid, block, 0, st, g, io);   \
-    else hipLaunchKernelGGL((zgemm_fp_kernel<R, BT, false, TA, TB, TC, TO>), grid, block, 0, st, g, io);
+#define PA_FP2(R, DU)                                                                                                   \
+    if (g.d % BT) hipLaunchKernelGGL((zgemm_fp_kernel<R, BT, true, DU, TA, TB, TC, TO>), grid, block, 0, st, g, io);   \
+    else hipLaunchKernelGGL((zgemm_fp_kernel<R, BT, false, DU, TA, TB, TC, TO>), grid, block, 0, st, g, io);
+#define PA_FP(R) if (g.A2) { PA_FP2(R, true) } else { PA_FP2(R, false) }
         case GEMM_F64: PA_FP(double) break;
         case GEMM_F32: PA_FP(float) break;
 #undef PA_FP
+#undef PA_FP2
 #define PA_SPLIT(NS, F16, DU)                                                                                          \
     if (g.d % BT) hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, true, DU, TA, TB, TC, TO>), grid, block, 0, st, g, io); \
     else hipLaunchKernelGGL((zgemm_split_kernel<NS, BT, F16, false, DU, TA, TB, TC, TO>), grid, block, 0, st, g, io);
@@ -503,7 +516,7 @@ hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
     // 97-112: 5-6%; at 49-96 and 113-128 it was level or slower)
     if constexpr (sizeof(TA) == 8 && sizeof(TB) == 8 && sizeof(TO) == 8) {
         const bool row_default = (g.d > 32 && g.d <= 48) || (g.d > 96 && g.d <= 112);
-        if (mode == GEMM_F32 && g.d > 32 && g.d <= 128 && (g.tile == 16 || (g.tile == 0 && row_default)))
+        if (mode == GEMM_F32 && !g.A2 && g.d > 32 && g.d <= 128 && (g.tile == 16 || (g.tile == 0 && row_default)))
             return launch_row(g, io, st);
     }
     return gemm_tile(g) == 32 ? launch_tiled<32>(g, mode, io, st) : launch_tiled<64>(g, mode, io, st);
@@ -516,7 +529,8 @@ hipError_t launch_typed(const GemmArgs& g, int mode, hipStream_t st) {
 // low-precision tree levels) and complex128 ones; the FP32 and split modes run on complex64.
 hipError_t launch_zgemm(const GemmArgs& g, int mode, hipStream_t st) {
     if (g.d <= 0 || g.batch < 0) return hipErrorInvalidValue;
-    if ((g.A2 || g.B2) && (!g.A2 || !g.B2 || mode != GEMM_F16X2 || g.add_ab)) return hipErrorInvalidValue;
+    if ((g.A2 || g.B2) && (!g.A2 || !g.B2 || mode == GEMM_BF16X2 || mode == GEMM_BF16X3 || g.add_ab))
+        return hipErrorInvalidValue;
     if (mode == GEMM_F64) {
         if (!g.out_c128) return hipErrorInvalidValue;
         if (g.a_c128 && g.b_c128) return launch_typed<double2, double2, double2, double2>(g, mode, st);

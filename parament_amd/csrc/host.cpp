@@ -325,16 +325,18 @@ hipError_t zgemm_native(const cplx<R>* A, long sa, const cplx<R>* B, long sb, co
                  beta, diag, d, batch, st, add_ab, tile);
 }
 // C = alpha (A B + A2 B2) + beta Cin + beta2 Cin2 + diag I: two products summed in one launch
-// (the fp16 split mode; complex64, unit matrix strides over the batch).
-inline hipError_t zgemm_sum(int mode, const float2* A, const float2* B, const float2* A2, const float2* B2,
-                            const float2* Cin, double2 beta, const float2* Cin2, double2 beta2, float2* C, double2 alpha,
-                            double2 diag, int d, int batch, hipStream_t st) {
+// (FP64, FP32 and fp16 split modes; the context's element type, unit matrix strides over the batch).
+template <class R>
+hipError_t zgemm_sum(int mode, const cplx<R>* A, const cplx<R>* B, const cplx<R>* A2, const cplx<R>* B2,
+                     const cplx<R>* Cin, double2 beta, const cplx<R>* Cin2, double2 beta2, cplx<R>* C, double2 alpha,
+                     double2 diag, int d, int batch, hipStream_t st) {
+    const int w = sizeof(R) == 8;
     pa::GemmArgs g;
     g.A = A; g.sa = 1; g.B = B; g.sb = 1; g.A2 = A2; g.sa2 = 1; g.B2 = B2; g.sb2 = 1;
     g.Cin = Cin; g.sc = Cin ? 1 : 0; g.Cin2 = Cin2; g.sc2 = Cin2 ? 1 : 0; g.C = C; g.so = 1;
     g.alpha_re = alpha.x; g.alpha_im = alpha.y; g.beta_re = beta.x; g.beta_im = beta.y;
     g.beta2_re = beta2.x; g.beta2_im = beta2.y; g.diag_re = diag.x; g.diag_im = diag.y;
-    g.d = d; g.batch = batch; g.add_ab = 0; g.a_c128 = g.b_c128 = g.out_c128 = 0;
+    g.d = d; g.batch = batch; g.add_ab = 0; g.a_c128 = g.b_c128 = g.out_c128 = w;
     return pa::launch_zgemm(g, mode, st);
 }
 inline double2 c2(double re, double im = 0.0) { return pa::mk<double>(re, im); }
@@ -1312,6 +1314,12 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         auto gemm = [&](const C* A, const C* B, const C* Cin, C* out, double2 alpha, double2 beta, double2 diag) {
             if (r == hipSuccess) r = gm(A, 1, B, 1, Cin, 1, out, 1, alpha, beta, diag, cnt, false);
         };
+        // alpha (A B + A2 B2) + beta Cin + beta2 Cin2 + diag I: the two products that share an output in
+        // one launch, accumulated before one rounding
+        auto gsum = [&](const C* A, const C* B, const C* A2, const C* B2, const C* Cin, double2 beta, const C* Cin2,
+                        double2 beta2, C* out, double2 alpha, double2 diag) {
+            if (r == hipSuccess) r = zgemm_sum<R>(mode, A, B, A2, B2, Cin, beta, Cin2, beta2, out, alpha, diag, d, cnt, st);
+        };
         if (power) {
             // Horner's rule in S = Y^2 (Y = ty X_buf) differentiated in the direction V = tv W:
             //   dS = Y V + V Y,  Q_J = m_{2J} I + m_{2J+1} Y,  dQ_J = m_{2J+1} V,
@@ -1326,14 +1334,6 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
                     r = zgemm(mode, op(A), op(B), op(Cin), out, w64, 1, alpha, beta, diag, d, cnt, st, false, 0, op(Cin2),
                               beta2);
             };
-            // split only: alpha (A B + A2 B2) + beta Cin + beta2 Cin2 + diag I in one launch
-            auto gsum = [&](const C* A, const C* B, const C* A2, const C* B2, const C* Cin, double2 beta, const C* Cin2,
-                            double2 beta2, C* out, double2 alpha, double2 diag) {
-                if (r != hipSuccess) return;
-                if constexpr (sizeof(R) == 4)
-                    r = zgemm_sum(mode, A, B, A2, B2, Cin, beta, Cin2, beta2, out, alpha, diag, d, cnt, st);
-                else r = hipErrorInvalidValue;
-            };
             const int J = (M - 1) / 2;
             C* S = w[0];
             C* dS = w[1];
@@ -1343,56 +1343,28 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
                 keep_at(j0, kq);
                 S = kq[0];
                 C* dq = w[3];
-                if (split) {
-                    // each pair of products that share an output in one launch (ty = tv = 1)
-                    gsum(X, Wj, Wj, X, nullptr, c2(0), nullptr, c2(0), dS, c2(1), c2(0));
-                    gsum(dS, X, S, Wj, dS, m[2 * J], Wj, m[2 * J - 1], dq, m[2 * J + 1], c2(0));
-                    for (int j = J - 2; j >= 0; --j) {
-                        C* z = dq == w[3] ? w[2] : w[3];
-                        const double f = j == 0 ? fin : 1.0;
-                        gsum(dS, kq[j + 1], S, dq, Wj, cscale(f, m[2 * j + 1]), nullptr, c2(0), z, c2(f), c2(0));
-                        dq = z;
-                    }
-                    if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st, wscale);
-                    continue;
-                }
-                gemm(X, Wj, nullptr, dS, c2(ty * tv), c2(0), c2(0));
-                gemm(Wj, X, dS, dS, c2(ty * tv), c2(1), c2(0));
-                gemm2(dS, X, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
-                gemm(S, Wj, dq, dq, cscale(tv, m[2 * J + 1]), c2(1), c2(0));
+                gsum(X, Wj, Wj, X, nullptr, c2(0), nullptr, c2(0), dS, c2(ty * tv), c2(0));
+                gsum(dS, X, S, Wj, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
                 for (int j = J - 2; j >= 0; --j) {
                     C* z = dq == w[3] ? w[2] : w[3];
                     const double f = j == 0 ? fin : 1.0;
-                    gemm(dS, kq[j + 1], Wj, z, c2(f), cscale(f * tv, m[2 * j + 1]), c2(0));
-                    gemm(S, dq, z, z, c2(f), c2(1), c2(0));
+                    gsum(dS, kq[j + 1], S, dq, Wj, cscale(f * tv, m[2 * j + 1]), nullptr, c2(0), z, c2(f), c2(0));
                     dq = z;
                 }
                 if (r == hipSuccess) r = pa::launch_grad_trace<R>(dq, cnt, d, ca.opsT, ca.ncoef, b.dg, N, j0, st, wscale);
                 continue;
             }
             gemm(X, X, nullptr, S, c2(ty * ty), c2(0), c2(0));
-            if (split) gsum(X, Wj, Wj, X, nullptr, c2(0), nullptr, c2(0), dS, c2(1), c2(0));
-            else {
-                gemm(X, Wj, nullptr, dS, c2(ty * tv), c2(0), c2(0));
-                gemm(Wj, X, dS, dS, c2(ty * tv), c2(1), c2(0));
-            }
+            gsum(X, Wj, Wj, X, nullptr, c2(0), nullptr, c2(0), dS, c2(ty * tv), c2(0));
             C* q = w[2];
             C* dq = w[3];
             C* spare = w[4];
             gemm2(S, X, S, m[2 * J], X, cscale(ty, m[2 * J - 1]), q, cscale(ty, m[2 * J + 1]), m[2 * J - 2]);
-            if (split) gsum(dS, X, S, Wj, dS, m[2 * J], Wj, m[2 * J - 1], dq, m[2 * J + 1], c2(0));
-            else {
-                gemm2(dS, X, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
-                gemm(S, Wj, dq, dq, cscale(tv, m[2 * J + 1]), c2(1), c2(0));
-            }
+            gsum(dS, X, S, Wj, dS, m[2 * J], Wj, cscale(tv, m[2 * J - 1]), dq, cscale(ty, m[2 * J + 1]), c2(0));
             for (int j = J - 2; j >= 0; --j) {
                 C* z = j == 0 ? X : spare;   // X is dead once no Q_j with j >= 1 is left to form
                 const double f = j == 0 ? fin : 1.0;
-                if (split) gsum(dS, q, S, dq, Wj, cscale(f, m[2 * j + 1]), nullptr, c2(0), z, c2(f), c2(0));
-                else {
-                    gemm(dS, q, Wj, z, c2(f), cscale(f * tv, m[2 * j + 1]), c2(0));
-                    gemm(S, dq, z, z, c2(f), c2(1), c2(0));
-                }
+                gsum(dS, q, S, dq, Wj, cscale(f * tv, m[2 * j + 1]), nullptr, c2(0), z, c2(f), c2(0));
                 if (j >= 1) {
                     gemm(S, q, X, dq, c2(1), cscale(ty, m[2 * j + 1]), m[2 * j]);
                     spare = q;
@@ -1404,10 +1376,8 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
             continue;
         }
         gemm(X, X, X, w[0], cscale(s * s, a[M]), cscale(s, a[M - 1]), c2(a[M - 2].x - a[M].x, a[M - 2].y - a[M].y));
-        gemm(X, Wj, Wj, w[2], cscale(s * s, a[M]), cscale(s, a[M - 1]), c2(0));
-        gemm(Wj, X, w[2], w[2], cscale(s * s, a[M]), c2(1), c2(0));
-        gemm(X, w[2], Wj, w[3], c2(s), cscale(beta3 * s, a[M]), c2(0));
-        gemm(Wj, w[0], w[3], w[3], c2(s), c2(1), c2(0));
+        gsum(X, Wj, Wj, X, Wj, cscale(s, a[M - 1]), nullptr, c2(0), w[2], cscale(s * s, a[M]), c2(0));
+        gsum(X, w[2], Wj, w[0], Wj, cscale(beta3 * s, a[M]), nullptr, c2(0), w[3], c2(s), c2(0));
         if (k3 > 0)
             gemm(X, w[0], X, w[1], c2(s), cscale(beta3 * s, a[M]),
                  c2(a[k3].x + beta3 * a[M - 1].x, a[k3].y + beta3 * a[M - 1].y));
@@ -1417,8 +1387,7 @@ hipError_t grad_stepwise_enqueue(pa::ChebArgs<R> ca, const StepGradPlan& p, cons
         C* d1 = w[3];
         C* d2 = w[2];
         for (int k = M - 4; k >= 0; --k) {
-            gemm(X, d1, d2, d2, c2(s), c2(k == 0 ? -2.0 : -1.0), c2(0));
-            gemm(Wj, b1, d2, d2, c2(s), c2(1), c2(0));
+            gsum(X, d1, Wj, b1, d2, c2(k == 0 ? -2.0 : -1.0), nullptr, c2(0), d2, c2(s), c2(0));
             if (k > 0) gemm(X, b1, b2, b2, c2(s), c2(-1.0), a[k]);
             std::swap(b1, b2);
             std::swap(d1, d2);

@@ -97,8 +97,8 @@ struct GemmArgs {
     double alpha_re, alpha_im, beta_re, beta_im, diag_re, diag_im;
     const void* Cin2 = nullptr; long sc2 = 0;
     double beta2_re = 0.0, beta2_im = 0.0;
-    // a second product added before the epilogue, C = alpha (A B + A2 B2) + ..., complex64
-    // operands, the fp16 split mode only (A2 and B2 both set, no add_ab)
+    // a second product added before the epilogue, C = alpha (A B + A2 B2) + ..., operands of A's
+    // and B's types; FP64, FP32 and fp16 split modes (A2 and B2 both set, no add_ab)
     const void* A2 = nullptr; long sa2 = 0;
     const void* B2 = nullptr; long sb2 = 0;
     int d, batch, add_ab;
