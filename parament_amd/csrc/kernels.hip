@@ -500,7 +500,7 @@ template hipError_t launch_tree<double>(const void*, bool, int, int, double2*, d
 // (one read per slice had the operators streamed from the cache hierarchy once per slice,
 // (1 + ncoef) x 16 bytes per element of every slice).  Per slice the sum keeps its order.
 // Dynamic LDS: the group's coefficients, kAsmSlices x ncoef.
-constexpr int kAsmSlices = 8;
+constexpr int kAsmSlices = 16;
 
 template <class R>
 __device__ __forceinline__ void asm_group_coefficients(const ChebArgs<R>& a, int j0, int ns, cplx<R>* coef,
